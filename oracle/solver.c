@@ -1,0 +1,199 @@
+/* Oracle solver drivers.  TEST INFRASTRUCTURE ONLY.
+ * Restates solver/NSSolver.cpp:50-124, solver/NSTempTurbSolver.cpp:94-248,
+ * solver/SolverController.cpp:26-43,229-307, field/FieldController.cpp:101-185
+ * and one iteration of TimeIntegration.cpp:86-149. */
+#include "oracle_internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct or_sim {
+    const or_domain *D;
+    or_sim_params p;
+    or_mg *mg;
+    size_t n;
+    double *u, *v, *w, *u0, *v0, *w0, *ut, *vt, *wt;
+    double *p_, *rhs, *T, *T0, *Tt, *Ta, *nu_t, *kappa_t, *fx, *fy, *fz, *S_T, *spatial;
+    int counters[5];
+};
+
+static double *alloc0(size_t n) { return (double *)calloc(n, sizeof(double)); }
+static void cp(double *dst, const double *src, size_t n) { memcpy(dst, src, n * sizeof(double)); }
+
+static int has_dir(const int *dir, int a) { return dir[a] != 0; }
+
+/* FieldController::couple_vector / couple_scalar, FieldController.cpp:151-186 */
+static void couple_vector(or_sim *s) {
+    cp(s->u0, s->u, s->n); cp(s->ut, s->u, s->n);
+    cp(s->v0, s->v, s->n); cp(s->vt, s->v, s->n);
+    cp(s->w0, s->w, s->n); cp(s->wt, s->w, s->n);
+}
+static void couple_scalar(or_sim *s) {
+    cp(s->T0, s->T, s->n);
+    cp(s->Tt, s->T, s->n);
+}
+
+static int diffuse(or_sim *s, double *out, double *in, const double *b, double Dc, const double *ev, int ft) {
+    const or_sim_params *p = &s->p;
+    if (p->diff_type == 0)
+        return or_jacobi_diffuse(s->D, out, in, b, Dc, ev, p->dt, p->diff_max_iter, p->diff_tol, p->diff_w, ft, NULL);
+    return or_cgs_diffuse(s->D, out, in, b, Dc, ev, p->dt, p->diff_max_iter, p->diff_tol, p->diff_w, ft, NULL);
+}
+
+/* SolverController::update_sources, SolverController.cpp:263-307 */
+static void update_sources(or_sim *s, double t) {
+    const or_sim_params *p = &s->p;
+    if (p->has_force) {
+        double *f[3] = {s->fx, s->fy, s->fz};
+        for (int a = 0; a < 3; a++)
+            if (has_dir(p->force_dir, a)) or_buoyancy(s->D, f[a], s->T, s->Ta, p->beta, p->g);
+    }
+    if (p->solver == 1 && p->has_temp_source) { /* GaussFunction::update_source, GaussFunction.cpp:27-40 */
+        double tv = tanh(t / p->tau);
+        cp(s->S_T, s->spatial, s->n);
+        for (size_t i = 0; i < s->n; i++) s->S_T[i] *= tv;
+    }
+}
+
+/* FieldController::update_data, FieldController.cpp:119-135 */
+static void update_data(or_sim *s) {
+    cp(s->u0, s->u, s->n); cp(s->v0, s->v, s->n); cp(s->w0, s->w, s->n);
+    cp(s->ut, s->u, s->n); cp(s->vt, s->v, s->n); cp(s->wt, s->w, s->n);
+    cp(s->T0, s->T, s->n); cp(s->Tt, s->T, s->n);
+}
+
+or_sim *or_sim_create(const or_domain *D, const or_sim_params *p, const double *u_ic, const double *v_ic,
+                      const double *w_ic, const double *T_ic) {
+    or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
+    s->D = D;
+    s->p = *p;
+    s->n = D->size[0];
+    const size_t n = s->n;
+    double **all[] = {&s->u, &s->v, &s->w, &s->u0, &s->v0, &s->w0, &s->ut, &s->vt, &s->wt, &s->p_, &s->rhs,
+                      &s->T, &s->T0, &s->Tt, &s->Ta, &s->nu_t, &s->kappa_t, &s->fx, &s->fy, &s->fz, &s->S_T,
+                      &s->spatial};
+    for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) *all[i] = alloc0(n);
+    s->mg = or_mg_create(D, &p->mg);
+    /* set_up_fields: initial conditions on the inner list, SolverController.cpp:95-205 */
+    const ivec *inner = &D->inner[0];
+    for (size_t l = 0; l < inner->n; l++) {
+        size_t i = inner->d[l];
+        if (u_ic) s->u[i] = u_ic[i];
+        if (v_ic) s->v[i] = v_ic[i];
+        if (w_ic) s->w[i] = w_ic[i];
+        if (T_ic) s->T[i] = T_ic[i];
+    }
+    if (p->solver == 1 && p->has_force) /* force_source: T_ambient.set_value, :229-236 */
+        for (size_t i = 0; i < n; i++) s->Ta[i] = p->T_ambient;
+    if (p->solver == 1 && p->has_temp_source) or_gauss_spatial(D, s->spatial, p->gauss);
+    /* FieldController::set_up_boundary, FieldController.cpp:101-112 */
+    or_apply_bc(D, s->u, OR_U, 0);
+    or_apply_bc(D, s->v, OR_V, 0);
+    or_apply_bc(D, s->w, OR_W, 0);
+    or_apply_bc(D, s->p_, OR_P, 0);
+    or_apply_bc(D, s->T, OR_T, 0);
+    if (p->solver == 1) or_apply_bc(D, s->Ta, OR_T, 0);
+    else { /* T_ambient of a solver without temperature is never read */ }
+    update_sources(s, 0);
+    update_data(s);
+    return s;
+}
+
+void or_sim_free(or_sim *s) {
+    if (!s) return;
+    double *all[] = {s->u, s->v, s->w, s->u0, s->v0, s->w0, s->ut, s->vt, s->wt, s->p_, s->rhs,
+                     s->T, s->T0, s->Tt, s->Ta, s->nu_t, s->kappa_t, s->fx, s->fy, s->fz, s->S_T, s->spatial};
+    for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) free(all[i]);
+    or_mg_free(s->mg);
+    free(s);
+}
+
+static void do_step_ns(or_sim *s, double t) {
+    const or_domain *D = s->D;
+    const or_sim_params *p = &s->p;
+    or_sl_advect(D, s->u, s->u0, s->u0, s->v0, s->w0, p->dt, OR_U);
+    or_sl_advect(D, s->v, s->v0, s->u0, s->v0, s->w0, p->dt, OR_V);
+    or_sl_advect(D, s->w, s->w0, s->u0, s->v0, s->w0, p->dt, OR_W);
+    couple_vector(s);
+    if (p->nu != 0.) {
+        s->counters[0] = diffuse(s, s->u, s->u0, s->ut, p->nu, NULL, OR_U);
+        s->counters[1] = diffuse(s, s->v, s->v0, s->vt, p->nu, NULL, OR_V);
+        s->counters[2] = diffuse(s, s->w, s->w0, s->wt, p->nu, NULL, OR_W);
+        couple_vector(s);
+    }
+    if (p->has_force) { /* ExplicitEulerSource vector add_source, ExplicitEulerSource.cpp:25-76 */
+        if (has_dir(p->force_dir, 0)) or_add_source(D, s->u, s->fx, p->dt, OR_U);
+        if (has_dir(p->force_dir, 1)) or_add_source(D, s->v, s->fy, p->dt, OR_V);
+        if (has_dir(p->force_dir, 2)) or_add_source(D, s->w, s->fz, p->dt, OR_W);
+        couple_vector(s);
+    }
+    or_divergence(D, s->rhs, s->ut, s->vt, s->wt);
+    s->counters[4] = or_mg_pressure(s->mg, s->p_, s->rhs, t, p->dt);
+    or_projection(D, s->u, s->v, s->w, s->ut, s->vt, s->wt, s->p_);
+}
+
+static void do_step_nstempturb(or_sim *s, double t) {
+    const or_domain *D = s->D;
+    const or_sim_params *p = &s->p;
+    or_sl_advect(D, s->u, s->u0, s->u0, s->v0, s->w0, p->dt, OR_U);
+    or_sl_advect(D, s->v, s->v0, s->u0, s->v0, s->w0, p->dt, OR_V);
+    or_sl_advect(D, s->w, s->w0, s->u0, s->v0, s->w0, p->dt, OR_W);
+    couple_vector(s);
+    or_smagorinsky(D, s->nu_t, s->u, s->v, s->w, p->cs, OR_NU);
+    s->counters[0] = diffuse(s, s->u, s->u0, s->ut, p->nu, s->nu_t, OR_U);
+    s->counters[1] = diffuse(s, s->v, s->v0, s->vt, p->nu, s->nu_t, OR_V);
+    s->counters[2] = diffuse(s, s->w, s->w0, s->wt, p->nu, s->nu_t, OR_W);
+    couple_vector(s);
+    if (p->has_force) {
+        if (has_dir(p->force_dir, 0)) or_add_source(D, s->u, s->fx, p->dt, OR_U);
+        if (has_dir(p->force_dir, 1)) or_add_source(D, s->v, s->fy, p->dt, OR_V);
+        if (has_dir(p->force_dir, 2)) or_add_source(D, s->w, s->fz, p->dt, OR_W);
+        couple_vector(s);
+    }
+    or_divergence(D, s->rhs, s->ut, s->vt, s->wt);
+    s->counters[4] = or_mg_pressure(s->mg, s->p_, s->rhs, t, p->dt);
+    or_projection(D, s->u, s->v, s->w, s->ut, s->vt, s->wt, s->p_);
+    or_sl_advect(D, s->T, s->T0, s->u, s->v, s->w, p->dt, OR_T);
+    couple_scalar(s);
+    if (p->prandtl > 0) {
+        const double rPr = 1. / p->prandtl;
+        for (size_t i = 0; i < s->n; ++i) s->kappa_t[i] = s->nu_t[i] * rPr;
+        s->counters[3] = diffuse(s, s->T, s->T0, s->Tt, p->kappa, s->kappa_t, OR_T);
+        couple_scalar(s);
+    } else if (p->kappa != 0.) {
+        s->counters[3] = diffuse(s, s->T, s->T0, s->Tt, p->kappa, NULL, OR_T);
+        couple_scalar(s);
+    }
+    if (p->has_temp_source) {
+        /* the scalar add_source ignores the direction list, ExplicitEulerSource.cpp:84-104 */
+        or_add_source(D, s->T, s->S_T, p->dt, OR_T);
+        couple_scalar(s);
+    }
+}
+
+void or_sim_step(or_sim *s, double t) {
+    memset(s->counters, 0, sizeof(s->counters));
+    if (s->p.solver == 0) do_step_ns(s, t);
+    else do_step_nstempturb(s, t);
+    update_sources(s, t);
+    update_data(s);
+}
+
+double *or_sim_field(or_sim *s, const char *name) {
+    if (!strcmp(name, "u")) return s->u;
+    if (!strcmp(name, "v")) return s->v;
+    if (!strcmp(name, "w")) return s->w;
+    if (!strcmp(name, "p")) return s->p_;
+    if (!strcmp(name, "T")) return s->T;
+    if (!strcmp(name, "rhs")) return s->rhs;
+    if (!strcmp(name, "nu_t")) return s->nu_t;
+    if (!strcmp(name, "S_T")) return s->S_T;
+    if (!strcmp(name, "f_x")) return s->fx;
+    if (!strcmp(name, "f_y")) return s->fy;
+    if (!strcmp(name, "f_z")) return s->fz;
+    if (!strcmp(name, "T_ambient")) return s->Ta;
+    return NULL;
+}
+
+void or_sim_counters(const or_sim *s, int *out5) { memcpy(out5, s->counters, sizeof(s->counters)); }
