@@ -1,0 +1,329 @@
+"""MI355X-native ARTSS solver hot path: Python bindings of the C ABI.
+
+The product is artss_amd/libartss_hip.so (HIP kernels for gfx950, C++ host
+code, C ABI declared in include/artss_hip.h).  This module is a thin ctypes
+layer over that ABI for tests, benchmarks and scripting; it performs no
+computation of its own and raises if the library has not been built
+(`python -m artss_amd.build`).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libartss_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "artss_hip.h")
+CONFIG_DIR = os.path.join(HERE, "configs")
+
+FIELD_TYPES = {"rho": 0, "u": 1, "v": 2, "w": 3, "p": 4, "T": 5, "nu": 6}
+RHO, U, V, W, P, T, NU = range(7)
+
+_lib = None
+
+
+class ArtssError(RuntimeError):
+    pass
+
+
+class LevelInfo(C.Structure):
+    _fields_ = [("N", C.c_int * 3), ("n", C.c_int * 3), ("size", C.c_longlong), ("h", C.c_double * 3),
+                ("inner_cells", C.c_longlong), ("n_obstacles", C.c_int)]
+
+
+def lib():
+    """Loads libartss_hip.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ArtssError("artss_amd/libartss_hip.so is not built; run `python -m artss_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    sig = {
+        "artss_last_error": (C.c_char_p, []),
+        "artss_version": (C.c_int, []),
+        "artss_domain_create": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp)]),
+        "artss_domain_destroy": (C.c_int, [vp]),
+        "artss_domain_levels": (C.c_int, [vp, ip]),
+        "artss_domain_level_info": (C.c_int, [vp, C.c_int, C.POINTER(LevelInfo)]),
+        "artss_domain_list": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint64), C.c_size_t,
+                                        C.POINTER(C.c_size_t)]),
+        "artss_domain_synchronize": (C.c_int, [vp]),
+        "artss_field_alloc": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+        "artss_field_free": (C.c_int, [vp, vp]),
+        "artss_field_upload": (C.c_int, [vp, vp, dp, C.c_size_t]),
+        "artss_field_download": (C.c_int, [vp, dp, vp, C.c_size_t]),
+        "artss_apply_boundary": (C.c_int, [vp, vp, C.c_int, C.c_int]),
+        "artss_sl_advect": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int]),
+        "artss_diffuse": (C.c_int, [vp, C.c_char_p, vp, vp, vp, C.c_double, vp, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, ip, dp]),
+        "artss_jacobi_step": (C.c_int, [vp, C.c_int, vp, vp, vp] + [C.c_double] * 6),
+        "artss_cgs_step": (C.c_int, [vp, C.c_int, vp, vp] + [C.c_double] * 6),
+        "artss_divergence": (C.c_int, [vp, vp, vp, vp, vp]),
+        "artss_projection": (C.c_int, [vp] + [vp] * 7),
+        "artss_smagorinsky": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, C.c_int]),
+        "artss_add_source": (C.c_int, [vp, vp, vp, C.c_int]),
+        "artss_buoyancy_force": (C.c_int, [vp, vp, vp, vp]),
+        "artss_gauss_source": (C.c_int, [vp, vp, C.c_double]),
+        "artss_mg_create": (C.c_int, [vp, C.POINTER(vp)]),
+        "artss_mg_destroy": (C.c_int, [vp]),
+        "artss_mg_pressure": (C.c_int, [vp, vp, vp, C.c_double, ip]),
+        "artss_mg_vcycle": (C.c_int, [vp, vp, vp]),
+        "artss_mg_residual": (C.c_int, [vp, vp, vp, dp]),
+        "artss_solver_create": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp)]),
+        "artss_solver_destroy": (C.c_int, [vp]),
+        "artss_solver_domain": (C.c_int, [vp, C.POINTER(vp)]),
+        "artss_solver_step": (C.c_int, [vp, C.c_double]),
+        "artss_solver_run": (C.c_int, [vp, C.c_long, C.c_long, dp]),
+        "artss_solver_field": (C.c_int, [vp, C.c_char_p, C.POINTER(vp), C.POINTER(C.c_size_t)]),
+        "artss_solver_counters": (C.c_int, [vp, ip, ip, dp]),
+        "artss_solver_set_pressure_mode": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise ArtssError("artss error %d: %s" % (rc, lib().artss_last_error().decode()))
+
+
+def _as_text(xml):
+    if not xml.lstrip().startswith("<"):
+        with open(xml) as fh:
+            xml = fh.read()
+    return xml.encode()
+
+
+class Domain:
+    """DomainData + DomainController for one XML input.  device=-1: host-only
+    (geometry and index lists without a GPU)."""
+
+    def __init__(self, xml=None, device=0, _handle=None, _owner=None):
+        self.L = lib()
+        self._owner = _owner
+        if _handle is not None:
+            self.h = _handle
+            self._owned = False
+        else:
+            h = C.c_void_p()
+            _check(self.L.artss_domain_create(_as_text(xml), device, C.byref(h)))
+            self.h = h
+            self._owned = True
+        n = C.c_int()
+        _check(self.L.artss_domain_levels(self.h, C.byref(n)))
+        self.levels = n.value
+
+    def close(self):
+        if getattr(self, "_owned", False) and self.h:
+            _check(self.L.artss_domain_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self, level=0):
+        i = LevelInfo()
+        _check(self.L.artss_domain_level_info(self.h, level, C.byref(i)))
+        return dict(N=tuple(i.N), n=tuple(i.n), size=i.size, h=tuple(i.h), inner_cells=i.inner_cells,
+                    n_obstacles=i.n_obstacles)
+
+    def size(self, level=0):
+        return self.info(level)["size"]
+
+    def list(self, which, level=0, obstacle=0):
+        n = C.c_size_t()
+        _check(self.L.artss_domain_list(self.h, which, level, obstacle, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint64)
+        if n.value:
+            _check(self.L.artss_domain_list(self.h, which, level, obstacle,
+                                            out.ctypes.data_as(C.POINTER(C.c_uint64)), n.value, C.byref(n)))
+        return out
+
+    def sync(self):
+        _check(self.L.artss_domain_synchronize(self.h))
+
+    # ---- device fields
+    def alloc(self, level=0):
+        return DeviceField(self, level)
+
+    def field(self, host, level=0):
+        f = DeviceField(self, level)
+        f.upload(host)
+        return f
+
+    # ---- operators (all synchronous)
+    def apply_boundary(self, f, ft, level=0):
+        _check(self.L.artss_apply_boundary(self.h, f.ptr, ft, level))
+
+    def sl_advect(self, out, inp, u, v, w, ft):
+        _check(self.L.artss_sl_advect(self.h, out.ptr, inp.ptr, u.ptr, v.ptr, w.ptr, ft))
+
+    def diffuse(self, method, out, inp, b, D, ev, ft, max_iter, tol, w):
+        it = C.c_int()
+        res = C.c_double()
+        _check(self.L.artss_diffuse(self.h, method.encode(), out.ptr, inp.ptr, b.ptr, D, ev.ptr if ev else None, ft,
+                                    max_iter, tol, w, C.byref(it), C.byref(res)))
+        return it.value, res.value
+
+    def jacobi_step(self, level, out, inp, b, ax, ay, az, beta, dsign, w):
+        _check(self.L.artss_jacobi_step(self.h, level, out.ptr, inp.ptr, b.ptr, ax, ay, az, beta, dsign, w))
+
+    def cgs_step(self, level, out, b, ax, ay, az, beta, dsign, w):
+        _check(self.L.artss_cgs_step(self.h, level, out.ptr, b.ptr, ax, ay, az, beta, dsign, w))
+
+    def divergence(self, out, x, y, z):
+        _check(self.L.artss_divergence(self.h, out.ptr, x.ptr, y.ptr, z.ptr))
+
+    def projection(self, ou, ov, ow, iu, iv, iw, ip):
+        _check(self.L.artss_projection(self.h, ou.ptr, ov.ptr, ow.ptr, iu.ptr, iv.ptr, iw.ptr, ip.ptr))
+
+    def smagorinsky(self, ev, u, v, w, cs, ft=NU):
+        _check(self.L.artss_smagorinsky(self.h, ev.ptr, u.ptr, v.ptr, w.ptr, cs, ft))
+
+    def add_source(self, out, s, ft):
+        _check(self.L.artss_add_source(self.h, out.ptr, s.ptr, ft))
+
+    def buoyancy_force(self, out, T_, Ta):
+        _check(self.L.artss_buoyancy_force(self.h, out.ptr, T_.ptr, Ta.ptr))
+
+    def gauss_source(self, out, t):
+        _check(self.L.artss_gauss_source(self.h, out.ptr, t))
+
+    def multigrid(self):
+        return Multigrid(self)
+
+
+class DeviceField:
+    """A float64 field in HBM (reference layout) owned by a domain."""
+
+    def __init__(self, dom, level=0):
+        self.dom = dom
+        self.level = level
+        self.n = dom.size(level)
+        p = C.c_void_p()
+        _check(dom.L.artss_field_alloc(dom.h, level, C.byref(p)))
+        self.ptr = p
+        self._owned = True
+
+    @classmethod
+    def view(cls, dom, ptr, n, level=0):
+        f = cls.__new__(cls)
+        f.dom, f.level, f.n, f.ptr, f._owned = dom, level, n, C.c_void_p(ptr), False
+        return f
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        if a.size != self.n:
+            raise ValueError("size mismatch: %d != %d" % (a.size, self.n))
+        _check(self.dom.L.artss_field_upload(self.dom.h, self.ptr, a.ctypes.data_as(C.POINTER(C.c_double)), a.size))
+
+    def download(self):
+        out = np.empty(self.n, np.float64)
+        _check(self.dom.L.artss_field_download(self.dom.h, out.ctypes.data_as(C.POINTER(C.c_double)), self.ptr,
+                                               self.n))
+        return out
+
+    def free(self):
+        if self._owned and self.ptr:
+            _check(self.dom.L.artss_field_free(self.dom.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Multigrid:
+    """IPressure::pressure -> VCycleMG with the XML pressure settings."""
+
+    def __init__(self, dom):
+        self.dom = dom
+        h = C.c_void_p()
+        _check(dom.L.artss_mg_create(dom.h, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.dom.L.artss_mg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def pressure(self, p, rhs, t):
+        c = C.c_int()
+        _check(self.dom.L.artss_mg_pressure(self.h, p.ptr, rhs.ptr, t, C.byref(c)))
+        return c.value
+
+    def vcycle(self, p, rhs):
+        _check(self.dom.L.artss_mg_vcycle(self.h, p.ptr, rhs.ptr))
+
+    def residual(self, p, rhs):
+        r = C.c_double()
+        _check(self.dom.L.artss_mg_residual(self.h, p.ptr, rhs.ptr, C.byref(r)))
+        return r.value
+
+
+class Solver:
+    """SolverController + TimeIntegration for one XML input, resident on one GPU."""
+
+    def __init__(self, xml, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        _check(self.L.artss_solver_create(_as_text(xml), device, C.byref(h)))
+        self.h = h
+        d = C.c_void_p()
+        _check(self.L.artss_solver_domain(self.h, C.byref(d)))
+        self.domain = Domain(_handle=d, _owner=self)
+
+    def close(self):
+        if self.h:
+            _check(self.L.artss_solver_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, t):
+        _check(self.L.artss_solver_step(self.h, t))
+
+    def run(self, first_step, count):
+        s = C.c_double()
+        _check(self.L.artss_solver_run(self.h, first_step, count, C.byref(s)))
+        return s.value
+
+    def device_field(self, name):
+        p = C.c_void_p()
+        n = C.c_size_t()
+        _check(self.L.artss_solver_field(self.h, name.encode(), C.byref(p), C.byref(n)))
+        return DeviceField.view(self.domain, p.value, n.value)
+
+    def field(self, name):
+        return self.device_field(name).download()
+
+    def set_field(self, name, host):
+        self.device_field(name).upload(host)
+
+    def counters(self):
+        d = (C.c_int * 4)()
+        vc = C.c_int()
+        r = C.c_double()
+        _check(self.L.artss_solver_counters(self.h, d, C.byref(vc), C.byref(r)))
+        return dict(diffusion=list(d), vcycles=vc.value, residual=r.value)
+
+    def set_pressure_mode(self, to_tolerance, tolerance=1e-8, max_cycles=200):
+        _check(self.L.artss_solver_set_pressure_mode(self.h, 1 if to_tolerance else 0, tolerance, max_cycles))

@@ -1,0 +1,75 @@
+// Device-side descriptors shared by the HIP kernels and the host code.
+//
+// Field layout is the reference's (utility/GlobalMacrosTypes.h:70): float64,
+// x fastest, idx = i + Nx*j + Nx*Ny*k, one ghost layer, so a device array can
+// be handed to / taken from a reference Field without reordering.
+//
+// The reference iterates 8-byte index lists (GPULists/SingleJoinedList).
+// Here every level is a structured box and a cell is an inner cell when it
+// lies in [1,n] on each axis and in none of the level's obstacle boxes, which
+// is exactly the reference's inner list (Domain.cpp:180-210 subtracts the
+// merged obstacle list, Multigrid.cpp:438-514 restricts obstacles as boxes).
+// Kernels therefore read no index list at all and access memory coalesced
+// along x.
+#pragma once
+
+#include <cstdint>
+
+namespace artss {
+
+constexpr int kMaxObstacles = 32;  // obstacle boxes per level carried in kernel arguments
+
+struct Box {
+    int lo[3];
+    int hi[3];  // inclusive
+};
+
+struct LevelDesc {
+    int N[3];       // cells including ghosts (Nx, Ny, Nz)
+    int n[3];       // inner cells
+    long long size; // Nx*Ny*Nz
+    double h[3];    // spacing
+    int n_obst;
+    Box obst[kMaxObstacles];
+};
+
+// Domain ghost fill for one (field type, level): per patch an affine map
+// g = sign * f[ref] + value.  The kernel evaluates, per ghost cell, the chain
+// of patches that write it in the reference order LEFT, RIGHT, BOTTOM, TOP,
+// FRONT, BACK (DomainBoundary.cpp:193-239), so edges and corners get exactly
+// the value the sequential patch loop leaves there.
+struct DomainBC {
+    int active;        // field type has boundary data (BoundaryData::is_empty)
+    double sign[6];
+    double value[6];
+    int target[6];     // coordinate along the patch axis of the referenced cell
+};
+
+// One obstacle face write: f[idx] = sign * f[idx + ref] + value, parameters
+// from a per-(field type, level) table indexed by `param`.
+struct FaceEntry {
+    uint32_t idx;
+    int32_t ref;
+    int32_t param;  // obstacle * 6 + patch
+    int32_t pad;
+};
+
+struct FaceParam {
+    double sign;
+    double value;
+    int active;  // 0: no write for this field type (no data or periodic)
+    int pad;
+};
+
+#if defined(__HIPCC__)
+__host__ __device__ inline bool in_obstacle(const LevelDesc &L, int i, int j, int k) {
+    for (int o = 0; o < L.n_obst; o++) {
+        const Box &b = L.obst[o];
+        if (i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2])
+            return true;
+    }
+    return false;
+}
+#endif
+
+}  // namespace artss

@@ -1,0 +1,75 @@
+// Host-side launchers for the HIP kernels of the solver hot path.
+// Every launcher enqueues on the given stream and returns immediately; the
+// caller owns synchronisation.  All arrays are device pointers in the
+// reference layout (device.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace artss {
+namespace k {
+
+// ---- element-wise over the whole array (Field.h:64-135) ----
+void copy(hipStream_t s, double *dst, const double *src, long long n);
+void fill(hipStream_t s, double *dst, double v, long long n);
+void scale(hipStream_t s, double *dst, double v, long long n);           // dst *= v
+void add(hipStream_t s, double *dst, const double *src, long long n);    // dst += src
+void scale_to(hipStream_t s, double *dst, const double *src, double v, long long n);  // dst = src * v
+// copy on the domain list (inner + ghost shell, obstacle cells untouched), JacobiDiffuse.cpp:85-99
+void copy_domain(hipStream_t s, const LevelDesc &L, double *dst, const double *src);
+
+// ---- boundary fill (Multigrid.cpp:555-572) ----
+void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p, double *f);
+void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f);
+
+// ---- SLAdvect::advect (SLAdvect.cpp:71-163) ----
+void sl_advect(hipStream_t s, const LevelDesc &L, double dt, double *out, const double *in, const double *u,
+               const double *v, const double *w);
+
+// ---- Jacobi diffusion (JacobiDiffuse.cpp) with fused update-size partials ----
+// plain level-0 form divides by rb (:241-249), turbulent form (:345-362) when ev != nullptr.
+// partial[b] = sum over the block of (rb*(out-in))^2 (:72-77, :162-172)
+void jacobi_diffuse_step(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b,
+                         double ax, double ay, double az, double rb, double D, const double *ev, double dt,
+                         double w, double *partial);
+// multigrid form, multiplies by beta (:292-300)
+void jacobi_step_mg(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b, double ax,
+                    double ay, double az, double beta, double dsign, double w);
+
+// ---- colored Gauss-Seidel (ColoredGaussSeidelDiffuse.cpp:190-341), one color per launch ----
+// ev == nullptr: constant ax,ay,az,beta; else per-cell coefficients from D, ev, dt
+void cgs_color(hipStream_t s, const LevelDesc &L, int color, double *out, const double *b, double ax, double ay,
+               double az, double beta, double dsign, double w, double D, const double *ev, double dt);
+// (rb*(out-in))^2 partials over inner cells, ColoredGaussSeidelDiffuse.cpp:77-82, :147-157
+void update_norm_partials(hipStream_t s, const LevelDesc &L, const double *out, const double *in, double rb,
+                          double D, const double *ev, double dt, double *partial);
+
+// ---- multigrid (VCycleMG.cpp) ----
+void residuum(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *b);   // :346-354
+void residual_partials(hipStream_t s, const LevelDesc &L, const double *x, const double *b,
+                       double *partial);                                                          // :186-193
+void restrict_(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine);
+void prolongate(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse);
+
+// ---- reductions ----
+int partial_count(const LevelDesc &L);  // number of partial slots a level-wide kernel writes
+void reduce_sum(hipStream_t s, const double *partial, int n, double *out);  // fixed-order sum
+
+// ---- IPressure (IPressure.cpp) ----
+void divergence(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *y, const double *z);
+void projection(hipStream_t s, const LevelDesc &L, double *ou, double *ov, double *ow, const double *iu,
+                const double *iv, const double *iw, const double *p);
+
+// ---- ConstSmagorinsky (ConstSmagorinsky.cpp:60-80) ----
+void smagorinsky(hipStream_t s, const LevelDesc &L, double *ev, const double *u, const double *v, const double *w,
+                 double cs2_delta2);
+
+// ---- sources ----
+void add_source(hipStream_t s, const LevelDesc &L, double *out, const double *src, double dt);   // inner list
+void buoyancy(hipStream_t s, const LevelDesc &L, double *out, const double *T, const double *Ta, double beta,
+              double g);                                                                           // domain list
+
+}  // namespace k
+}  // namespace artss

@@ -1,0 +1,569 @@
+// HIP kernels of the ARTSS solver hot path for gfx950 (MI355X).
+//
+// All stencils are HBM-bandwidth bound FP64 work (0.1-0.5 flop/byte), so the
+// design goal is one coalesced pass per operator: a workgroup is one 64-wide
+// wavefront along x times 4 rows in y, a wavefront reads 512 contiguous bytes
+// per array, and inner/obstacle membership is decided from obstacle boxes in
+// kernel arguments instead of 8-byte index lists.
+//
+// Bit-for-bit agreement with the reference serial build: this file is built
+// with -ffp-contract=off and every expression keeps the reference's operand
+// order (cited per kernel); reductions write per-workgroup partial sums that
+// a single workgroup adds in a fixed order, so results are deterministic run
+// to run (their rounding differs from the serial left-to-right sum).
+#include "kernels.h"
+
+#include <cmath>
+
+namespace artss {
+namespace k {
+
+namespace {
+
+constexpr int BX = 64;  // one wavefront along x
+constexpr int BY = 4;   // 4 wavefronts per workgroup
+
+inline dim3 inner_grid(const LevelDesc &L) {
+    return dim3((L.n[0] + BX - 1) / BX, (L.n[1] + BY - 1) / BY, L.n[2]);
+}
+inline dim3 full_grid(const LevelDesc &L) {
+    return dim3((L.N[0] + BX - 1) / BX, (L.N[1] + BY - 1) / BY, L.N[2]);
+}
+inline dim3 block() { return dim3(BX, BY, 1); }
+inline unsigned grid1d(long long n, int threads) {
+    long long g = (n + threads - 1) / threads;
+    return static_cast<unsigned>(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+
+__device__ inline bool inner_cell(const LevelDesc &L, int &i, int &j, int &k) {
+    i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    k = 1 + static_cast<int>(blockIdx.z);
+    return i <= L.n[0] && j <= L.n[1] && !in_obstacle(L, i, j, k);
+}
+
+__device__ inline long long lin(const LevelDesc &L, long long i, long long j, long long k) {
+    return i + static_cast<long long>(L.N[0]) * (j + static_cast<long long>(L.N[1]) * k);
+}
+
+// workgroup sum of one double per thread (64x4 block), result valid in thread (0,0)
+__device__ inline double block_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __shared__ double wsum[BY];
+    if (threadIdx.x == 0) wsum[threadIdx.y] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        for (int w = 0; w < BY; w++) r += wsum[w];
+    return r;
+}
+
+__device__ inline void write_partial(double *partial, double v) {
+    double s = block_sum(v);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// ------------------------------------------------------------------ element-wise
+__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i];
+}
+__global__ void k_fill(double *__restrict__ d, double v, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = v;
+}
+__global__ void k_scale(double *__restrict__ d, double v, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] *= v;
+}
+__global__ void k_add(double *__restrict__ d, const double *__restrict__ s, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] += s[i];
+}
+__global__ void k_scale_to(double *__restrict__ d, const double *__restrict__ s, double v, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i] * v;
+}
+
+// copy over inner + ghost shell, leaving obstacle cells alone
+__global__ void k_copy_domain(LevelDesc L, double *__restrict__ d, const double *__restrict__ s) {
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    if (i >= L.N[0] || j >= L.N[1]) return;
+    const bool ghost = i == 0 || j == 0 || k == 0 || i == L.N[0] - 1 || j == L.N[1] - 1 || k == L.N[2] - 1;
+    if (!ghost && in_obstacle(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    d[idx] = s[idx];
+}
+
+// ------------------------------------------------------------------ boundary
+// ObstacleBoundary.cpp:42-48: f[idx] = sign * f[idx +- ref] + value
+__global__ void k_bc_faces(const FaceEntry *__restrict__ e, long long n, const FaceParam *__restrict__ p,
+                           double *__restrict__ f) {
+    const long long t = blockIdx.x * 256LL + threadIdx.x;
+    if (t >= n) return;
+    const FaceEntry x = e[t];
+    const FaceParam q = p[x.param];
+    if (!q.active) return;
+    f[x.idx] = q.sign * f[static_cast<long long>(x.idx) + x.ref] + q.value;
+}
+
+// DomainBoundary.cpp:25-49 for all six patches at once.  Each ghost cell is
+// owned by one thread; its final value is the composition of the affine maps
+// of the patches containing it, applied in patch order to the inner cell the
+// chain of reference offsets ends on.
+__global__ void k_bc_domain(LevelDesc L, DomainBC bc, double *__restrict__ f) {
+    const long long Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
+    long long t = blockIdx.x * 256LL + threadIdx.x;
+    long long i, j, k;
+    const long long fx = Ny * Nz, fy = (Nx - 2) * Nz, fz = (Nx - 2) * (Ny - 2);
+    if (t < 2 * fx) {
+        const long long side = t / fx, r = t % fx;
+        i = side ? Nx - 1 : 0;
+        j = r % Ny;
+        k = r / Ny;
+    } else if ((t -= 2 * fx) < 2 * fy) {
+        const long long side = t / fy, r = t % fy;
+        j = side ? Ny - 1 : 0;
+        i = 1 + r % (Nx - 2);
+        k = r / (Nx - 2);
+    } else if ((t -= 2 * fy) < 2 * fz) {
+        const long long side = t / fz, r = t % fz;
+        k = side ? Nz - 1 : 0;
+        i = 1 + r % (Nx - 2);
+        j = 1 + r / (Nx - 2);
+    } else {
+        return;
+    }
+    const int px = i == 0 ? 0 : (i == Nx - 1 ? 1 : -1);
+    const int py = j == 0 ? 2 : (j == Ny - 1 ? 3 : -1);
+    const int pz = k == 0 ? 4 : (k == Nz - 1 ? 5 : -1);
+    long long ii = i, jj = j, kk = k;
+    if (px >= 0) ii = bc.target[px];
+    if (py >= 0) jj = bc.target[py];
+    if (pz >= 0) kk = bc.target[pz];
+    double v = f[ii + Nx * (jj + Ny * kk)];
+    if (px >= 0) v = bc.sign[px] * v + bc.value[px];
+    if (py >= 0) v = bc.sign[py] * v + bc.value[py];
+    if (pz >= 0) v = bc.sign[pz] * v + bc.value[pz];
+    f[i + Nx * (j + Ny * k)] = v;
+}
+
+// ------------------------------------------------------------------ advection
+// SLAdvect.cpp:71-163.  fmod(C, 1) of a finite C > 0 equals C - trunc(C)
+// exactly (the difference is representable, Sterbenz), so no library call.
+__global__ void k_sl_advect(LevelDesc L, double dtx, double dty, double dtz, double *__restrict__ out,
+                            const double *__restrict__ in, const double *__restrict__ u,
+                            const double *__restrict__ v, const double *__restrict__ w) {
+    int ci, cj, ck;
+    if (!inner_cell(L, ci, cj, ck)) return;
+    const long long idx = lin(L, ci, cj, ck);
+    const double eps = 1e-6;
+    const long i = ci, j = cj, k = ck;
+    const double Ci = dtx * u[idx], Cj = dty * v[idx], Ck = dtz * w[idx];
+    long i0 = i, i1 = i, j0 = j, j1 = j, k0 = k, k1 = k;
+    double r = 1, s = 1, t = 1;
+    // cell shift (long)C; |C| is capped at 1e12 only so the cast cannot
+    // overflow (the clamp to the inner range below already covers any such C)
+    auto shift = [](double C) { return static_cast<long>(fmin(fmax(C, -1e12), 1e12)); };
+    if (Ci > eps) {
+        i0 = max(1L, i - shift(Ci));
+        i1 = i0 - 1;
+        r = Ci - trunc(Ci);
+    } else if (Ci < -eps) {
+        i0 = min(static_cast<long>(L.n[0]), i - shift(Ci));
+        i1 = i0 + 1;
+        r = (-Ci) - trunc(-Ci);
+    }
+    if (Cj > eps) {
+        j0 = max(1L, j - shift(Cj));
+        j1 = j0 - 1;
+        s = Cj - trunc(Cj);
+    } else if (Cj < -eps) {
+        j0 = min(static_cast<long>(L.n[1]), j - shift(Cj));
+        j1 = j0 + 1;
+        s = (-Cj) - trunc(-Cj);
+    }
+    if (Ck > eps) {
+        k0 = max(1L, k - shift(Ck));
+        k1 = k0 - 1;
+        t = Ck - trunc(Ck);
+    } else if (Ck < -eps) {
+        k0 = min(static_cast<long>(L.n[2]), k - shift(Ck));
+        k1 = k0 + 1;
+        t = (-Ck) - trunc(-Ck);
+    }
+    const double d000 = in[lin(L, i0, j0, k0)], d100 = in[lin(L, i1, j0, k0)];
+    const double d010 = in[lin(L, i0, j1, k0)], d110 = in[lin(L, i1, j1, k0)];
+    const double d001 = in[lin(L, i0, j0, k1)], d101 = in[lin(L, i1, j0, k1)];
+    const double d011 = in[lin(L, i0, j1, k1)], d111 = in[lin(L, i1, j1, k1)];
+    const double r100 = d000 + r * (d100 - d000);
+    const double r110 = d010 + r * (d110 - d010);
+    const double r101 = d001 + r * (d101 - d001);
+    const double r111 = d011 + r * (d111 - d011);
+    const double s110 = r100 + s * (r110 - r100);
+    const double s111 = r101 + s * (r111 - r101);
+    out[idx] = s110 + t * (s111 - s110);
+}
+
+// ------------------------------------------------------------------ Jacobi
+struct DiffCoef {
+    double ax, ay, az, rb;   // constant form
+    double D, dt, rdx, rdy, rdz;
+};
+
+// JacobiDiffuse.cpp:241-249 (ev == nullptr) and :345-362 (turbulent), plus the
+// update-size norm of :72-77 / :162-172 as per-workgroup partial sums.
+__global__ void k_jacobi_diffuse(LevelDesc L, DiffCoef c, double w, double *__restrict__ out,
+                                 const double *__restrict__ in, const double *__restrict__ b,
+                                 const double *__restrict__ ev, double *__restrict__ partial) {
+    int i, j, k;
+    double sq = 0.0;
+    if (inner_cell(L, i, j, k)) {
+        const long long idx = lin(L, i, j, k);
+        const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+        const double dsign = 1;
+        double rb, out_h;
+        if (ev == nullptr) {
+            rb = c.rb;
+            out_h = (dsign * b[idx] + c.ax * (in[idx + 1] + in[idx - 1]) + c.ay * (in[idx + nj] + in[idx - nj]) +
+                     c.az * (in[idx + nk] + in[idx - nk])) / rb;
+        } else {
+            const double e = ev[idx];
+            const double aX = (c.D + e) * c.dt * c.rdx * c.rdx;
+            const double aY = (c.D + e) * c.dt * c.rdy * c.rdy;
+            const double aZ = (c.D + e) * c.dt * c.rdz * c.rdz;
+            rb = (1. + 2. * (aX + aY + aZ));
+            const double bb = 1. / rb;
+            const double d_aX = aX * (in[idx + 1] + in[idx - 1]);
+            const double d_aY = aY * (in[idx + nj] + in[idx - nj]);
+            const double d_aZ = aZ * (in[idx + nk] + in[idx - nk]);
+            out_h = bb * (dsign * b[idx] + d_aX + d_aY + d_aZ);
+        }
+        const double x = in[idx];
+        const double o = (1 - w) * x + w * out_h;
+        out[idx] = o;
+        const double res = rb * (o - x);
+        sq = res * res;
+    }
+    write_partial(partial, sq);
+}
+
+// JacobiDiffuse.cpp:292-300
+__global__ void k_jacobi_mg(LevelDesc L, double ax, double ay, double az, double beta, double dsign, double w,
+                            double *__restrict__ out, const double *__restrict__ in, const double *__restrict__ b) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const double out_h = beta * (dsign * b[idx] + ax * (in[idx + 1] + in[idx - 1]) +
+                                 ay * (in[idx + nj] + in[idx - nj]) + az * (in[idx + nk] + in[idx - nk]));
+    out[idx] = (1 - w) * in[idx] + w * out_h;
+}
+
+// ------------------------------------------------------------------ colored Gauss-Seidel
+// One color of ColoredGaussSeidelDiffuse::colored_gauss_seidel_step
+// (:212-222, :270-294) with the stencil of :316-341.  Cells of one color only
+// read cells of the other color, so the order inside a color is free.
+// Thread t of a row handles i = i_first + 2t.
+__global__ void k_cgs_color(LevelDesc L, int color, double ax, double ay, double az, double beta, double dsign,
+                            double w, double D, const double *__restrict__ ev, double dt, double rdx, double rdy,
+                            double rdz, double *__restrict__ out, const double *__restrict__ b) {
+    const int t = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = 1 + static_cast<int>(blockIdx.z);
+    if (j > L.n[1]) return;
+    const int i = (((1 + j + k) & 1) == color ? 1 : 2) + 2 * t;
+    if (i > L.n[0] || in_obstacle(L, i, j, k)) return;
+    const long long index = lin(L, i, j, k);
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    double aX = ax, aY = ay, aZ = az, bb = beta;
+    if (ev != nullptr) {
+        const double e = ev[index];
+        aX = (D + e) * dt * rdx * rdx;
+        aY = (D + e) * dt * rdy * rdy;
+        aZ = (D + e) * dt * rdz * rdz;
+        const double rb = (1. + 2. * (aX + aY + aZ));
+        bb = 1. / rb;
+    }
+    const double ox = out[index + 1], ox2 = out[index - 1];
+    const double oy = out[index + nj], oy2 = out[index - nj];
+    const double oz = out[index + nk], oz2 = out[index - nk];
+    const double db = b[index], r_out = out[index];
+    const double out_h = bb * (dsign * db + aX * (ox + ox2) + aY * (oy + oy2) + aZ * (oz + oz2));
+    out[index] = (1 - w) * r_out + w * out_h;
+}
+
+__global__ void k_update_norm(LevelDesc L, DiffCoef c, const double *__restrict__ out, const double *__restrict__ in,
+                              const double *__restrict__ ev, double *__restrict__ partial) {
+    int i, j, k;
+    double sq = 0.0;
+    if (inner_cell(L, i, j, k)) {
+        const long long idx = lin(L, i, j, k);
+        double rb = c.rb;
+        if (ev != nullptr) {
+            const double e = ev[idx];
+            const double aX = (c.D + e) * c.dt * c.rdx * c.rdx;
+            const double aY = (c.D + e) * c.dt * c.rdy * c.rdy;
+            const double aZ = (c.D + e) * c.dt * c.rdz * c.rdz;
+            rb = (1. + 2. * (aX + aY + aZ));
+        }
+        const double r = rb * (out[idx] - in[idx]);
+        sq = r * r;
+    }
+    write_partial(partial, sq);
+}
+
+// ------------------------------------------------------------------ multigrid
+__device__ inline double laplace(const LevelDesc &L, const double *__restrict__ x, long long i) {
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const double rdx2 = 1. / (L.h[0] * L.h[0]), rdy2 = 1. / (L.h[1] * L.h[1]), rdz2 = 1. / (L.h[2] * L.h[2]);
+    return (rdx2 * (x[i - 1] - 2 * x[i] + x[i + 1]) + rdy2 * (x[i - nj] - 2 * x[i] + x[i + nj]) +
+            rdz2 * (x[i - nk] - 2 * x[i] + x[i + nk]));
+}
+
+// VCycleMG.cpp:346-354
+__global__ void k_residuum(LevelDesc L, double *__restrict__ out, const double *__restrict__ x,
+                           const double *__restrict__ b) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const double weighted = laplace(L, x, idx);
+    out[idx] = b[idx] - weighted;
+}
+
+// VCycleMG.cpp:186-193 / :668-675 / :777-785
+__global__ void k_residual_partials(LevelDesc L, const double *__restrict__ x, const double *__restrict__ b,
+                                    double *__restrict__ partial) {
+    int i, j, k;
+    double sq = 0.0;
+    if (inner_cell(L, i, j, k)) {
+        const long long idx = lin(L, i, j, k);
+        const double r = b[idx] - laplace(L, x, idx);
+        sq = r * r;
+    }
+    write_partial(partial, sq);
+}
+
+// VCycleMG.cpp:404-421, iterating the coarse inner cells
+__global__ void k_restrict(LevelDesc Lc, LevelDesc Lf, double *__restrict__ out, const double *__restrict__ in) {
+    int i, j, k;
+    if (!inner_cell(Lc, i, j, k)) return;
+    const long long i2 = 2LL * i, j2 = 2LL * j, k2 = 2LL * k;
+    out[lin(Lc, i, j, k)] = 0.125 * (in[lin(Lf, i2 - 1, j2 - 1, k2 - 1)] + in[lin(Lf, i2, j2 - 1, k2 - 1)] +
+                                     in[lin(Lf, i2 - 1, j2, k2 - 1)] + in[lin(Lf, i2, j2, k2 - 1)] +
+                                     in[lin(Lf, i2 - 1, j2 - 1, k2)] + in[lin(Lf, i2, j2 - 1, k2)] +
+                                     in[lin(Lf, i2 - 1, j2, k2)] + in[lin(Lf, i2, j2, k2)]);
+}
+
+// VCycleMG.cpp:466-546: eight fine cells per coarse inner cell, weights
+// (27, 9, 9, 9, 3, 3, 3, 1) / 64 toward the neighbours on the fine cell's side.
+__global__ void k_prolongate(LevelDesc Lc, LevelDesc Lf, double *__restrict__ out, const double *__restrict__ in) {
+    int i, j, k;
+    if (!inner_cell(Lc, i, j, k)) return;
+    const long long idx = lin(Lc, i, j, k);
+    const long long ni = 1, nj = Lc.N[0], nk = static_cast<long long>(Lc.N[0]) * Lc.N[1];
+    const double c0 = in[idx];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        // reference order of the eight writes: (2i,2j,2k), (2i,2j,2k-1), (2i,2j-1,2k), (2i,2j-1,2k-1),
+        // (2i-1,2j,2k), (2i-1,2j,2k-1), (2i-1,2j-1,2k), (2i-1,2j-1,2k-1)
+        const int di = (q & 4) ? -1 : 1, dj = (q & 2) ? -1 : 1, dk = (q & 1) ? -1 : 1;
+        const long long oi = di * ni, oj = dj * nj, ok = dk * nk;
+        const double val = 0.015625 * (27 * c0 + 9 * in[idx + oi] + 9 * in[idx + oj] + 9 * in[idx + ok] +
+                                       3 * in[idx + oi + oj] + 3 * in[idx + oi + ok] + 3 * in[idx + oj + ok] +
+                                       in[idx + oi + oj + ok]);
+        out[lin(Lf, 2LL * i - (q & 4 ? 1 : 0), 2LL * j - (q & 2 ? 1 : 0), 2LL * k - (q & 1 ? 1 : 0))] = val;
+    }
+}
+
+__global__ void k_reduce_sum(const double *__restrict__ partial, int n, double *__restrict__ out) {
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += partial[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __shared__ double wsum[16];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < 16; w++) r += wsum[w];
+        out[0] = r;
+    }
+}
+
+// ------------------------------------------------------------------ IPressure
+// IPressure.cpp:58-104: divergence on inner cells, zero on the ghost shell
+__global__ void k_divergence(LevelDesc L, double hx, double hy, double hz, double *__restrict__ out,
+                             const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z) {
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    if (i >= L.N[0] || j >= L.N[1]) return;
+    const long long idx = lin(L, i, j, k);
+    if (i == 0 || j == 0 || k == 0 || i == L.N[0] - 1 || j == L.N[1] - 1 || k == L.N[2] - 1) {
+        out[idx] = 0;
+        return;
+    }
+    if (in_obstacle(L, i, j, k)) return;
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const double vx = hx * (x[idx + 1] - x[idx - 1]);
+    const double vy = hy * (y[idx + nj] - y[idx - nj]);
+    const double vz = hz * (z[idx + nk] - z[idx - nk]);
+    out[idx] = vx + vy + vz;
+}
+
+// IPressure.cpp:154-161
+__global__ void k_projection(LevelDesc L, double hx, double hy, double hz, double *__restrict__ ou,
+                             double *__restrict__ ov, double *__restrict__ ow, const double *__restrict__ iu,
+                             const double *__restrict__ iv, const double *__restrict__ iw,
+                             const double *__restrict__ p) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    ou[idx] = iu[idx] - hx * (p[idx + 1] - p[idx - 1]);
+    ov[idx] = iv[idx] - hy * (p[idx + nj] - p[idx - nj]);
+    ow[idx] = iw[idx] - hz * (p[idx + nk] - p[idx - nk]);
+}
+
+// ------------------------------------------------------------------ ConstSmagorinsky
+__global__ void k_smagorinsky(LevelDesc L, double rdx, double rdy, double rdz, double cdd, double *__restrict__ ev,
+                              const double *__restrict__ u, const double *__restrict__ v,
+                              const double *__restrict__ w) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long ni = 1, nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const double S11 = (u[idx + ni] - u[idx - ni]) * 0.5 * rdx;
+    const double S22 = (v[idx + nj] - v[idx - nj]) * 0.5 * rdy;
+    const double S33 = (w[idx + nk] - w[idx - nk]) * 0.5 * rdz;
+    const double S12 = 0.5 * ((u[idx + nj] - u[idx - nj]) * 0.5 * rdy + (v[idx + ni] - v[idx - ni]) * 0.5 * rdx);
+    const double S13 = 0.5 * ((u[idx + nk] - u[idx - nk]) * 0.5 * rdz + (w[idx + ni] - w[idx - ni]) * 0.5 * rdx);
+    const double S23 = 0.5 * ((v[idx + nk] - v[idx - nk]) * 0.5 * rdz + (w[idx + nj] - w[idx - nj]) * 0.5 * rdy);
+    const double S_bar = sqrt(2. * (S11 * S11 + S22 * S22 + S33 * S33 + 2. * (S12 * S12) + 2. * (S13 * S13) +
+                                    2. * (S23 * S23)));
+    ev[idx] = cdd * S_bar;
+}
+
+// ------------------------------------------------------------------ sources
+__global__ void k_add_source(LevelDesc L, double dt, double *__restrict__ out, const double *__restrict__ s) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    out[idx] += dt * s[idx];
+}
+
+__global__ void k_buoyancy(LevelDesc L, double beta, double g, double *__restrict__ out,
+                           const double *__restrict__ T, const double *__restrict__ Ta) {
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    if (i >= L.N[0] || j >= L.N[1]) return;
+    const bool ghost = i == 0 || j == 0 || k == 0 || i == L.N[0] - 1 || j == L.N[1] - 1 || k == L.N[2] - 1;
+    if (!ghost && in_obstacle(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    out[idx] = -beta * (T[idx] - Ta[idx]) * g;
+}
+
+}  // namespace
+
+// ================================================================== launchers
+void copy(hipStream_t s, double *dst, const double *src, long long n) {
+    k_copy<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n);
+}
+void fill(hipStream_t s, double *dst, double v, long long n) { k_fill<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
+void scale(hipStream_t s, double *dst, double v, long long n) { k_scale<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
+void add(hipStream_t s, double *dst, const double *src, long long n) {
+    k_add<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n);
+}
+void scale_to(hipStream_t s, double *dst, const double *src, double v, long long n) {
+    k_scale_to<<<grid1d(n, 256), 256, 0, s>>>(dst, src, v, n);
+}
+void copy_domain(hipStream_t s, const LevelDesc &L, double *dst, const double *src) {
+    k_copy_domain<<<full_grid(L), block(), 0, s>>>(L, dst, src);
+}
+
+void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p, double *f) {
+    if (n <= 0) return;
+    k_bc_faces<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(e, n, p, f);
+}
+void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f) {
+    if (!bc.active) return;
+    const long long Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
+    const long long n = 2 * Ny * Nz + 2 * (Nx - 2) * Nz + 2 * (Nx - 2) * (Ny - 2);
+    k_bc_domain<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, bc, f);
+}
+
+void sl_advect(hipStream_t s, const LevelDesc &L, double dt, double *out, const double *in, const double *u,
+               const double *v, const double *w) {
+    k_sl_advect<<<inner_grid(L), block(), 0, s>>>(L, dt / L.h[0], dt / L.h[1], dt / L.h[2], out, in, u, v, w);
+}
+
+void jacobi_diffuse_step(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b,
+                         double ax, double ay, double az, double rb, double D, const double *ev, double dt,
+                         double w, double *partial) {
+    DiffCoef c{ax, ay, az, rb, D, dt, 1. / L.h[0], 1. / L.h[1], 1. / L.h[2]};
+    k_jacobi_diffuse<<<inner_grid(L), block(), 0, s>>>(L, c, w, out, in, b, ev, partial);
+}
+void jacobi_step_mg(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b, double ax,
+                    double ay, double az, double beta, double dsign, double w) {
+    k_jacobi_mg<<<inner_grid(L), block(), 0, s>>>(L, ax, ay, az, beta, dsign, w, out, in, b);
+}
+
+void cgs_color(hipStream_t s, const LevelDesc &L, int color, double *out, const double *b, double ax, double ay,
+               double az, double beta, double dsign, double w, double D, const double *ev, double dt) {
+    const int half = (L.n[0] + 1) / 2;
+    dim3 g((half + BX - 1) / BX, (L.n[1] + BY - 1) / BY, L.n[2]);
+    k_cgs_color<<<g, block(), 0, s>>>(L, color, ax, ay, az, beta, dsign, w, D, ev, dt, 1. / L.h[0], 1. / L.h[1],
+                                      1. / L.h[2], out, b);
+}
+void update_norm_partials(hipStream_t s, const LevelDesc &L, const double *out, const double *in, double rb,
+                          double D, const double *ev, double dt, double *partial) {
+    DiffCoef c{0, 0, 0, rb, D, dt, 1. / L.h[0], 1. / L.h[1], 1. / L.h[2]};
+    k_update_norm<<<inner_grid(L), block(), 0, s>>>(L, c, out, in, ev, partial);
+}
+
+void residuum(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *b) {
+    k_residuum<<<inner_grid(L), block(), 0, s>>>(L, out, x, b);
+}
+void residual_partials(hipStream_t s, const LevelDesc &L, const double *x, const double *b, double *partial) {
+    k_residual_partials<<<inner_grid(L), block(), 0, s>>>(L, x, b, partial);
+}
+void restrict_(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine) {
+    k_restrict<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, coarse, fine);
+}
+void prolongate(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse) {
+    k_prolongate<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, fine, coarse);
+}
+
+int partial_count(const LevelDesc &L) {
+    const dim3 g = inner_grid(L);
+    return static_cast<int>(g.x * g.y * g.z);
+}
+void reduce_sum(hipStream_t s, const double *partial, int n, double *out) {
+    k_reduce_sum<<<1, 1024, 0, s>>>(partial, n, out);
+}
+
+void divergence(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *y, const double *z) {
+    const double hx = 0.5 * (1. / L.h[0]), hy = 0.5 * (1. / L.h[1]), hz = 0.5 * (1. / L.h[2]);
+    k_divergence<<<full_grid(L), block(), 0, s>>>(L, hx, hy, hz, out, x, y, z);
+}
+void projection(hipStream_t s, const LevelDesc &L, double *ou, double *ov, double *ow, const double *iu,
+                const double *iv, const double *iw, const double *p) {
+    const double hx = 0.5 * (1. / L.h[0]), hy = 0.5 * (1. / L.h[1]), hz = 0.5 * (1. / L.h[2]);
+    k_projection<<<inner_grid(L), block(), 0, s>>>(L, hx, hy, hz, ou, ov, ow, iu, iv, iw, p);
+}
+
+void smagorinsky(hipStream_t s, const LevelDesc &L, double *ev, const double *u, const double *v, const double *w,
+                 double cdd) {
+    k_smagorinsky<<<inner_grid(L), block(), 0, s>>>(L, 1. / L.h[0], 1. / L.h[1], 1. / L.h[2], cdd, ev, u, v, w);
+}
+
+void add_source(hipStream_t s, const LevelDesc &L, double *out, const double *src, double dt) {
+    k_add_source<<<inner_grid(L), block(), 0, s>>>(L, dt, out, src);
+}
+void buoyancy(hipStream_t s, const LevelDesc &L, double *out, const double *T, const double *Ta, double beta,
+              double g) {
+    k_buoyancy<<<full_grid(L), block(), 0, s>>>(L, beta, g, out, T, Ta);
+}
+
+}  // namespace k
+}  // namespace artss

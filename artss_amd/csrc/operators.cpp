@@ -1,0 +1,180 @@
+#include "operators.h"
+
+#include <cmath>
+
+namespace artss {
+
+namespace {
+void maybe_sync(Domain &d, bool sync) {
+    if (sync) ARTSS_HIP(hipStreamSynchronize(d.stream()));
+}
+}  // namespace
+
+// IPressure.cpp:20-110
+void IPressure::divergence(Field &out, const Field &in_x, const Field &in_y, const Field &in_z, bool sync) {
+    k::divergence(m_domain.stream(), m_domain.level(0), out.data, in_x.data, in_y.data, in_z.data);
+    maybe_sync(m_domain, sync);
+}
+
+// IPressure.cpp:125-172
+void IPressure::projection(Field &out_u, Field &out_v, Field &out_w, const Field &in_u, const Field &in_v,
+                           const Field &in_w, const Field &in_p, bool sync) {
+    k::projection(m_domain.stream(), m_domain.level(0), out_u.data, out_v.data, out_w.data, in_u.data, in_v.data,
+                  in_w.data, in_p.data);
+    m_domain.apply_boundary(out_u.data, out_u.get_type(), 0);
+    m_domain.apply_boundary(out_v.data, out_v.get_type(), 0);
+    m_domain.apply_boundary(out_w.data, out_w.get_type(), 0);
+    maybe_sync(m_domain, sync);
+}
+
+// ISource.cpp:26-52
+void ISource::buoyancy_force(Field &out, const Field &in, const Field &in_a, double beta, double g, bool sync) {
+    Domain &d = out.domain();
+    k::buoyancy(d.stream(), d.level(0), out.data, in.data, in_a.data, beta, g);
+    maybe_sync(d, sync);
+}
+
+void SLAdvect::advect(Field &out, const Field &in, const Field &u, const Field &v, const Field &w, bool sync) {
+    k::sl_advect(m_domain.stream(), m_domain.level(0), m_domain.settings().dt, out.data, in.data, u.data, v.data,
+                 w.data);
+    m_domain.apply_boundary(out.data, out.get_type(), 0);
+    maybe_sync(m_domain, sync);
+}
+
+// JacobiDiffuse.cpp:31-110 (EV == nullptr) and :123-203.
+// Iterate while the update size exceeds tol_res; `in` follows `out` on the
+// domain list after every sweep, its obstacle cells keep their entry values.
+void JacobiDiffuse::run(Field &out, const Field &in, const Field &b, double D, const Field *EV) {
+    Domain &dm = m_domain;
+    const LevelDesc &L = dm.level(0);
+    const double dt = dm.settings().dt;
+    const double rdx = 1. / L.h[0], rdy = 1. / L.h[1], rdz = 1. / L.h[2];
+    const double ax = D * dt * rdx * rdx, ay = D * dt * rdy * rdy, az = D * dt * rdz * rdz;
+    const double rb = (1. + 2. * (ax + ay + az));
+    const int n_part = k::partial_count(L);
+    int it = 0;
+    double res = 1.;
+    while (res > m_settings.tol_res && it < m_settings.max_iter) {
+        k::jacobi_diffuse_step(dm.stream(), L, out.data, in.data, b.data, ax, ay, az, rb, D,
+                               EV ? EV->data : nullptr, dt, m_settings.w, dm.partial_buffer());
+        dm.apply_boundary(out.data, out.get_type(), 0);
+        res = std::sqrt(dm.reduce(dm.partial_buffer(), n_part));
+        it++;
+        k::copy_domain(dm.stream(), L, in.data, out.data);
+    }
+    if (it % 2 != 0) k::copy_domain(dm.stream(), L, in.data, out.data);
+    m_iterations = it;
+    m_residual = res;
+}
+
+void JacobiDiffuse::diffuse(Field &out, const Field &in, const Field &b, double D, bool sync) {
+    run(out, in, b, D, nullptr);
+    maybe_sync(m_domain, sync);
+}
+void JacobiDiffuse::diffuse(Field &out, const Field &in, const Field &b, double D, const Field &EV, bool sync) {
+    run(out, in, b, D, &EV);
+    maybe_sync(m_domain, sync);
+}
+
+// ColoredGaussSeidelDiffuse.cpp:38-173: red then black in place on `out`,
+// `in` holds the previous iterate (whole-array copy) for the update norm.
+void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b, double D, const Field *EV) {
+    Domain &dm = m_domain;
+    const LevelDesc &L = dm.level(0);
+    const double dt = dm.settings().dt;
+    const double rdx = 1. / L.h[0], rdy = 1. / L.h[1], rdz = 1. / L.h[2];
+    const double ax = D * dt * rdx * rdx, ay = D * dt * rdy * rdy, az = D * dt * rdz * rdz;
+    const double rb = (1. + 2. * (ax + ay + az));
+    const double beta = 1. / rb;
+    const double *ev = EV ? EV->data : nullptr;
+    const int n_part = k::partial_count(L);
+    out.copy_data(in);
+    int it = 0;
+    double res = 1.;
+    while (res > m_settings.tol_res && it < m_settings.max_iter) {
+        in.copy_data(out);
+        for (int color = 0; color < 2; color++)
+            k::cgs_color(dm.stream(), L, color, out.data, b.data, ax, ay, az, beta, 1.0, m_settings.w, D, ev, dt);
+        dm.apply_boundary(out.data, out.get_type(), 0);
+        k::update_norm_partials(dm.stream(), L, out.data, in.data, rb, D, ev, dt, dm.partial_buffer());
+        res = std::sqrt(dm.reduce(dm.partial_buffer(), n_part));
+        it++;
+    }
+    m_iterations = it;
+    m_residual = res;
+}
+
+void ColoredGaussSeidelDiffuse::diffuse(Field &out, const Field &in, const Field &b, double D, bool sync) {
+    run(out, in, b, D, nullptr);
+    maybe_sync(m_domain, sync);
+}
+void ColoredGaussSeidelDiffuse::diffuse(Field &out, const Field &in, const Field &b, double D, const Field &EV,
+                                        bool sync) {
+    run(out, in, b, D, &EV);
+    maybe_sync(m_domain, sync);
+}
+
+// ConstSmagorinsky.cpp:30-87; Cs*Cs*delta*delta is folded on the host in the
+// reference's left-to-right order before the per-cell multiply by S_bar.
+void ConstSmagorinsky::calc_turbulent_viscosity(Field &ev, const Field &u, const Field &v, const Field &w,
+                                                bool sync) {
+    const LevelDesc &L = m_domain.level(0);
+    const double delta_s = std::cbrt(L.h[0] * L.h[1] * L.h[2]);
+    const double cdd = m_cs * m_cs * delta_s * delta_s;
+    k::smagorinsky(m_domain.stream(), L, ev.data, u.data, v.data, w.data, cdd);
+    m_domain.apply_boundary(ev.data, ev.get_type(), 0);
+    maybe_sync(m_domain, sync);
+}
+
+void ExplicitEulerSource::add_source(Field &out_x, Field &out_y, Field &out_z, const Field &S_x, const Field &S_y,
+                                     const Field &S_z, bool sync) {
+    const double dt = m_domain.settings().dt;
+    Field *o[3] = {&out_x, &out_y, &out_z};
+    const Field *s[3] = {&S_x, &S_y, &S_z};
+    for (int a = 0; a < 3; a++) {
+        bool on = false;
+        for (int d : m_dir) on = on || d == a;
+        if (!on) continue;
+        k::add_source(m_domain.stream(), m_domain.level(0), o[a]->data, s[a]->data, dt);
+        m_domain.apply_boundary(o[a]->data, o[a]->get_type(), 0);
+    }
+    maybe_sync(m_domain, sync);
+}
+
+void ExplicitEulerSource::add_source(Field &out, const Field &S, bool sync) {
+    k::add_source(m_domain.stream(), m_domain.level(0), out.data, S.data, m_domain.settings().dt);
+    m_domain.apply_boundary(out.data, out.get_type(), 0);
+    maybe_sync(m_domain, sync);
+}
+
+GaussFunction::GaussFunction(Domain &d, const GaussSettings &g)
+    : m_domain(d), m_settings(g), m_spatial(d, RHO) {
+    const std::vector<double> host = d.gauss_spatial(g);
+    ARTSS_HIP(hipMemcpyAsync(m_spatial.data, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice,
+                             d.stream()));
+    ARTSS_HIP(hipStreamSynchronize(d.stream()));
+}
+
+// GaussFunction.cpp:27-40, 121-123: out = spatial * tanh(t / tau)
+void GaussFunction::update_source(Field &out, double t_cur) {
+    const double tv = std::tanh(t_cur / m_settings.tau);
+    k::scale_to(m_domain.stream(), out.data, m_spatial.data, tv, out.get_size());
+}
+
+std::unique_ptr<IAdvection> make_advection(Domain &d, const std::string &type) {
+    if (type == "SemiLagrangian") return std::make_unique<SLAdvect>(d);
+    throw config_error("advection method '" + type + "' is not implemented");
+}
+
+std::unique_ptr<IDiffusion> make_diffusion(Domain &d, const DiffusionSettings &s) {
+    if (s.type == "Jacobi") return std::make_unique<JacobiDiffuse>(d, s);
+    if (s.type == "ColoredGaussSeidel") return std::make_unique<ColoredGaussSeidelDiffuse>(d, s);
+    throw config_error("diffusion method '" + s.type + "' is not implemented");
+}
+
+std::unique_ptr<ITurbulence> make_turbulence(Domain &d, const Settings &s) {
+    if (s.turbulence_type == "ConstSmagorinsky") return std::make_unique<ConstSmagorinsky>(d, s.cs);
+    throw config_error("turbulence method '" + s.turbulence_type + "' is not implemented");
+}
+
+}  // namespace artss

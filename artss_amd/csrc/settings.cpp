@@ -1,0 +1,289 @@
+#include "settings.h"
+
+#include <cerrno>
+#include <cstdlib>
+#include <memory>
+
+#include "xml.h"
+
+namespace artss {
+namespace {
+
+using Map = std::map<std::string, std::string>;
+
+// Settings.cpp map_parameter_section / map_parameter_line: attributes plus
+// the text of each child element.
+Map values_of(const xml::Element *e) {
+    Map m;
+    if (!e) return m;
+    for (const auto &a : e->attributes) m[a.first] = a.second;
+    for (const auto &c : e->children)
+        if (c->has_text) m[c->name] = c->text;
+    return m;
+}
+
+const xml::Element *section(const xml::Element *head, const std::string &name, const std::string &ctx) {
+    const xml::Element *e = head ? head->child(name) : nullptr;
+    if (!e) throw config_error("section " + ctx + "/" + name + " is required.");
+    return e;
+}
+
+double to_real(const std::string &s, const std::string &key, const std::string &ctx) {
+    errno = 0;
+    char *end = nullptr;
+    double v = std::strtod(s.c_str(), &end);
+    if (end == s.c_str() || errno == ERANGE) throw config_error("Value " + key + " is for " + ctx + " required.");
+    return v;
+}
+long to_int(const std::string &s, const std::string &key, const std::string &ctx) {
+    errno = 0;
+    char *end = nullptr;
+    long v = std::strtol(s.c_str(), &end, 10);
+    if (end == s.c_str() || errno == ERANGE) throw config_error("Value " + key + " is for " + ctx + " required.");
+    return v;
+}
+double req_real(const Map &m, const std::string &k, const std::string &ctx) {
+    auto it = m.find(k);
+    if (it == m.end()) throw config_error("Value " + k + " is for " + ctx + " required.");
+    return to_real(it->second, k, ctx);
+}
+double opt_real(const Map &m, const std::string &k, double dflt) {
+    auto it = m.find(k);
+    return it == m.end() ? dflt : to_real(it->second, k, "");
+}
+int req_int(const Map &m, const std::string &k, const std::string &ctx) {
+    auto it = m.find(k);
+    if (it == m.end()) throw config_error("Value " + k + " is for " + ctx + " required.");
+    return static_cast<int>(to_int(it->second, k, ctx));
+}
+std::string req_str(const Map &m, const std::string &k, const std::string &ctx) {
+    auto it = m.find(k);
+    if (it == m.end()) throw config_error("Value " + k + " is for " + ctx + " required.");
+    return it->second;
+}
+bool req_bool(const Map &m, const std::string &k, const std::string &ctx) { return req_str(m, k, ctx) == "Yes"; }
+bool opt_bool(const Map &m, const std::string &k, bool dflt) {
+    auto it = m.find(k);
+    return it == m.end() ? dflt : it->second == "Yes";
+}
+
+std::vector<std::string> split(const std::string &s, char d) {
+    std::vector<std::string> out;
+    size_t p = 0;
+    for (;;) {
+        size_t e = s.find(d, p);
+        out.push_back(s.substr(p, e == std::string::npos ? std::string::npos : e - p));
+        if (e == std::string::npos) break;
+        p = e + 1;
+    }
+    return out;
+}
+
+int match(const std::vector<std::string> &names, const std::string &s, const std::string &what) {
+    for (size_t i = 0; i < names.size(); i++)
+        if (names[i] == s) return static_cast<int>(i);
+    throw config_error("unknown " + what + " '" + s + "'");
+}
+const std::vector<std::string> kFieldNames = {"rho", "u", "v", "w", "p", "T", "nu"};
+const std::vector<std::string> kPatchNames = {"left", "right", "bottom", "top", "front", "back"};
+const std::vector<std::string> kBCNames = {"neumann", "dirichlet", "periodic"};
+const std::vector<std::string> kAxisNames = {"x", "y", "z"};
+
+std::vector<int> axes(const std::string &s) {
+    std::vector<int> out;
+    for (const auto &a : split(s, ',')) out.push_back(match(kAxisNames, a, "axis"));
+    return out;
+}
+
+// Settings.cpp:372-389
+BoundarySpec parse_boundary(const xml::Element *e, const std::string &ctx) {
+    Map v = values_of(e);
+    BoundarySpec b;
+    b.bc = match(kBCNames, req_str(v, "type", ctx), "boundary condition");
+    if (b.bc != PERIODIC) b.value = req_real(v, "value", ctx);
+    for (const auto &f : split(req_str(v, "field", ctx), ',')) b.fields.push_back(match(kFieldNames, f, "field"));
+    for (const auto &p : split(req_str(v, "patch", ctx), ',')) b.patches.push_back(match(kPatchNames, p, "patch"));
+    return b;
+}
+
+// Settings.cpp:613-635
+DiffusionSettings parse_diffusion(const xml::Element *head, const std::string &ctx) {
+    const xml::Element *e = section(head, "diffusion", ctx);
+    Map v = values_of(e);
+    DiffusionSettings d;
+    d.type = req_str(v, "type", ctx + "/diffusion");
+    if (d.type == "Jacobi" || d.type == "ColoredGaussSeidel") {
+        d.w = req_real(v, "w", ctx);
+        d.tol_res = req_real(v, "tol_res", ctx);
+        d.max_iter = req_int(v, "max_iter", ctx);
+    } else if (d.type != "Explicit") {
+        throw config_error("diffusion solver '" + d.type + "' has no parsing implementation.");
+    }
+    return d;
+}
+
+}  // namespace
+
+Settings parse_settings(const std::string &xml_text) {
+    std::unique_ptr<xml::Element> root;
+    try {
+        root = xml::parse(xml_text);
+    } catch (const xml::parse_error &e) {
+        throw config_error(e.what());
+    }
+    Settings s;
+    const xml::Element *solver = section(root.get(), "solver", "");
+    Map sv = values_of(solver);
+    s.solver = req_str(sv, "description", "solver");
+    const std::string &d = s.solver;
+    const bool ns = d.find("NS") != std::string::npos;
+
+    // physical_parameters, Settings.cpp:535-570
+    Map pv = values_of(section(root.get(), "physical_parameters", ""));
+    s.t_end = req_real(pv, "t_end", "physical_parameters");
+    s.dt = req_real(pv, "dt", "physical_parameters");
+    if (d != "AdvectionSolver" && d != "PressureSolver") {
+        s.nu = req_real(pv, "nu", "physical_parameters");
+        s.has_nu = true;
+    }
+    s.beta = opt_real(pv, "beta", 3.34e-3);
+    s.g = opt_real(pv, "g", -9.81);
+    if (d == "NSTempSolver" || d == "NSTempTurbSolver" || d == "NSTempTurbConSolver" || d == "NSTempConSolver") {
+        s.kappa = req_real(pv, "kappa", "physical_parameters");
+        s.has_kappa = true;
+    }
+
+    // domain_parameters, Settings.cpp:126-148
+    Map dv = values_of(section(root.get(), "domain_parameters", ""));
+    if (opt_bool(dv, "enable_computational_domain", false))
+        throw config_error("a computational domain smaller than the physical domain is not supported");
+    s.X[0] = req_real(dv, "X1", "domain_parameters");
+    s.X[1] = req_real(dv, "X2", "domain_parameters");
+    s.Y[0] = req_real(dv, "Y1", "domain_parameters");
+    s.Y[1] = req_real(dv, "Y2", "domain_parameters");
+    s.Z[0] = req_real(dv, "Z1", "domain_parameters");
+    s.Z[1] = req_real(dv, "Z2", "domain_parameters");
+    s.n[0] = req_int(dv, "nx", "domain_parameters");
+    s.n[1] = req_int(dv, "ny", "domain_parameters");
+    s.n[2] = req_int(dv, "nz", "domain_parameters");
+    for (int a = 0; a < 3; a++)
+        if (s.n[a] < 1) throw config_error("domain needs at least one inner cell per axis");
+
+    // solver sub-sections, Settings.cpp:905-970
+    const bool wants_adv = d != "DiffusionSolver" && d != "DiffusionTurbSolver" && d != "PressureSolver";
+    const bool wants_dif = d != "AdvectionSolver" && d != "PressureSolver";
+    if (wants_adv) {
+        Map av = values_of(section(solver, "advection", "solver"));
+        s.advection_type = req_str(av, "type", "solver/advection");
+        s.has_advection = true;
+        if (s.advection_type != "SemiLagrangian")
+            throw config_error("advection solver '" + s.advection_type + "' has no parsing implementation.");
+    }
+    if (wants_dif) {
+        s.diffusion = parse_diffusion(solver, "solver");
+        s.has_diffusion = true;
+    }
+    if (d == "DiffusionTurbSolver" || d == "NSTurbSolver" || d == "NSTempTurbSolver" || d == "NSTempTurbConSolver") {
+        Map tv = values_of(section(solver, "turbulence", "solver"));
+        s.turbulence_type = req_str(tv, "type", "solver/turbulence");
+        s.has_turbulence = true;
+        if (s.turbulence_type == "ConstSmagorinsky") s.cs = req_real(tv, "Cs", "solver/turbulence");
+        else if (s.turbulence_type != "DynamicSmagorinsky")
+            throw config_error("turbulence solver '" + s.turbulence_type + "' has no parsing implementation.");
+    }
+    if (ns) {
+        Map srv = values_of(section(solver, "source", "solver"));
+        s.has_source = true;
+        s.source_type = req_str(srv, "type", "solver/source");
+        s.force_fct = req_str(srv, "force_fct", "solver/source");
+        if (s.force_fct == "Buoyancy") {
+            s.use_init_values = req_bool(srv, "use_init_values", "solver/source");
+            if (!s.use_init_values)
+                s.ambient_temperature = req_real(srv, "ambient_temperature_value", "solver/source");
+        } else if (s.force_fct != "Zero" && s.force_fct != "Uniform") {
+            throw config_error("source type '" + s.source_type + "' has no parsing implementation.");
+        }
+        s.force_dir = axes(req_str(srv, "dir", "solver/source"));
+    }
+    if (ns || d == "PressureSolver") {
+        const xml::Element *pe = section(solver, "pressure", "solver");
+        Map prv = values_of(pe);
+        if (req_str(prv, "type", "solver/pressure") != "VCycleMG")
+            throw config_error("pressure solver has no parsing implementation.");
+        s.has_pressure = true;
+        s.pressure.n_level = req_int(prv, "n_level", "solver/pressure");
+        s.pressure.n_cycle = req_int(prv, "n_cycle", "solver/pressure");
+        s.pressure.max_cycle = req_int(prv, "max_cycle", "solver/pressure");
+        s.pressure.n_relax = req_int(prv, "n_relax", "solver/pressure");
+        s.pressure.tol_res = req_real(prv, "tol_res", "solver/pressure");
+        s.pressure.smoother = parse_diffusion(pe, "solver/pressure");
+        s.levels = s.pressure.n_level;
+    }
+    if (d == "NSTempSolver" || d == "NSTempTurbSolver" || d == "NSTempConSolver" || d == "NSTempTurbConSolver") {
+        const xml::Element *te = section(solver, "temperature", "solver");
+        s.has_temperature = true;
+        Map tav = values_of(section(te, "advection", "solver/temperature"));
+        s.temp_advection_type = req_str(tav, "type", "solver/temperature/advection");
+        s.temp_diffusion = parse_diffusion(te, "solver/temperature");
+        if (d == "NSTempTurbSolver" || d == "NSTempTurbConSolver") {
+            Map ttv = values_of(section(te, "turbulence", "solver/temperature"));
+            s.temp_turbulence = req_bool(ttv, "include", "solver/temperature/turbulence");
+            if (s.temp_turbulence) s.prandtl = req_real(ttv, "Pr_T", "solver/temperature/turbulence");
+        }
+        Map tsv = values_of(section(te, "source", "solver/temperature"));
+        s.temp_source_type = req_str(tsv, "type", "solver/temperature/source");
+        s.temp_dir = axes(req_str(tsv, "dir", "solver/temperature/source"));
+        s.dissipation = req_bool(tsv, "dissipation", "solver/temperature/source");
+        s.temp_random = req_bool(tsv, "random", "solver/temperature/source");
+        s.temp_fct = req_str(tsv, "temp_fct", "solver/temperature/source");
+        if (s.temp_fct == "Gauss") {
+            const std::string c = "solver/temperature/source/Gauss";
+            s.gauss.hrr = req_real(tsv, "HRR", c);
+            s.gauss.cp = req_real(tsv, "cp", c);
+            const char *ax[3] = {"x", "y", "z"};
+            for (int a = 0; a < 3; a++) {
+                s.gauss.pos[a] = req_real(tsv, std::string(ax[a]) + "0", c);
+                s.gauss.sigma[a] = req_real(tsv, std::string("sigma_") + ax[a], c);
+            }
+            s.gauss.tau = req_real(tsv, "tau", c);
+        } else if (s.temp_fct != "Zero") {
+            throw config_error("temperature source function '" + s.temp_fct + "' has no implementation here.");
+        }
+    }
+
+    // boundaries, Settings.cpp:391-402
+    const xml::Element *be = section(root.get(), "boundaries", "");
+    for (const auto &c : be->children)
+        if (c->name == "boundary") s.boundaries.push_back(parse_boundary(c.get(), "boundaries"));
+
+    // obstacles, Settings.cpp:404-447
+    const xml::Element *oe = root->child("obstacles");
+    if (oe) {
+        Map ov = values_of(oe);
+        if (req_bool(ov, "enabled", "obstacles")) {
+            for (const auto &c : oe->children) {
+                ObstacleSpec o;
+                Map onv = values_of(c.get());
+                o.name = req_str(onv, "name", "obstacles/obstacle");
+                Map gv = values_of(section(c.get(), "geometry", "obstacles/obstacle"));
+                const char *keys[6] = {"ox1", "ox2", "oy1", "oy2", "oz1", "oz2"};
+                for (int k = 0; k < 6; k++) o.box[k] = req_real(gv, keys[k], "obstacles/obstacle/geometry");
+                for (const auto &b : c->children)
+                    if (b->name == "boundary") o.boundaries.push_back(parse_boundary(b.get(), "obstacle"));
+                s.obstacles.push_back(o);
+            }
+        }
+    }
+    const xml::Element *se = root->child("surfaces");
+    if (se && opt_bool(values_of(se), "enabled", false))
+        throw config_error("surfaces are not supported (their reference list builder never terminates)");
+
+    // initial conditions, Settings.cpp:330-370
+    Map iv = values_of(section(root.get(), "initial_conditions", ""));
+    s.ic = req_str(iv, "usr_fct", "initial_conditions");
+    s.ic_random = req_bool(iv, "random", "initial_conditions");
+    s.ic_values = iv;
+    return s;
+}
+
+}  // namespace artss

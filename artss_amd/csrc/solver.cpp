@@ -1,0 +1,302 @@
+#include "solver.h"
+
+#include <cmath>
+
+namespace artss {
+
+FieldController::FieldController(Domain &d)
+    : u(d, U), v(d, V), w(d, W), u0(d, U), v0(d, V), w0(d, W), u_tmp(d, U), v_tmp(d, V), w_tmp(d, W),
+      nu_t(d, NU), kappa_t(d, artss::T), p(d, P), rhs(d, P), T(d, artss::T), T0(d, artss::T), T_tmp(d, artss::T),
+      T_ambient(d, artss::T), f_x(d, U), f_y(d, V), f_z(d, W), S_T(d, artss::T) {}
+
+void FieldController::couple_vector() {
+    u0.copy_data(u);
+    u_tmp.copy_data(u);
+    v0.copy_data(v);
+    v_tmp.copy_data(v);
+    w0.copy_data(w);
+    w_tmp.copy_data(w);
+}
+
+void FieldController::couple_scalar() {
+    T0.copy_data(T);
+    T_tmp.copy_data(T);
+}
+
+void FieldController::update_data() {
+    u0.copy_data(u);
+    v0.copy_data(v);
+    w0.copy_data(w);
+    u_tmp.copy_data(u);
+    v_tmp.copy_data(v);
+    w_tmp.copy_data(w);
+    T0.copy_data(T);
+    T_tmp.copy_data(T);
+}
+
+Field *FieldController::by_name(const std::string &n) {
+    if (n == "u") return &u;
+    if (n == "v") return &v;
+    if (n == "w") return &w;
+    if (n == "u0") return &u0;
+    if (n == "v0") return &v0;
+    if (n == "w0") return &w0;
+    if (n == "u_tmp") return &u_tmp;
+    if (n == "v_tmp") return &v_tmp;
+    if (n == "w_tmp") return &w_tmp;
+    if (n == "p") return &p;
+    if (n == "rhs") return &rhs;
+    if (n == "T") return &T;
+    if (n == "T0") return &T0;
+    if (n == "T_tmp") return &T_tmp;
+    if (n == "T_ambient") return &T_ambient;
+    if (n == "nu_t") return &nu_t;
+    if (n == "kappa_t") return &kappa_t;
+    if (n == "f_x") return &f_x;
+    if (n == "f_y") return &f_y;
+    if (n == "f_z") return &f_z;
+    if (n == "S_T") return &S_T;
+    return nullptr;
+}
+
+// Functions.cpp initial conditions, evaluated with libm on the host on the
+// inner list; "TaylorGreen" has no reference counterpart (u = sin2πx cos2πy
+// cos2πz, v = -cos2πx sin2πy cos2πz, w = 0).  Returns a packed array of
+// three fields for TaylorGreen, one field otherwise.
+std::vector<double> initial_condition(const Domain &d, const std::string &what) {
+    const Settings &s = d.settings();
+    const LevelDesc &L = d.level(0);
+    const size_t n = static_cast<size_t>(L.size);
+    const std::vector<uint64_t> inner = d.inner_list(0);
+    const double X1 = s.X[0], Y1 = s.Y[0], Z1 = s.Z[0];
+    const double dx = L.h[0], dy = L.h[1], dz = L.h[2];
+    auto coords = [&](uint64_t idx, size_t &i, size_t &j, size_t &k) {
+        k = idx / (static_cast<size_t>(L.N[0]) * L.N[1]);
+        j = (idx - k * L.N[0] * L.N[1]) / L.N[0];
+        i = idx - k * L.N[0] * L.N[1] - j * L.N[0];
+    };
+    if (what == "TaylorGreen") {
+        std::vector<double> out(3 * n, 0.0);
+        for (uint64_t idx : inner) {
+            size_t i, j, k;
+            coords(idx, i, j, k);
+            const double x = X1 + (static_cast<double>(i) - 0.5) * dx;
+            const double y = Y1 + (static_cast<double>(j) - 0.5) * dy;
+            const double z = Z1 + (static_cast<double>(k) - 0.5) * dz;
+            out[idx] = std::sin(2. * M_PI * x) * std::cos(2. * M_PI * y) * std::cos(2. * M_PI * z);
+            out[n + idx] = -std::cos(2. * M_PI * x) * std::sin(2. * M_PI * y) * std::cos(2. * M_PI * z);
+            out[2 * n + idx] = 0.;
+        }
+        return out;
+    }
+    std::vector<double> out(n, 0.0);
+    if (what == "Uniform") {  // Functions.cpp:820-832
+        auto it = s.ic_values.find("val");
+        if (it == s.ic_values.end()) throw config_error("Uniform initial condition needs <val>");
+        const double val = std::strtod(it->second.c_str(), nullptr);
+        for (uint64_t idx : inner) out[idx] = val;
+    } else if (what == "SinSinSin") {  // Functions.cpp:781-811
+        auto it = s.ic_values.find("l");
+        if (it == s.ic_values.end()) throw config_error("SinSinSin initial condition needs <l>");
+        const double l = std::strtod(it->second.c_str(), nullptr);
+        for (uint64_t idx : inner) {
+            size_t i, j, k;
+            coords(idx, i, j, k);
+            out[idx] = std::sin(l * M_PI * (X1 + (static_cast<double>(i) - 0.5) * dx)) *
+                       std::sin(l * M_PI * (Y1 + (static_cast<double>(j) - 0.5) * dy)) *
+                       std::sin(l * M_PI * (Z1 + (static_cast<double>(k) - 0.5) * dz));
+        }
+    }
+    return out;
+}
+
+namespace {
+void upload(Domain &d, double *dst, const double *src, size_t n) {
+    ARTSS_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, d.stream()));
+}
+}  // namespace
+
+Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
+    Domain &d = *m_domain;
+    const Settings &s = d.settings();
+    if (s.solver == "NSSolver") m_kind = 0;
+    else if (s.solver == "NSTempTurbSolver") m_kind = 1;
+    else if (s.solver == "PressureSolver") m_kind = 2;
+    else throw config_error("solver '" + s.solver + "' is not part of this build (NSSolver, NSTempTurbSolver, PressureSolver)");
+    if (s.has_source && s.force_fct == "Buoyancy" && s.use_init_values)
+        throw config_error("buoyancy with use_init_values is not supported");
+    if (s.has_source && s.force_fct == "Uniform") throw config_error("uniform momentum source is not supported");
+    if (s.has_temperature && (s.dissipation || s.temp_random))
+        throw config_error("dissipation / random temperature source are not supported");
+    if (s.ic_random) throw config_error("random initial conditions are not supported");
+
+    m_fields = std::make_unique<FieldController>(d);
+    // NSSolver / NSTempTurbSolver constructors (SolverSelection::set_*)
+    if (m_kind != 2) {
+        m_adv = make_advection(d, s.advection_type);
+        m_dif = make_diffusion(d, s.diffusion);
+        m_src_vel = std::make_unique<ExplicitEulerSource>(d, s.force_dir);
+        m_add_source = s.force_fct != "Zero";
+    }
+    m_pres = std::make_unique<VCycleMG>(d, s.pressure);
+    if (m_kind == 1) {
+        if (s.temp_advection_type != "SemiLagrangian") throw config_error("temperature advection must be SemiLagrangian");
+        m_turb = make_turbulence(d, s);
+        m_dif_temp = make_diffusion(d, s.diffusion);  // NSTempTurbSolver.cpp:50 uses the velocity settings
+        m_src_temp = std::make_unique<ExplicitEulerSource>(d, s.temp_dir);
+        if (s.temp_fct == "Gauss") m_gauss = std::make_unique<GaussFunction>(d, s.gauss);
+        m_add_temp_source = s.temp_fct != "Zero";
+    }
+    set_up_fields();
+}
+
+Solver::~Solver() {
+    if (m_domain) (void)hipStreamSynchronize(m_domain->stream());
+}
+
+// SolverController.cpp:26-43, 95-205
+void Solver::set_up_fields() {
+    Domain &d = *m_domain;
+    const Settings &s = d.settings();
+    FieldController &f = *m_fields;
+    const size_t n = static_cast<size_t>(d.level(0).size);
+    if (s.ic == "TaylorGreen") {
+        std::vector<double> ic = initial_condition(d, "TaylorGreen");
+        upload(d, f.u.data, ic.data(), n);
+        upload(d, f.v.data, ic.data() + n, n);
+        upload(d, f.w.data, ic.data() + 2 * n, n);
+        ARTSS_HIP(hipStreamSynchronize(d.stream()));
+    } else if (s.ic == "Uniform") {
+        std::vector<double> ic = initial_condition(d, "Uniform");
+        upload(d, f.T.data, ic.data(), n);
+        ARTSS_HIP(hipStreamSynchronize(d.stream()));
+    } else if (s.ic == "SinSinSin") {
+        std::vector<double> ic = initial_condition(d, "SinSinSin");
+        f.p.set_value(0.);
+        upload(d, f.rhs.data, ic.data(), n);
+        ARTSS_HIP(hipStreamSynchronize(d.stream()));
+    } else if (s.ic != "Zero") {
+        throw config_error("initial condition '" + s.ic + "' is not supported");
+    }
+    if (m_kind == 1 && s.force_fct == "Buoyancy") f.T_ambient.set_value(s.ambient_temperature);  // force_source
+    // FieldController::set_up_boundary, FieldController.cpp:101-112
+    d.apply_boundary(f.u.data, U, 0);
+    d.apply_boundary(f.v.data, V, 0);
+    d.apply_boundary(f.w.data, W, 0);
+    d.apply_boundary(f.p.data, P, 0);
+    d.apply_boundary(f.T.data, artss::T, 0);
+    d.apply_boundary(f.T_ambient.data, artss::T, 0);
+    update_sources(0);
+    f.update_data();
+    ARTSS_HIP(hipStreamSynchronize(d.stream()));
+}
+
+// SolverController::update_sources, SolverController.cpp:263-307
+void Solver::update_sources(double t) {
+    Domain &d = *m_domain;
+    const Settings &s = d.settings();
+    FieldController &f = *m_fields;
+    if (m_kind != 2 && s.force_fct == "Buoyancy") {
+        Field *out[3] = {&f.f_x, &f.f_y, &f.f_z};
+        for (int a : s.force_dir) ISource::buoyancy_force(*out[a], f.T, f.T_ambient, s.beta, s.g, false);
+    }
+    if (m_gauss) m_gauss->update_source(f.S_T, t);
+}
+
+void Solver::pressure_step(double t) {
+    FieldController &f = *m_fields;
+    if (!m_pmode.to_tolerance) {
+        m_pres->pressure(f.p, f.rhs, t, false);
+        m_counters.vcycles = m_pres->last_cycles();
+        return;
+    }
+    double r = m_pres->residual(f.p, f.rhs);
+    int cycles = 0;
+    while (r > m_pmode.tolerance && cycles < m_pmode.max_cycles) {
+        m_pres->vcycle(f.p, f.rhs);
+        cycles++;
+        r = m_pres->residual(f.p, f.rhs);
+    }
+    m_counters.vcycles = cycles;
+    m_counters.residual = r;
+}
+
+// NSSolver.cpp:50-124
+void Solver::do_step_ns(double t) {
+    FieldController &f = *m_fields;
+    const double nu = m_domain->settings().nu;
+    m_adv->advect(f.u, f.u0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.v, f.v0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.w, f.w0, f.u0, f.v0, f.w0, false);
+    f.couple_vector();
+    if (nu != 0.) {
+        m_dif->diffuse(f.u, f.u0, f.u_tmp, nu, false);
+        m_counters.diffusion[0] = m_dif->last_iterations();
+        m_dif->diffuse(f.v, f.v0, f.v_tmp, nu, false);
+        m_counters.diffusion[1] = m_dif->last_iterations();
+        m_dif->diffuse(f.w, f.w0, f.w_tmp, nu, false);
+        m_counters.diffusion[2] = m_dif->last_iterations();
+        f.couple_vector();
+    }
+    if (m_add_source) {
+        m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
+        f.couple_vector();
+    }
+    m_pres->divergence(f.rhs, f.u_tmp, f.v_tmp, f.w_tmp, false);
+    pressure_step(t);
+    m_pres->projection(f.u, f.v, f.w, f.u_tmp, f.v_tmp, f.w_tmp, f.p, false);
+}
+
+// NSTempTurbSolver.cpp:94-248
+void Solver::do_step_nstempturb(double t) {
+    Domain &d = *m_domain;
+    const Settings &s = d.settings();
+    FieldController &f = *m_fields;
+    m_adv->advect(f.u, f.u0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.v, f.v0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.w, f.w0, f.u0, f.v0, f.w0, false);
+    f.couple_vector();
+    m_turb->calc_turbulent_viscosity(f.nu_t, f.u, f.v, f.w, false);
+    m_dif->diffuse(f.u, f.u0, f.u_tmp, s.nu, f.nu_t, false);
+    m_counters.diffusion[0] = m_dif->last_iterations();
+    m_dif->diffuse(f.v, f.v0, f.v_tmp, s.nu, f.nu_t, false);
+    m_counters.diffusion[1] = m_dif->last_iterations();
+    m_dif->diffuse(f.w, f.w0, f.w_tmp, s.nu, f.nu_t, false);
+    m_counters.diffusion[2] = m_dif->last_iterations();
+    f.couple_vector();
+    if (m_add_source) {
+        m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
+        f.couple_vector();
+    }
+    m_pres->divergence(f.rhs, f.u_tmp, f.v_tmp, f.w_tmp, false);
+    pressure_step(t);
+    m_pres->projection(f.u, f.v, f.w, f.u_tmp, f.v_tmp, f.w_tmp, f.p, false);
+    m_adv->advect(f.T, f.T0, f.u, f.v, f.w, false);
+    f.couple_scalar();
+    if (s.temp_turbulence) {
+        const double rPr_T = 1. / s.prandtl;
+        k::scale_to(d.stream(), f.kappa_t.data, f.nu_t.data, rPr_T, f.kappa_t.get_size());
+        m_dif_temp->diffuse(f.T, f.T0, f.T_tmp, s.kappa, f.kappa_t, false);
+        m_counters.diffusion[3] = m_dif_temp->last_iterations();
+        f.couple_scalar();
+    } else if (s.kappa != 0.) {
+        m_dif_temp->diffuse(f.T, f.T0, f.T_tmp, s.kappa, false);
+        m_counters.diffusion[3] = m_dif_temp->last_iterations();
+        f.couple_scalar();
+    }
+    if (m_add_temp_source) {
+        m_src_temp->add_source(f.T, f.S_T, false);
+        f.couple_scalar();
+    }
+}
+
+void Solver::step(double t) {
+    m_counters = StepCounters{};
+    if (m_kind == 0) do_step_ns(t);
+    else if (m_kind == 1) do_step_nstempturb(t);
+    else pressure_step(t);
+    update_sources(t);
+    m_fields->update_data();
+}
+
+}  // namespace artss

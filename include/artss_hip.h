@@ -1,0 +1,135 @@
+/*
+ * artss_hip.h - C ABI of the MI355X (gfx950) implementation of the ARTSS
+ * solver hot path.
+ *
+ * The reference (FireDynamics/ARTSS) has no foreign-function interface: its
+ * operators are C++ classes called through IAdvection, IDiffusion,
+ * IPressure, ITurbulence and ISource from the solver classes.  Each entry
+ * point below is what a drop-in for one of those calls binds; the comment
+ * names the reference interface it replaces (file:line in the reference's
+ * src/).  INTEGRATION.md shows the reference-side adapter classes.
+ *
+ * Conventions
+ *  - Fields are float64 device arrays (HBM) in the reference layout: x
+ *    fastest, idx = i + Nx*j + Nx*Ny*k, one ghost layer
+ *    (utility/GlobalMacrosTypes.h:70).  Sizes come from artss_domain_level_info.
+ *  - field_type is the reference's FieldType (utility/Mapping.h:48-51):
+ *    RHO 0, U 1, V 2, W 3, P 4, T 5, NU 6; it selects the boundary data.
+ *  - Every call returns 0 on success or an ARTSS_ERR_* code, with a message
+ *    in artss_last_error() (per thread).  Operator calls are complete on the
+ *    device when they return (the reference's sync=true).
+ *  - A domain owns one HIP stream; calls on one domain must not overlap.
+ */
+#ifndef ARTSS_HIP_H
+#define ARTSS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARTSS_OK 0
+#define ARTSS_ERR_ARGUMENT 1 /* bad pointer, size, level or name */
+#define ARTSS_ERR_CONFIG 2   /* XML input rejected (Settings::config_error in the reference) */
+#define ARTSS_ERR_HIP 3      /* HIP runtime failure */
+#define ARTSS_ERR_INTERNAL 4
+
+typedef struct artss_domain artss_domain;
+typedef struct artss_mg artss_mg;
+typedef struct artss_solver artss_solver;
+
+typedef struct {
+    int N[3];               /* cells per axis including ghosts (DomainData::get_number_of_cells) */
+    int n[3];               /* inner cells per axis (DomainData::get_number_of_inner_cells) */
+    long long size;         /* N[0]*N[1]*N[2] (DomainData::get_size) */
+    double h[3];            /* spacing (DomainData::get_spacing) */
+    long long inner_cells;  /* fluid inner cells (DomainController::get_size_domain_inner_list_level_joined) */
+    int n_obstacles;
+} artss_level_info;
+
+const char *artss_last_error(void);
+int artss_version(void);
+
+/* ---- domain: DomainData::init + DomainController::init (main.cpp:62-78) ---- */
+int artss_domain_create(const char *xml_text, int device, artss_domain **out);
+int artss_domain_destroy(artss_domain *d);
+int artss_domain_levels(const artss_domain *d, int *levels);
+int artss_domain_level_info(const artss_domain *d, int level, artss_level_info *info);
+/* Reference index lists, for parity checks.  which: 0 inner list
+ * (Domain::get_inner_list), 1 domain list, 2 obstacle cells, 3..8 domain
+ * boundary patch LEFT..BACK, 10..15 face list LEFT..BACK of `obstacle`
+ * (after Obstacle::remove_circular_constraints at level 0).  With out ==
+ * NULL only *count is set. */
+int artss_domain_list(const artss_domain *d, int which, int level, int obstacle, uint64_t *out, size_t capacity,
+                      size_t *count);
+int artss_domain_synchronize(artss_domain *d);
+
+/* ---- fields: Field ctor/dtor, update_dev/update_host (field/Field.h:45-52) ---- */
+int artss_field_alloc(artss_domain *d, int level, double **dev);
+int artss_field_free(artss_domain *d, double *dev);
+int artss_field_upload(artss_domain *d, double *dev, const double *host, size_t n);
+int artss_field_download(artss_domain *d, double *host, const double *dev, size_t n);
+
+/* ---- operators ---- */
+/* DomainController::apply_boundary, domain/DomainController.cpp:145 (Multigrid.cpp:555-572) */
+int artss_apply_boundary(artss_domain *d, double *field, int field_type, int level);
+/* IAdvection::advect -> SLAdvect::advect, advection/SLAdvect.cpp:34 */
+int artss_sl_advect(artss_domain *d, double *out, const double *in, const double *u, const double *v,
+                    const double *w, int out_type);
+/* IDiffusion::diffuse (both overloads, ev == NULL for the laminar one),
+ * method "Jacobi" (diffusion/JacobiDiffuse.cpp:31,123) or
+ * "ColoredGaussSeidel" (diffusion/ColoredGaussSeidelDiffuse.cpp:38,111).
+ * `in` is modified as in the reference. */
+int artss_diffuse(artss_domain *d, const char *method, double *out, double *in, const double *b, double D,
+                  const double *ev, int field_type, int max_iter, double tol_res, double w, int *iterations,
+                  double *residual);
+/* JacobiDiffuse::JacobiStep multigrid form, diffusion/JacobiDiffuse.cpp:274 */
+int artss_jacobi_step(artss_domain *d, int level, double *out, const double *in, const double *b, double alpha_x,
+                      double alpha_y, double alpha_z, double beta, double dsign, double w);
+/* ColoredGaussSeidelDiffuse::colored_gauss_seidel_step, diffusion/ColoredGaussSeidelDiffuse.cpp:190 */
+int artss_cgs_step(artss_domain *d, int level, double *out, const double *b, double alpha_x, double alpha_y,
+                   double alpha_z, double beta, double dsign, double w);
+/* IPressure::divergence, interfaces/IPressure.cpp:20 */
+int artss_divergence(artss_domain *d, double *out, const double *in_x, const double *in_y, const double *in_z);
+/* IPressure::projection, interfaces/IPressure.cpp:125 */
+int artss_projection(artss_domain *d, double *out_u, double *out_v, double *out_w, const double *in_u,
+                     const double *in_v, const double *in_w, const double *in_p);
+/* ITurbulence::calc_turbulent_viscosity -> ConstSmagorinsky, turbulence/ConstSmagorinsky.cpp:30 */
+int artss_smagorinsky(artss_domain *d, double *ev, const double *u, const double *v, const double *w, double cs,
+                      int ev_type);
+/* ISource::add_source (scalar form) -> ExplicitEulerSource, source/ExplicitEulerSource.cpp:84 */
+int artss_add_source(artss_domain *d, double *out, const double *s, int field_type);
+/* ISource::buoyancy_force, interfaces/ISource.cpp:26 (beta, g from the XML) */
+int artss_buoyancy_force(artss_domain *d, double *out, const double *T, const double *T_ambient);
+/* ISourceFunction::update_source -> GaussFunction, source/GaussFunction.cpp:27 (XML temperature source) */
+int artss_gauss_source(artss_domain *d, double *out, double t);
+
+/* ---- IPressure::pressure -> VCycleMG, pressure/VCycleMG.cpp:17,142 (XML pressure section) ---- */
+int artss_mg_create(artss_domain *d, artss_mg **out);
+int artss_mg_destroy(artss_mg *mg);
+int artss_mg_pressure(artss_mg *mg, double *p, const double *rhs, double t, int *cycles);
+/* one VCycleMG::VCycleMultigrid (:215) and the residual norm of :186-195 */
+int artss_mg_vcycle(artss_mg *mg, double *p, const double *rhs);
+int artss_mg_residual(artss_mg *mg, const double *p, const double *rhs, double *residual);
+
+/* ---- solver: SolverController (solver/SolverController.cpp:26) + TimeIntegration ---- */
+int artss_solver_create(const char *xml_text, int device, artss_solver **out);
+int artss_solver_destroy(artss_solver *s);
+int artss_solver_domain(artss_solver *s, artss_domain **d); /* borrowed, owned by the solver */
+/* one TimeIntegration iteration: do_step(t), update_sources(t), update_data (TimeIntegration.cpp:93,141-142) */
+int artss_solver_step(artss_solver *s, double t);
+/* TimeIntegration::run loop for steps first..first+count-1 (t = step*dt); seconds = wall time */
+int artss_solver_run(artss_solver *s, long first_step, long count, double *seconds);
+/* device pointer of a named field: u v w p rhs T nu_t kappa_t f_x f_y f_z S_T T_ambient u0 ... */
+int artss_solver_field(artss_solver *s, const char *name, double **dev, size_t *n);
+/* last step: diffusion iterations (u, v, w, T), V-cycles, final pressure residual (-1 if not measured) */
+int artss_solver_counters(artss_solver *s, int *diffusion4, int *vcycles, double *residual);
+/* 0: reference pressure schedule (VCycleMG::pressure); 1: V-cycles until the residual <= tolerance */
+int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tolerance, int max_cycles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

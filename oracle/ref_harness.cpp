@@ -14,6 +14,7 @@
 //   run   <xml> <outdir> <dump_steps,comma> [ic]   whole solver time loop
 //   time  <xml> <steps> [ic]                        time loop, no dumps
 //   lists <xml> <outdir>                            index lists per level
+//   state <xml> <dir> <first> <count>               continue from a saved state, timed
 //   op    <xml> <opname> <outdir> <seed> [amp]      one operator call on
 //                                                   seeded random fields
 // `ic` = "taylor_green" overrides the velocity initial condition (the
@@ -187,6 +188,47 @@ int mode_run(const std::string &xml, const std::string &outdir, const std::strin
     size_t n_inner = DomainController::getInstance()->get_size_domain_inner_list_level_joined(0);
     std::printf("steps=%ld inner_cells=%zu seconds=%.6f size=%zu Nx=%zu Ny=%zu Nz=%zu\n",
                 iteration_step - 1, n_inner, secs, dd->get_size(), dd->get_Nx(), dd->get_Ny(), dd->get_Nz());
+    delete sc;
+    return 0;
+}
+
+// Continue a run from a saved state (u, v, w, p, T after step first-1, raw
+// float64 files) and time `count` steps.  The reference keeps no other state
+// between steps: sources and the *0 / *_tmp copies are rebuilt exactly as
+// TimeIntegration.cpp:141-142 leaves them.
+int mode_state(const std::string &xml, const std::string &dir, long first, long count) {
+    Settings::Settings settings = Settings::parse_settings(xml);
+    init_domain(settings);
+    auto *sc = new SolverController(settings);
+    FieldController *fc = sc->get_field_controller();
+    auto load = [&](const char *name, Field &f) {
+        const std::string path = dir + "/" + name + ".f64";
+        FILE *fp = std::fopen(path.c_str(), "rb");
+        if (!fp) { std::perror(path.c_str()); std::exit(2); }
+        size_t got = std::fread(f.data, sizeof(double), f.get_size(), fp);
+        std::fclose(fp);
+        if (got != f.get_size()) { std::cerr << "short state file " << path << std::endl; std::exit(2); }
+    };
+    load("u", fc->get_field_u());
+    load("v", fc->get_field_v());
+    load("w", fc->get_field_w());
+    load("p", fc->get_field_p());
+    load("T", fc->get_field_T());
+    const double dt = settings.physical_parameters.dt;
+    sc->update_sources(static_cast<double>(first - 1) * dt, true);
+    fc->update_data();
+    auto start = std::chrono::steady_clock::now();
+    for (long n = first; n < first + count; n++) {
+        const double t = static_cast<double>(n) * dt;
+        sc->solver_do_step(t, false);
+        sc->update_sources(t, false);
+        fc->update_data(false);
+    }
+    auto end = std::chrono::steady_clock::now();
+    size_t n_inner = DomainController::getInstance()->get_size_domain_inner_list_level_joined(0);
+    std::printf("steps=%ld inner_cells=%zu seconds=%.6f\n", count, n_inner,
+                std::chrono::duration<double>(end - start).count());
+    if (std::getenv("ARTSS_HARNESS_DUMP")) dump_state(std::getenv("ARTSS_HARNESS_DUMP"), fc, -1);
     delete sc;
     return 0;
 }
@@ -440,6 +482,7 @@ int main(int argc, char **argv) {
         if (mode == "time" && argc >= 4)
             return mode_run(argv[2], "", "", argc > 4 ? argv[4] : "", false, std::stol(argv[3]));
         if (mode == "lists" && argc >= 4) return mode_lists(argv[2], argv[3]);
+        if (mode == "state" && argc >= 6) return mode_state(argv[2], argv[3], std::stol(argv[4]), std::stol(argv[5]));
         if (mode == "op" && argc >= 6)
             return mode_op(argv[2], argv[3], argv[4], std::stoull(argv[5]), argc > 6 ? std::stod(argv[6]) : 1.0);
     } catch (const std::exception &ex) {

@@ -1,0 +1,246 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference.
+
+Three layers of checks, all bit for bit (np.array_equal on float64) unless
+stated otherwise:
+
+1. golden fixtures written by the real reference (tests/golden/*.npz): the
+   same seeded inputs go through each HIP operator and must give the
+   reference's outputs exactly;
+2. the CPU oracle on larger seeded inputs with several obstacles, adjacent
+   ones included;
+3. whole solver steps against the golden runs and, over 100 steps, against
+   the oracle (tolerance 1e-10 relative per checked step, 1e-8 at the end,
+   the bounds the north star sets for FP64 fields; residual norms are summed
+   in a different order than the serial build, so an iteration count can in
+   principle flip at a tolerance threshold -- the tests report counts).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import artss_amd
+import artss_xml
+import cases
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    z = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    xml = bytes(z["xml"]).decode()
+    return z, xml, artss_xml.parse(xml)
+
+
+def same(a, b, what=""):
+    assert a.shape == b.shape, what
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        raise AssertionError("%s: %d cells differ, max |d| %.3g at %d" % (what, (d != 0).sum(), d.max(), d.argmax()))
+
+
+@pytest.fixture(scope="module")
+def ops_jacobi():
+    z, xml, cfg = load("ops_jacobi")
+    return z, cfg, artss_amd.Domain(xml)
+
+
+@pytest.fixture(scope="module")
+def ops_cgs():
+    z, xml, cfg = load("ops_cgs")
+    return z, cfg, artss_amd.Domain(xml)
+
+
+@pytest.fixture(scope="module")
+def geo_adjacent():
+    z, xml, cfg = load("geo_adjacent")
+    return z, cfg, artss_amd.Domain(xml)
+
+
+# ------------------------------------------------------------------ golden
+@pytest.mark.parametrize("case", ["ops_jacobi", "geo_adjacent"])
+def test_bc_golden(case, request):
+    z, cfg, dom = request.getfixturevalue(case)
+    for ft in range(7):
+        for level in range(cfg["levels"] + 1):
+            key = "%d_%d" % (ft, level)
+            f = dom.field(z["bc/bc_in_" + key], level)
+            dom.apply_boundary(f, ft, level)
+            same(f.download(), z["bc/bc_out_" + key], "bc " + key)
+
+
+def test_advect_golden(ops_jacobi):
+    z, cfg, dom = ops_jacobi
+    u, v, w = (dom.field(z["advect/adv_" + c]) for c in "uvw")
+    for ft in (1, 5):
+        out = dom.field(z["advect/adv_out0_%d" % ft])
+        dom.sl_advect(out, dom.field(z["advect/adv_in_%d" % ft]), u, v, w, ft)
+        same(out.download(), z["advect/adv_out_%d" % ft], "advect %d" % ft)
+
+
+@pytest.mark.parametrize("case,op", [("ops_jacobi", "jacobi"), ("ops_jacobi", "jacobi_turb"),
+                                     ("ops_cgs", "cgs"), ("ops_cgs", "cgs_turb")])
+def test_diffusion_golden(case, op, request):
+    z, cfg, dom = request.getfixturevalue(case)
+    d = cfg["diffusion"]
+    out = dom.field(z[op + "/dif_out0"])
+    inp = dom.field(z[op + "/dif_in0"])
+    ev = dom.field(z[op + "/dif_ev"]) if op.endswith("turb") else None
+    it, _ = dom.diffuse(d["type"], out, inp, dom.field(z[op + "/dif_b"]), cfg["nu"], ev, 1, d["max_iter"],
+                        d["tol_res"], d["w"])
+    assert it >= 2
+    same(out.download(), z[op + "/dif_out"], op + " out")
+    same(inp.download(), z[op + "/dif_in"], op + " in")
+
+
+@pytest.mark.parametrize("case", ["ops_jacobi", "ops_cgs"])
+@pytest.mark.parametrize("op", ["vcycle1", "vcycle2"])
+def test_vcycle_golden(case, op, request):
+    z, cfg, dom = request.getfixturevalue(case)
+    mg = dom.multigrid()
+    p = dom.field(z[op + "/mg_p0"])
+    t = cfg["dt"] if op == "vcycle1" else 2 * cfg["dt"]
+    cycles = mg.pressure(p, dom.field(z[op + "/mg_rhs"]), t)
+    assert cycles >= cfg["pressure"]["n_cycle"]
+    same(p.download(), z[op + "/mg_p"], op)
+
+
+def test_divergence_projection_golden(ops_jacobi):
+    z, cfg, dom = ops_jacobi
+    out = dom.field(z["divergence/div_out0"])
+    dom.divergence(out, *(dom.field(z["divergence/div_" + c]) for c in "xyz"))
+    same(out.download(), z["divergence/div_out"], "divergence")
+    o = [dom.field(z["projection/prj_o%s0" % c]) for c in "uvw"]
+    dom.projection(*o, *(dom.field(z["projection/prj_i" + c]) for c in "uvwp"))
+    for c, a in zip("uvw", o):
+        same(a.download(), z["projection/prj_o" + c], "projection " + c)
+
+
+def test_smagorinsky_sources_golden(ops_jacobi):
+    z, cfg, dom = ops_jacobi
+    ev = dom.field(z["smagorinsky/smg_ev0"])
+    dom.smagorinsky(ev, *(dom.field(z["smagorinsky/smg_" + c]) for c in "uvw"), cfg["turbulence"]["cs"])
+    same(ev.download(), z["smagorinsky/smg_ev"], "smagorinsky")
+    out = dom.field(z["sources/src_out0"])
+    dom.add_source(out, dom.field(z["sources/src_s"]), 5)
+    same(out.download(), z["sources/src_out"], "add_source")
+    bo = dom.field(z["sources/buo_out0"])
+    dom.buoyancy_force(bo, dom.field(z["sources/buo_T"]), dom.field(z["sources/buo_Ta"]))
+    same(bo.download(), z["sources/buo_out"], "buoyancy")
+    st = dom.alloc()
+    dom.gauss_source(st, 0.37)
+    same(st.download(), z["sources/gauss_st"], "gauss")
+
+
+@pytest.mark.parametrize("case", ["run_ns_tg", "run_plume_jacobi", "run_plume_cgs"])
+def test_solver_golden(case):
+    z, xml, cfg = load(case)
+    xml = xml.replace('usr_fct="Zero"', 'usr_fct="TaylorGreen"') if cfg["solver"] == "NSSolver" else xml
+    s = artss_amd.Solver(xml)
+    nsteps = int(round(cfg["t_end"] / cfg["dt"]))
+    for f in "uvwpT":
+        same(s.field(f), z["run/%s_0" % f], "initial " + f)
+    for step in range(1, nsteps + 1):
+        s.step(step * cfg["dt"])
+        if step in (1, 2):
+            for f in ["u", "v", "w", "p", "T", "rhs"]:
+                same(s.field(f), z["run/%s_%d" % (f, step)], "%s step %d" % (f, step))
+    for f in "uvwpT":
+        same(s.field(f), z["run/%s_-1" % f], "final " + f)
+
+
+# ------------------------------------------------------------------ oracle, larger seeded inputs
+OBST3 = [(0.25, 0.5, 0.0, 0.375, 0.5, 0.75),
+         (0.5, 0.75, 0.25, 0.5, 0.25, 0.5),      # touches the first on x = 0.5
+         (1.25, 1.5, 0.5, 0.75, 0.25, 0.75)]
+
+
+@pytest.fixture(scope="module")
+def big():
+    xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=3, steps=2, dt=0.01, diffusion="Jacobi",
+                      obstacles=OBST3, hrr=3.0, source_x=1.0)
+    cfg = artss_xml.parse(xml)
+    return cfg, artss_amd.Domain(xml), O.Domain(cfg), np.random.default_rng(7)
+
+
+def test_ops_vs_oracle_big(big):
+    cfg, dom, odom, rng = big
+    n = odom.size(0)
+    r = lambda s=1.0: s * rng.standard_normal(n)
+    # boundary fill at every level and field type
+    for ft in range(7):
+        for level in range(cfg["levels"] + 1):
+            a = rng.standard_normal(odom.size(level))
+            g = dom.field(a, level)
+            dom.apply_boundary(g, ft, level)
+            odom.apply_bc(a, ft, level)
+            same(g.download(), a, "bc %d/%d" % (ft, level))
+    # advection with CFL up to ~2.5 in every direction
+    h = odom.info(0)["h"]
+    u, v, w = (r(2.5 * h[a] / cfg["dt"]) for a in range(3))
+    inp, out = r(), r()
+    g = dom.field(out)
+    dom.sl_advect(g, dom.field(inp), dom.field(u), dom.field(v), dom.field(w), 5)
+    odom.sl_advect(out, inp, u, v, w, cfg["dt"], 5)
+    same(g.download(), out, "advect")
+    # turbulent Jacobi and CGS diffusion
+    for method in ("Jacobi", "ColoredGaussSeidel"):
+        o, i, b, ev = r(), r(), r(), np.abs(r(1e-2))
+        go, gi = dom.field(o), dom.field(i)
+        it_g, _ = dom.diffuse(method, go, gi, dom.field(b), 1e-3, dom.field(ev), 1, 100, 1e-9, 1.0)
+        it_o, _ = odom.diffuse(method, o, i, b, 1e-3, ev, cfg["dt"], 100, 1e-9, 1.0, 1)
+        assert it_g == it_o
+        same(go.download(), o, method)
+        same(gi.download(), i, method + " in")
+    # smagorinsky
+    ev = r()
+    g = dom.field(ev)
+    dom.smagorinsky(g, dom.field(u), dom.field(v), dom.field(w), 0.2)
+    odom.smagorinsky(ev, u, v, w, 0.2)
+    same(g.download(), ev, "smagorinsky")
+
+
+@pytest.mark.parametrize("smoother", ["Jacobi", "ColoredGaussSeidel"])
+def test_vcycle_vs_oracle_big(smoother):
+    xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=3, steps=2, dt=0.01, diffusion="Jacobi",
+                      smoother=smoother, obstacles=OBST3, hrr=3.0, source_x=1.0)
+    cfg = artss_xml.parse(xml)
+    dom, odom = artss_amd.Domain(xml), O.Domain(cfg)
+    rng = np.random.default_rng(11)
+    p, rhs = 0.01 * rng.standard_normal(odom.size(0)), rng.standard_normal(odom.size(0))
+    gp, grhs = dom.field(p), dom.field(rhs)
+    mg, omg = dom.multigrid(), O.MG(odom, cfg)
+    for t in (cfg["dt"], 2 * cfg["dt"], 3 * cfg["dt"]):
+        c_g = mg.pressure(gp, grhs, t)
+        c_o = omg.pressure(p, rhs, t, cfg["dt"])
+        assert c_g == c_o
+        same(gp.download(), p, "vcycle t=%g" % t)
+
+
+# ------------------------------------------------------------------ multi-step vs oracle
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_taylor_green_100_steps_vs_oracle():
+    """C1 physics (NSSolver, periodic Taylor-Green, nu=0.01, dt=1e-3) on 32^3, 100 steps."""
+    xml_ref = cases.ns_taylor_green(n=32, levels=3, steps=100, reference=True)
+    cfg = artss_xml.parse(xml_ref)
+    odom = O.Domain(cfg)
+    u, v, w = cases.taylor_green_ic(odom)
+    osim = O.Sim(odom, cfg, u, v, w, None)
+    s = artss_amd.Solver(xml_ref.replace('usr_fct="Zero"', 'usr_fct="TaylorGreen"'))
+    worst = 0.0
+    for step in range(1, 101):
+        t = step * cfg["dt"]
+        s.step(t)
+        osim.step(t)
+        assert s.counters()["diffusion"][:3] == osim.counters()[:3], step
+        if step % 10 == 0 or step < 3:
+            for f in "uvwp":
+                e = _rel(s.field(f), osim.field(f))
+                worst = max(worst, e)
+                assert e <= 1e-10, (f, step, e)
+    assert worst <= 1e-8
