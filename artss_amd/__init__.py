@@ -79,6 +79,8 @@ def lib():
         "artss_solver_field": (C.c_int, [vp, C.c_char_p, C.POINTER(vp), C.POINTER(C.c_size_t)]),
         "artss_solver_counters": (C.c_int, [vp, ip, ip, dp]),
         "artss_solver_set_pressure_mode": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
+        "artss_profile_arm": (C.c_int, [vp, C.c_char_p]),
+        "artss_profile_read": (C.c_int, [vp, C.POINTER(C.c_longlong), dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -150,6 +152,16 @@ class Domain:
 
     def sync(self):
         _check(self.L.artss_domain_synchronize(self.h))
+
+    def profile_arm(self, tag):
+        """Time every launch of one kernel ("<kernel>@<level>") with HIP events."""
+        _check(self.L.artss_profile_arm(self.h, tag.encode()))
+
+    def profile_read(self):
+        n = C.c_longlong()
+        ms = C.c_double()
+        _check(self.L.artss_profile_read(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     # ---- device fields
     def alloc(self, level=0):

@@ -510,4 +510,20 @@ int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tol
     });
 }
 
+int artss_profile_arm(artss_domain *d, const char *tag) {
+    return guard([&] {
+        require(tag, "null tag");
+        dom(d).timer_arm(tag);
+        return ARTSS_OK;
+    });
+}
+
+int artss_profile_read(artss_domain *d, long long *launches, double *total_ms) {
+    return guard([&] {
+        require(launches && total_ms, "null argument");
+        dom(d).timer_read(launches, total_ms);
+        return ARTSS_OK;
+    });
+}
+
 }  // extern "C"

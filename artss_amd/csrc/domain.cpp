@@ -429,8 +429,10 @@ void Domain::apply_boundary(double *f, int ft, int level) {
     require_device();
     if (ft < 0 || ft >= NUMBER_OF_FIELD_TYPES) return;  // UNKNOWN_FIELD has no boundary data
     for (const Wave &w : m_waves[level][ft])
-        k::bc_faces(m_stream, m_faces_dev + w.offset, w.count, m_params_dev + m_param_offset[level][ft], f);
-    k::bc_domain(m_stream, m_level[level], m_dom_bc[level][ft], f);
+        timed("bc_faces", level, [&] {
+            k::bc_faces(m_stream, m_faces_dev + w.offset, w.count, m_params_dev + m_param_offset[level][ft], f);
+        });
+    timed("bc_domain", level, [&] { k::bc_domain(m_stream, m_level[level], m_dom_bc[level][ft], f); });
 }
 
 double Domain::reduce(const double *partial, int n) {

@@ -35,8 +35,10 @@ void ISource::buoyancy_force(Field &out, const Field &in, const Field &in_a, dou
 }
 
 void SLAdvect::advect(Field &out, const Field &in, const Field &u, const Field &v, const Field &w, bool sync) {
-    k::sl_advect(m_domain.stream(), m_domain.level(0), m_domain.settings().dt, out.data, in.data, u.data, v.data,
-                 w.data);
+    m_domain.timed("sl_advect", 0, [&] {
+        k::sl_advect(m_domain.stream(), m_domain.level(0), m_domain.settings().dt, out.data, in.data, u.data,
+                     v.data, w.data);
+    });
     m_domain.apply_boundary(out.data, out.get_type(), 0);
     maybe_sync(m_domain, sync);
 }
@@ -55,8 +57,10 @@ void JacobiDiffuse::run(Field &out, const Field &in, const Field &b, double D, c
     int it = 0;
     double res = 1.;
     while (res > m_settings.tol_res && it < m_settings.max_iter) {
-        k::jacobi_diffuse_step(dm.stream(), L, out.data, in.data, b.data, ax, ay, az, rb, D,
-                               EV ? EV->data : nullptr, dt, m_settings.w, dm.partial_buffer());
+        dm.timed("jacobi_diffuse", 0, [&] {
+            k::jacobi_diffuse_step(dm.stream(), L, out.data, in.data, b.data, ax, ay, az, rb, D,
+                                   EV ? EV->data : nullptr, dt, m_settings.w, dm.partial_buffer());
+        });
         dm.apply_boundary(out.data, out.get_type(), 0);
         res = std::sqrt(dm.reduce(dm.partial_buffer(), n_part));
         it++;
@@ -94,7 +98,10 @@ void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b,
     while (res > m_settings.tol_res && it < m_settings.max_iter) {
         in.copy_data(out);
         for (int color = 0; color < 2; color++)
-            k::cgs_color(dm.stream(), L, color, out.data, b.data, ax, ay, az, beta, 1.0, m_settings.w, D, ev, dt);
+            dm.timed("cgs_diffuse", 0, [&] {
+                k::cgs_color(dm.stream(), L, color, out.data, b.data, ax, ay, az, beta, 1.0, m_settings.w, D, ev,
+                             dt);
+            });
         dm.apply_boundary(out.data, out.get_type(), 0);
         k::update_norm_partials(dm.stream(), L, out.data, in.data, rb, D, ev, dt, dm.partial_buffer());
         res = std::sqrt(dm.reduce(dm.partial_buffer(), n_part));

@@ -39,7 +39,8 @@ void VCycleMG::coeffs(int level, double *ax, double *ay, double *az, double *bet
 
 double VCycleMG::level_residual(const Field &out, const Field &b, int level) {
     const LevelDesc &L = m_domain.level(level);
-    k::residual_partials(m_domain.stream(), L, out.data, b.data, m_domain.partial_buffer());
+    m_domain.timed("residual", level,
+                   [&] { k::residual_partials(m_domain.stream(), L, out.data, b.data, m_domain.partial_buffer()); });
     return std::sqrt(m_domain.reduce(m_domain.partial_buffer(), k::partial_count(L)));
 }
 
@@ -59,8 +60,10 @@ void VCycleMG::smooth_jacobi(Field &out, Field &tmp, const Field &b, int level) 
     const int effective = (n % 2 == 0 && n > 0) ? n - 1 : n;
     int swaps = 0;
     for (int i = 0; i < effective; i++) {
-        k::jacobi_step_mg(d.stream(), d.level(level), out.data, tmp.data, b.data, ax, ay, az, beta, kDsign,
-                          m_s.smoother.w);
+        d.timed("jacobi_mg", level, [&] {
+            k::jacobi_step_mg(d.stream(), d.level(level), out.data, tmp.data, b.data, ax, ay, az, beta, kDsign,
+                              m_s.smoother.w);
+        });
         d.apply_boundary(out.data, P, level);
         Field::swap(tmp, out);
         swaps++;
@@ -84,8 +87,10 @@ void VCycleMG::solve_jacobi(Field &out, Field &tmp, const Field &b, int level) {
     int it = 0;
     double res = 10000.;
     while (res > m_s.smoother.tol_res && it < m_s.smoother.max_iter) {
-        k::jacobi_step_mg(d.stream(), d.level(level), out.data, tmp.data, b.data, ax, ay, az, beta, kDsign,
-                          m_s.smoother.w);
+        d.timed("jacobi_mg", level, [&] {
+            k::jacobi_step_mg(d.stream(), d.level(level), out.data, tmp.data, b.data, ax, ay, az, beta, kDsign,
+                              m_s.smoother.w);
+        });
         d.apply_boundary(out.data, P, level);
         res = level_residual(out, b, level);
         Field::swap(tmp, out);
@@ -102,8 +107,10 @@ void VCycleMG::smooth_cgs(Field &out, const Field &b, int level) {
     coeffs(level, &ax, &ay, &az, &beta);
     for (int i = 0; i < m_s.n_relax; i++) {
         for (int color = 0; color < 2; color++)
-            k::cgs_color(d.stream(), d.level(level), color, out.data, b.data, ax, ay, az, beta, kDsign,
-                         m_s.smoother.w, 0.0, nullptr, 0.0);
+            d.timed("cgs_mg", level, [&] {
+                k::cgs_color(d.stream(), d.level(level), color, out.data, b.data, ax, ay, az, beta, kDsign,
+                             m_s.smoother.w, 0.0, nullptr, 0.0);
+            });
         d.apply_boundary(out.data, P, level);
     }
 }
@@ -118,8 +125,10 @@ void VCycleMG::solve_cgs(Field &out, const Field &b, int level) {
     double res = 10000.;
     while (res > m_s.smoother.tol_res && it < m_s.smoother.max_iter) {
         for (int color = 0; color < 2; color++)
-            k::cgs_color(d.stream(), d.level(level), color, out.data, b.data, ax, ay, az, beta, kDsign,
-                         m_s.smoother.w, 0.0, nullptr, 0.0);
+            d.timed("cgs_mg", level, [&] {
+                k::cgs_color(d.stream(), d.level(level), color, out.data, b.data, ax, ay, az, beta, kDsign,
+                             m_s.smoother.w, 0.0, nullptr, 0.0);
+            });
         d.apply_boundary(out.data, P, level);
         res = level_residual(out, b, level);
         it++;
@@ -158,15 +167,23 @@ void VCycleMG::vcycle(Field &out, const Field &b) {
     for (int level = 0; level < L; ++level) {
         const Field &rhs = level == 0 ? b : *m_res1[level];
         smooth(*m_err1[level], *m_tmp[level], rhs, level);
-        k::residuum(d.stream(), d.level(level), m_res0[level]->data, m_err1[level]->data, rhs.data);
+        d.timed("residuum", level, [&] {
+            k::residuum(d.stream(), d.level(level), m_res0[level]->data, m_err1[level]->data, rhs.data);
+        });
         d.apply_boundary(m_res0[level]->data, P, level);
-        k::restrict_(d.stream(), d.level(level + 1), d.level(level), m_res1[level + 1]->data, m_res0[level]->data);
+        d.timed("restrict", level, [&] {
+            k::restrict_(d.stream(), d.level(level + 1), d.level(level), m_res1[level + 1]->data,
+                         m_res0[level]->data);
+        });
         d.apply_boundary(m_res1[level + 1]->data, P, level + 1);
         m_err1[level + 1]->set_value(0);
     }
     for (int level = L; level > 0; --level) {
         const Field &rhs = level - 1 == 0 ? b : *m_res1[level - 1];
-        k::prolongate(d.stream(), d.level(level), d.level(level - 1), m_err0[level - 1]->data, m_err1[level]->data);
+        d.timed("prolongate", level - 1, [&] {
+            k::prolongate(d.stream(), d.level(level), d.level(level - 1), m_err0[level - 1]->data,
+                          m_err1[level]->data);
+        });
         d.apply_boundary(m_err0[level - 1]->data, P, level - 1);
         *m_err1[level - 1] += *m_err0[level - 1];
         if (level == L) solve(*m_err1[level - 1], *m_tmp[level - 1], rhs, level - 1);

@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark of the ARTSS solver hot path on one MI355X.
+
+Workload: the buoyant-plume LES (NSTempTurbSolver: SL advection, colored
+Gauss-Seidel diffusion, ConstSmagorinsky, Boussinesq buoyancy, Gaussian heat
+source, geometric multigrid pressure) on 256x128x128 cells over 2x1x1 m with
+one box obstacle, from artss_amd/configs/plume_256x128x128.xml.  A step is
+one full time step over all inner cells; the pressure equation is solved
+with V-cycles until ||rhs - lap p||_2 <= 1e-8 every step (plain FP64 path).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  `value` is cell-updates/s = inner cells x K / wall
+time of the K timed steps, fields resident in HBM throughout.  `roofline`
+is the dominant kernel's algorithmic bytes per launch over its mean launch
+time (HIP events on the solver's stream, timed region only) against the
+8 TB/s HBM3E peak.  `cpu_baseline` runs the reference's own serial build
+(oracle/_ref, compiled from the reference sources) for a bounded sample
+continuing from the same state on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s, 256x128x128 plume LES, learned+mixed vs plain FP64 MG at res 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per level-inner cell for one launch of a kernel (each
+# distinct array element read or written once, 8-byte doubles)
+BYTES_PER_CELL = {
+    "jacobi_mg": 24,       # read x, b; write x'
+    "cgs_mg": 16,          # one color: read x (all), b (half); write half
+    "cgs_diffuse": 20,     # one color, turbulent: + eddy viscosity (half)
+    "jacobi_diffuse": 32,  # turbulent: read x, b, ev; write x'
+    "residuum": 24,        # read x, b; write r
+    "residual": 16,        # read x, b
+    "sl_advect": 40,       # read in, u, v, w; write out
+    "restrict": 9,         # read 8 fine per coarse cell, write 1 (per fine cell: 8 + 1/8*8)
+    "prolongate": 9,
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default=os.path.join(ROOT, "artss_amd", "configs", "plume_256x128x128.xml"))
+    ap.add_argument("--tolerance", type=float, default=1e-8, help="pressure residual target per step")
+    ap.add_argument("--reference-schedule", action="store_true",
+                    help="use the reference's V-cycle schedule instead of the residual target")
+    ap.add_argument("--profile-kernel", default="jacobi_mg@0", help="kernel@level timed for the roofline")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the reference CPU sample (0: skip)")
+    ap.add_argument("--device", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(solver, config, first_step, steps, inner):
+    """Reference serial build on this host, continuing from the GPU state."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if steps <= 0 or not os.path.exists(harness):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        for f in ("u", "v", "w", "p", "T"):
+            solver.field(f).tofile(os.path.join(tmp, f + ".f64"))
+        r = subprocess.run([harness, "state", config, tmp, str(first_step), str(steps)], capture_output=True,
+                           text=True, timeout=900)
+    if r.returncode != 0:
+        return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": "failed: " + r.stderr.strip()[-200:]}
+    secs = float(r.stdout.split("seconds=")[1].split()[0])
+    return {"value": inner * steps / secs, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": "%d steps (%d..%d) of the same workload continuing from the GPU state, reference "
+                      "serial build with its own V-cycle schedule, %.1f s" % (steps, first_step,
+                                                                              first_step + steps - 1, secs)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: every box here has one MI355X and this build has no "
+              "multi-GPU path (the domain-decomposition design is in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import artss_amd
+
+    solver = artss_amd.Solver(args.config, device=args.device)
+    dom = solver.domain
+    info = dom.info(0)
+    inner = info["inner_cells"]
+    if not args.reference_schedule:
+        solver.set_pressure_mode(True, args.tolerance, 500)
+    with open(args.config) as fh:
+        dt = float(fh.read().split("<dt>")[1].split("</dt>")[0])
+
+    step = 1
+    for _ in range(args.warmup):
+        solver.step(step * dt)
+        step += 1
+    first = step
+    dom.profile_arm(args.profile_kernel)
+    dom.sync()
+    t0 = time.perf_counter()
+    cycles, diff_it, residuals = [], [], []
+    for _ in range(args.steps):
+        solver.step(step * dt)
+        c = solver.counters()
+        cycles.append(c["vcycles"])
+        diff_it.append(c["diffusion"])
+        residuals.append(c["residual"])
+        step += 1
+    dom.sync()
+    secs = time.perf_counter() - t0
+    launches, kernel_ms = dom.profile_read()
+
+    value = inner * args.steps / secs
+    kname, klevel = args.profile_kernel.split("@")
+    kinfo = dom.info(int(klevel))
+    per_launch_bytes = BYTES_PER_CELL.get(kname, 0) * kinfo["inner_cells"]
+    mean_ms = kernel_ms / launches if launches else float("nan")
+    achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9 if launches else None
+    roofline = {"bound": "hbm", "kernel": args.profile_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "bytes_per_launch": per_launch_bytes, "launches": launches, "mean_launch_us": mean_ms * 1e3,
+                "share_of_step_time": kernel_ms / (secs * 1e3)}
+
+    # state at the start of the timed region is gone; continue the CPU sample
+    # from the state after it (same workload, later steps)
+    cpu = cpu_baseline(solver, args.config, step, args.cpu_steps, inner)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: plume from rest, steps %d..%d" % (first, step - 1),
+        "config": {"workload": "plume_256x128x128", "grid": list(info["n"]), "inner_cells": inner,
+                   "path": "plain FP64",
+                   "pressure": ("reference V-cycle schedule" if args.reference_schedule else
+                                "V-cycles until residual <= %g" % args.tolerance),
+                   "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(
+                       diff_it, axis=0).tolist(), "max_final_residual": float(np.max(residuals))},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,15 @@ int artss_solver_counters(artss_solver *s, int *diffusion4, int *vcycles, double
 /* 0: reference pressure schedule (VCycleMG::pressure); 1: V-cycles until the residual <= tolerance */
 int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tolerance, int max_cycles);
 
+/* ---- live kernel timing (no reference counterpart; the reference only
+ * prints "Global Time", TimeIntegration.cpp:165-183) ----
+ * Arms HIP-event timing of every launch of one kernel at one level, tag
+ * "<kernel>@<level>" (kernels: jacobi_mg cgs_mg jacobi_diffuse cgs_diffuse
+ * sl_advect residuum residual restrict prolongate bc_faces bc_domain);
+ * read returns the launches seen since arming and their summed duration. */
+int artss_profile_arm(artss_domain *d, const char *tag);
+int artss_profile_read(artss_domain *d, long long *launches, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif
