@@ -72,9 +72,13 @@ def ns_taylor_green(n=64, levels=4, steps=100, dt=1e-3, nu=0.01, reference=False
 
 def plume(n=(128, 128, 128), X=(0.0, 1.0), levels=5, steps=1000, dt=5e-3, diffusion="ColoredGaussSeidel",
           smoother="Jacobi", obstacles=(), hrr=None, source_x=None, n_cycle=2, T_a=293.15, nu=3.1e-5,
-          kappa=4.2e-5, sigma=0.1, tau=5.0, diff_tol=1e-7):
+          kappa=4.2e-5, sigma=0.1, tau=5.0, diff_tol=1e-7, open_top=False):
     """NSTempTurbSolver buoyant plume: Boussinesq buoyancy along y, Gaussian
-    volumetric heat source at the floor centre, ConstSmagorinsky Cs=0.2."""
+    volumetric heat source at the floor centre, ConstSmagorinsky Cs=0.2.
+
+    open_top=False: closed box (u,v,w Dirichlet 0, p Neumann 0 everywhere).
+    open_top=True: the top patch is an outflow (u,v,w and T Neumann 0,
+    p Dirichlet 0), which makes the pressure Poisson problem non-singular."""
     if source_x is None:
         source_x = 0.5 * (X[0] + X[1])
     if hrr is None:
@@ -94,10 +98,18 @@ def plume(n=(128, 128, 128), X=(0.0, 1.0), levels=5, steps=1000, dt=5e-3, diffus
               '<HRR> {} </HRR><cp> 1. </cp><x0> {} </x0><y0> 0. </y0><z0> 0.5 </z0><sigma_x> {} </sigma_x>'
               '<sigma_y> {} </sigma_y><sigma_z> {} </sigma_z><tau> {} </tau></source></temperature>'.format(
                   repr(hrr), repr(source_x), repr(sigma), repr(sigma), repr(sigma), repr(tau)))
-    bnd = ('<boundary field="u,v,w" patch="front,back,bottom,top,left,right" type="dirichlet" value="0.0" />'
-           '<boundary field="p" patch="front,back,bottom,top,left,right" type="neumann" value="0.0" />'
-           '<boundary field="T" patch="front,back,top,left,right" type="dirichlet" value="{}" />'
-           '<boundary field="T" patch="bottom" type="neumann" value="0.0" />').format(repr(T_a))
+    if open_top:
+        bnd = ('<boundary field="u,v,w" patch="front,back,bottom,left,right" type="dirichlet" value="0.0" />'
+               '<boundary field="u,v,w" patch="top" type="neumann" value="0.0" />'
+               '<boundary field="p" patch="front,back,bottom,left,right" type="neumann" value="0.0" />'
+               '<boundary field="p" patch="top" type="dirichlet" value="0.0" />'
+               '<boundary field="T" patch="front,back,left,right" type="dirichlet" value="{}" />'
+               '<boundary field="T" patch="bottom,top" type="neumann" value="0.0" />').format(repr(T_a))
+    else:
+        bnd = ('<boundary field="u,v,w" patch="front,back,bottom,top,left,right" type="dirichlet" value="0.0" />'
+               '<boundary field="p" patch="front,back,bottom,top,left,right" type="neumann" value="0.0" />'
+               '<boundary field="T" patch="front,back,top,left,right" type="dirichlet" value="{}" />'
+               '<boundary field="T" patch="bottom" type="neumann" value="0.0" />').format(repr(T_a))
     ic = '<initial_conditions usr_fct="Uniform" random="No"><val> {} </val></initial_conditions>'.format(repr(T_a))
     return _wrap(phys, ("NSTempTurbSolver", solver), domain_xml(X, (0.0, 1.0), (0.0, 1.0), n), bnd,
                  obstacles_xml(list(obstacles)), ic)
