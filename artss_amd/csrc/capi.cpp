@@ -414,6 +414,24 @@ int artss_mg_vcycle(artss_mg *mg, double *p, const double *rhs) {
     });
 }
 
+int artss_mg_set_schedule(artss_mg *mg, int mode, long long tail_cells) {
+    return guard([&] {
+        require(mg && tail_cells >= 0 && mode >= 0 && mode <= 2, "bad argument");
+        mg->mg->set_schedule(mode != 0, tail_cells);
+        mg->mg->set_graphs(mode == 2);
+        finish(dom(mg->d));
+        return ARTSS_OK;
+    });
+}
+
+int artss_mg_tail_level(artss_mg *mg, int *first) {
+    return guard([&] {
+        require(mg && first, "null argument");
+        *first = mg->mg->tail_first();
+        return ARTSS_OK;
+    });
+}
+
 int artss_mg_residual(artss_mg *mg, const double *p, const double *rhs, double *residual) {
     return guard([&] {
         require(mg && p && rhs && residual, "null argument");
@@ -501,11 +519,31 @@ int artss_solver_counters(artss_solver *s, int *diffusion4, int *vcycles, double
 int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tolerance, int max_cycles) {
     return guard([&] {
         require(s && max_cycles >= 0, "bad argument");
-        PressureMode m;
+        PressureMode m = s->solver->pressure_mode();
         m.to_tolerance = to_tolerance != 0;
         m.tolerance = tolerance;
         m.max_cycles = max_cycles;
         s->solver->set_pressure_mode(m);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_set_pressure_method(artss_solver *s, int method) {
+    return guard([&] {
+        require(s && method >= 0 && method <= 1, "bad argument");
+        PressureMode m = s->solver->pressure_mode();
+        m.method = method;
+        s->solver->set_pressure_mode(m);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_set_mg_schedule(artss_solver *s, int mode, long long tail_cells) {
+    return guard([&] {
+        require(s && tail_cells >= 0 && mode >= 0 && mode <= 2, "bad argument");
+        s->solver->pressure_solver().set_schedule(mode != 0, tail_cells);
+        s->solver->pressure_solver().set_graphs(mode == 2);
+        finish(s->solver->domain());
         return ARTSS_OK;
     });
 }

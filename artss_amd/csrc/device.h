@@ -61,6 +61,26 @@ struct FaceParam {
     int pad;
 };
 
+// A ghost cell whose patch chain starts on an obstacle face cell, with the
+// face write (index into the face entries) that produces that cell.
+struct DepGhost {
+    uint32_t idx;
+    int32_t face;
+};
+
+// Everything a kernel needs to produce the values of an idempotent boundary
+// fill itself (faces of one wave, ghosts, ghosts depending on faces).
+struct BCView {
+    const FaceEntry *faces;
+    long long n_faces;
+    const FaceParam *params;
+    const DepGhost *deps;
+    long long n_deps;
+    const uint32_t *wbits;  // bit c set: cell c is written by the fill
+    DomainBC bc;
+    long long n_ghosts;     // 0 when the field type has no domain boundary data
+};
+
 #if defined(__HIPCC__)
 __host__ __device__ inline bool in_obstacle(const LevelDesc &L, int i, int j, int k) {
     for (int o = 0; o < L.n_obst; o++) {

@@ -134,6 +134,9 @@ Domain::~Domain() {
     (void)hipHostFree(m_sum_host);
     (void)hipFree(m_faces_dev);
     (void)hipFree(m_params_dev);
+    (void)hipFree(m_params_hom_dev);
+    (void)hipFree(m_bits_dev);
+    (void)hipFree(m_deps_dev);
     (void)hipStreamDestroy(m_stream);
 }
 
@@ -298,8 +301,16 @@ void Domain::build_boundary_tables() {
     m_dom_bc.assign(m_levels + 1, std::vector<DomainBC>(NUMBER_OF_FIELD_TYPES));
     m_waves.assign(m_levels + 1, std::vector<std::vector<Wave>>(NUMBER_OF_FIELD_TYPES));
     m_param_offset.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
+    m_simple.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 0));
     std::vector<FaceEntry> entries;
     std::vector<FaceParam> params;
+    std::vector<uint32_t> bits;
+    std::vector<DepGhost> deps;
+    m_idem.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 0));
+    m_face_values_zero.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 1));
+    m_view_bits.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
+    m_view_deps.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
+    m_view_ndeps.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
 
     for (int l = 0; l <= m_levels; l++) {
         const LevelDesc &L = m_level[l];
@@ -332,6 +343,7 @@ void Domain::build_boundary_tables() {
             }
             // ---- obstacle faces: parameter table + write schedule
             m_param_offset[l][f] = static_cast<long long>(params.size());
+            m_face_values_zero[l][f] = 1;
             for (int id = 0; id < n_obst; id++)
                 for (int p = 0; p < 6; p++) {
                     FaceParam q{};
@@ -348,6 +360,7 @@ void Domain::build_boundary_tables() {
                             q.value = -(v * h);
                         }
                     }
+                    if (q.active && q.value != 0.0) m_face_values_zero[l][f] = 0;
                     params.push_back(q);
                 }
             // writes in reference order (Multigrid.cpp:559-562, ObstacleBoundary.cpp:178-209)
@@ -400,6 +413,53 @@ void Domain::build_boundary_tables() {
                 r = std::max(r, wave);
                 n_waves = std::max(n_waves, wave + 1);
             }
+            // Classify the fill.  W = cells it writes (live faces, ghosts).
+            //  idempotent: no face write reads a cell of W, so one wave of
+            //    faces suffices and the result depends on non-W cells only;
+            //  simple: idempotent and no ghost reads a face cell either, so
+            //    faces and ghosts are independent.
+            std::vector<char> written(static_cast<size_t>(L.size), 0);
+            std::vector<long long> ghost_src;  // per ghost (k_bc_domain order) the cell its chain reads
+            {
+                for (const auto &w : ws)
+                    if (w.live) written[w.cell] = 2;  // 2: face-written
+                const long long Nz = L.N[2];
+                if (d.active) {
+                    const long long fx = Ny * Nz, fy = (Nx - 2) * Nz, fz = (Nx - 2) * (Ny - 2);
+                    const long long n_ghost = 2 * fx + 2 * fy + 2 * fz;
+                    ghost_src.resize(static_cast<size_t>(n_ghost));
+                    for (long long t0 = 0; t0 < n_ghost; t0++) {
+                        long long t = t0, i, j, k;
+                        if (t < 2 * fx) {
+                            i = (t / fx) ? Nx - 1 : 0;
+                            j = (t % fx) % Ny;
+                            k = (t % fx) / Ny;
+                        } else if ((t -= 2 * fx) < 2 * fy) {
+                            j = (t / fy) ? Ny - 1 : 0;
+                            i = 1 + (t % fy) % (Nx - 2);
+                            k = (t % fy) / (Nx - 2);
+                        } else {
+                            t -= 2 * fy;
+                            k = (t / fz) ? Nz - 1 : 0;
+                            i = 1 + (t % fz) % (Nx - 2);
+                            j = 1 + (t % fz) / (Nx - 2);
+                        }
+                        written[i + Nx * (j + Ny * k)] = 1;
+                        const long long ii = (i == 0 || i == Nx - 1) ? d.target[i == 0 ? 0 : 1] : i;
+                        const long long jj = (j == 0 || j == Ny - 1) ? d.target[j == 0 ? 2 : 3] : j;
+                        const long long kk = (k == 0 || k == Nz - 1) ? d.target[k == 0 ? 4 : 5] : k;
+                        ghost_src[static_cast<size_t>(t0)] = ii + Nx * (jj + Ny * kk);
+                    }
+                }
+                bool idem = true;
+                for (const auto &w : ws)
+                    if (w.live && written[w.read]) idem = false;
+                bool simple = idem;
+                for (long long src : ghost_src)
+                    if (written[static_cast<size_t>(src)]) simple = false;
+                m_simple[l][f] = simple ? 1 : 0;
+                m_idem[l][f] = idem ? 1 : 0;
+            }
             for (int wv = 0; wv < n_waves; wv++) {
                 Wave W{static_cast<long long>(entries.size()), 0};
                 for (const auto &w : ws)
@@ -410,6 +470,48 @@ void Domain::build_boundary_tables() {
                         W.count++;
                     }
                 m_waves[l][f].push_back(W);
+            }
+            // device tables of an idempotent fill: bitmap of W and the ghosts
+            // whose chain starts on a face cell, with that face's entry
+            m_view_bits[l][f] = static_cast<long long>(bits.size());
+            m_view_deps[l][f] = static_cast<long long>(deps.size());
+            m_view_ndeps[l][f] = 0;
+            if (m_idem[l][f]) {
+                const size_t words = (static_cast<size_t>(L.size) + 31) / 32;
+                const size_t b0 = bits.size();
+                bits.resize(b0 + words, 0u);
+                for (size_t c = 0; c < written.size(); c++)
+                    if (written[c]) bits[b0 + c / 32] |= 1u << (c % 32);
+                if (!m_waves[l][f].empty()) {
+                    std::unordered_map<uint64_t, long long> entry_of;
+                    const Wave &W0 = m_waves[l][f][0];
+                    for (long long e = W0.offset; e < W0.offset + W0.count; e++) entry_of[entries[e].idx] = e - W0.offset;
+                    const long long Nz = L.N[2];
+                    const long long fx = Ny * Nz, fy = (Nx - 2) * Nz;
+                    for (size_t t0 = 0; t0 < ghost_src.size(); t0++) {
+                        const long long src = ghost_src[t0];
+                        if (written[static_cast<size_t>(src)] != 2) continue;
+                        long long t = static_cast<long long>(t0), i, j, k;
+                        if (t < 2 * fx) {
+                            i = (t / fx) ? Nx - 1 : 0;
+                            j = (t % fx) % Ny;
+                            k = (t % fx) / Ny;
+                        } else if ((t -= 2 * fx) < 2 * fy) {
+                            j = (t / fy) ? Ny - 1 : 0;
+                            i = 1 + (t % fy) % (Nx - 2);
+                            k = (t % fy) / (Nx - 2);
+                        } else {
+                            const long long fz = (Nx - 2) * (Ny - 2);
+                            t -= 2 * fy;
+                            k = (t / fz) ? Nz - 1 : 0;
+                            i = 1 + (t % fz) % (Nx - 2);
+                            j = 1 + (t % fz) / (Nx - 2);
+                        }
+                        deps.push_back(DepGhost{static_cast<uint32_t>(i + Nx * (j + Ny * k)),
+                                                static_cast<int32_t>(entry_of.at(static_cast<uint64_t>(src)))});
+                    }
+                }
+                m_view_ndeps[l][f] = static_cast<long long>(deps.size()) - m_view_deps[l][f];
             }
         }
     }
@@ -422,7 +524,51 @@ void Domain::build_boundary_tables() {
     if (!params.empty()) {
         ARTSS_HIP(hipMalloc(&m_params_dev, sizeof(FaceParam) * params.size()));
         ARTSS_HIP(hipMemcpy(m_params_dev, params.data(), sizeof(FaceParam) * params.size(), hipMemcpyHostToDevice));
+        // the same table with every boundary value zero (linear part of the fill)
+        std::vector<FaceParam> hom = params;
+        for (FaceParam &q : hom) q.value = 0.0;
+        ARTSS_HIP(hipMalloc(&m_params_hom_dev, sizeof(FaceParam) * hom.size()));
+        ARTSS_HIP(hipMemcpy(m_params_hom_dev, hom.data(), sizeof(FaceParam) * hom.size(), hipMemcpyHostToDevice));
     }
+    if (!bits.empty()) {
+        ARTSS_HIP(hipMalloc(&m_bits_dev, sizeof(uint32_t) * bits.size()));
+        ARTSS_HIP(hipMemcpy(m_bits_dev, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice));
+    }
+    if (!deps.empty()) {
+        ARTSS_HIP(hipMalloc(&m_deps_dev, sizeof(DepGhost) * deps.size()));
+        ARTSS_HIP(hipMemcpy(m_deps_dev, deps.data(), sizeof(DepGhost) * deps.size(), hipMemcpyHostToDevice));
+    }
+}
+
+BCView Domain::bc_view(int level, int ft, bool homogeneous) const {
+    if (!m_idem[level][ft]) throw std::logic_error("bc_view of a boundary fill that is not idempotent");
+    BCView v{};
+    const auto &W = m_waves[level][ft];
+    v.faces = W.empty() ? nullptr : m_faces_dev + W[0].offset;
+    v.n_faces = W.empty() ? 0 : W[0].count;
+    FaceParam *params = homogeneous ? m_params_hom_dev : m_params_dev;
+    v.params = params ? params + m_param_offset[level][ft] : nullptr;
+    v.deps = m_deps_dev ? m_deps_dev + m_view_deps[level][ft] : nullptr;
+    v.n_deps = m_view_ndeps[level][ft];
+    v.wbits = m_bits_dev + m_view_bits[level][ft];
+    v.bc = m_dom_bc[level][ft];
+    if (homogeneous)
+        for (double &x : v.bc.value) x = 0.0;
+    v.n_ghosts = v.bc.active ? ghost_cells(level) : 0;
+    return v;
+}
+
+bool Domain::bc_values_zero(int level, int ft) const {
+    const DomainBC &d = m_dom_bc[level][ft];
+    for (int p = 0; p < 6; p++)
+        if (d.active && d.value[p] != 0.0) return false;
+    return m_face_values_zero[level][ft] != 0;
+}
+
+long long Domain::ghost_cells(int level) const {
+    const LevelDesc &L = m_level[level];
+    const long long Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
+    return 2 * Ny * Nz + 2 * (Nx - 2) * Nz + 2 * (Nx - 2) * (Ny - 2);
 }
 
 void Domain::apply_boundary(double *f, int ft, int level) {
@@ -435,10 +581,51 @@ void Domain::apply_boundary(double *f, int ft, int level) {
     timed("bc_domain", level, [&] { k::bc_domain(m_stream, m_level[level], m_dom_bc[level][ft], f); });
 }
 
-double Domain::reduce(const double *partial, int n) {
+void Domain::apply_boundary_merged(double *f, int ft, int level) {
+    require_device();
+    if (ft < 0 || ft >= NUMBER_OF_FIELD_TYPES) return;
+    if (!m_simple[level][ft]) throw std::logic_error("apply_boundary_merged on a fill that is not simple");
+    const auto &W = m_waves[level][ft];
+    const long long off = W.empty() ? 0 : W[0].offset, n = W.empty() ? 0 : W[0].count;
+    timed("bc_merged", level, [&] {
+        k::bc_merged(m_stream, m_level[level], m_dom_bc[level][ft], m_faces_dev + off, n,
+                     m_params_dev + m_param_offset[level][ft], f);
+    });
+}
+
+void Domain::apply_boundary_linear(double *f, int ft, int level) {
+    require_device();
+    if (ft < 0 || ft >= NUMBER_OF_FIELD_TYPES) return;
+    if (m_idem[level][ft]) {
+        timed("bc_merged", level, [&] { k::fill_bc(m_stream, m_level[level], bc_view(level, ft, true), f); });
+        return;
+    }
+    for (const Wave &w : m_waves[level][ft])
+        timed("bc_faces", level, [&] {
+            k::bc_faces(m_stream, m_faces_dev + w.offset, w.count, m_params_hom_dev + m_param_offset[level][ft], f);
+        });
+    DomainBC bc = m_dom_bc[level][ft];
+    for (double &x : bc.value) x = 0.0;
+    timed("bc_domain", level, [&] { k::bc_domain(m_stream, m_level[level], bc, f); });
+}
+
+void Domain::apply_faces(double *f, int ft, int level) {
+    require_device();
+    if (ft < 0 || ft >= NUMBER_OF_FIELD_TYPES) return;
+    for (const Wave &w : m_waves[level][ft])
+        timed("bc_faces", level, [&] {
+            k::bc_faces(m_stream, m_faces_dev + w.offset, w.count, m_params_dev + m_param_offset[level][ft], f);
+        });
+}
+
+void Domain::reduce_async(const double *partial, int n) {
     require_device();
     k::reduce_sum(m_stream, partial, n, m_sum_dev);
     ARTSS_HIP(hipMemcpyAsync(m_sum_host, m_sum_dev, sizeof(double), hipMemcpyDeviceToHost, m_stream));
+}
+
+double Domain::reduce(const double *partial, int n) {
+    reduce_async(partial, n);
     ARTSS_HIP(hipStreamSynchronize(m_stream));
     return *m_sum_host;
 }
@@ -585,21 +772,29 @@ void Domain::timer_arm(const std::string &tag) {
     require_device();
     ARTSS_HIP(hipStreamSynchronize(m_stream));
     timer_clear();
+    m_timer_count = 0;
+    m_timer_ms = 0.0;
     m_timer_tag = tag;
+}
+
+void Domain::timer_add(const EventPairs &pairs) {
+    for (const auto &e : pairs) {
+        float x = 0.f;
+        ARTSS_HIP(hipEventElapsedTime(&x, e.first, e.second));
+        m_timer_ms += x;
+        m_timer_count++;
+    }
 }
 
 void Domain::timer_read(long long *launches, double *total_ms) {
     require_device();
     ARTSS_HIP(hipStreamSynchronize(m_stream));
-    double ms = 0.0;
-    for (auto &e : m_timer_events) {
-        float x = 0.f;
-        ARTSS_HIP(hipEventElapsedTime(&x, e.first, e.second));
-        ms += x;
-    }
-    *launches = static_cast<long long>(m_timer_events.size());
-    *total_ms = ms;
+    timer_add(m_timer_events);
+    *launches = m_timer_count;
+    *total_ms = m_timer_ms;
     timer_clear();
+    m_timer_count = 0;
+    m_timer_ms = 0.0;
     m_timer_tag.clear();
 }
 

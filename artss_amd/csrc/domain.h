@@ -53,8 +53,46 @@ class Domain {
     // faces first, then the six domain patches.
     void apply_boundary(double *f, int field_type, int level);
 
+    // A fill is simple when no write of it reads a cell that any of its
+    // writes changes: it then needs one wave of face writes, faces and
+    // ghosts can share one launch, and applying it twice equals applying it
+    // once.
+    bool bc_simple(int level, int field_type) const { return m_simple[level][field_type] != 0; }
+    // An idempotent fill: no face write reads a cell the fill writes (it
+    // needs one wave of faces, its result depends only on cells it does not
+    // write, and applying it twice equals applying it once).  Kernels can
+    // then produce the filled values themselves from bc_view.
+    bool bc_idempotent(int level, int field_type) const { return m_idem[level][field_type] != 0; }
+    // homogeneous: the same fill with every boundary value zero (its linear part)
+    BCView bc_view(int level, int field_type, bool homogeneous = false) const;
+    // the fill has no non-zero boundary value (it is linear)
+    bool bc_values_zero(int level, int field_type) const;
+    long long ghost_cells(int level) const;
+    // apply_boundary as one launch; only for simple fills
+    void apply_boundary_merged(double *f, int field_type, int level);
+    // apply_boundary with every boundary value zero (the fill's linear part)
+    void apply_boundary_linear(double *f, int field_type, int level);
+    // the obstacle-face part of apply_boundary alone
+    void apply_faces(double *f, int field_type, int level);
+
+    // boundary tables, for kernels that apply the fill themselves
+    const DomainBC &domain_bc(int level, int field_type) const { return m_dom_bc[level][field_type]; }
+    int face_waves(int level, int field_type) const { return static_cast<int>(m_waves[level][field_type].size()); }
+    long long face_wave_offset(int level, int field_type, int w) const {
+        return m_waves[level][field_type][w].offset;
+    }
+    long long face_wave_count(int level, int field_type, int w) const { return m_waves[level][field_type][w].count; }
+    const FaceEntry *faces_dev() const { return m_faces_dev; }
+    const FaceParam *params_dev(int level, int field_type) const {
+        return m_params_dev ? m_params_dev + m_param_offset[level][field_type] : nullptr;
+    }
+
     // fixed-order sum of `n` partials, blocking (one 8-byte readback)
     double reduce(const double *partial, int n);
+    // the same, enqueued only: the sum lands in pinned host memory, readable
+    // with reduced_value() once the stream has been synchronised
+    void reduce_async(const double *partial, int n);
+    double reduced_value() const { return *m_sum_host; }
     double *partial_buffer() const { return m_partial; }
     int partial_capacity() const { return m_partial_cap; }
 
@@ -72,7 +110,11 @@ class Domain {
 
     // Live kernel timing: while a tag is armed, every launch issued through
     // timed(tag, level, ...) is bracketed by HIP events on the domain stream.
+    // (Work replayed from HIP graphs is not bracketed: owners of graphs run
+    // eagerly while a tag is armed.)
+    using EventPairs = std::vector<std::pair<hipEvent_t, hipEvent_t>>;
     void timer_arm(const std::string &tag);
+    const std::string &timer_tag() const { return m_timer_tag; }
     template <typename F>
     void timed(const char *kernel, int level, F &&launch) {
         if (m_timer_tag.empty() || m_timer_tag != std::string(kernel) + "@" + std::to_string(level)) {
@@ -110,13 +152,23 @@ class Domain {
     };
     std::vector<std::vector<std::vector<Wave>>> m_waves;
     std::vector<std::vector<long long>> m_param_offset;  // into m_params_dev
+    std::vector<std::vector<char>> m_simple;             // [level][field type]
+    std::vector<std::vector<char>> m_idem;               // [level][field type]
+    std::vector<std::vector<char>> m_face_values_zero;   // [level][field type]
+    FaceParam *m_params_hom_dev = nullptr;
+    std::vector<std::vector<long long>> m_view_bits, m_view_deps, m_view_ndeps;
+    uint32_t *m_bits_dev = nullptr;
+    DepGhost *m_deps_dev = nullptr;
     long long m_face_entries = 0;
     FaceEntry *m_faces_dev = nullptr;
     FaceParam *m_params_dev = nullptr;
 
     std::string m_timer_tag;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> m_timer_events;
+    EventPairs m_timer_events;
+    long long m_timer_count = 0;
+    double m_timer_ms = 0.0;
     void timer_clear();
+    void timer_add(const EventPairs &pairs);
 
     double *m_partial = nullptr;
     int m_partial_cap = 0;

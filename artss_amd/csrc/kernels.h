@@ -23,6 +23,10 @@ void copy_domain(hipStream_t s, const LevelDesc &L, double *dst, const double *s
 // ---- boundary fill (Multigrid.cpp:555-572) ----
 void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p, double *f);
 void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f);
+// faces (one wave) and ghosts in one launch; only for a fill none of whose
+// writes reads a cell another write changes
+void bc_merged(hipStream_t s, const LevelDesc &L, const DomainBC &bc, const FaceEntry *e, long long n_faces,
+               const FaceParam *p, double *f);
 
 // ---- SLAdvect::advect (SLAdvect.cpp:71-163) ----
 void sl_advect(hipStream_t s, const LevelDesc &L, double dt, double *out, const double *in, const double *u,
@@ -37,6 +41,9 @@ void jacobi_diffuse_step(hipStream_t s, const LevelDesc &L, double *out, const d
 // multigrid form, multiplies by beta (:292-300)
 void jacobi_step_mg(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b, double ax,
                     double ay, double az, double beta, double dsign, double w);
+// first sweep with the initial copy folded in: dst = update(src) on inner cells, src elsewhere
+void jacobi_mg_first(hipStream_t s, const LevelDesc &L, double *dst, const double *src, const double *b, double ax,
+                     double ay, double az, double beta, double dsign, double w);
 
 // ---- colored Gauss-Seidel (ColoredGaussSeidelDiffuse.cpp:190-341), one color per launch ----
 // ev == nullptr: constant ax,ay,az,beta; else per-cell coefficients from D, ev, dt
@@ -52,6 +59,21 @@ void residual_partials(hipStream_t s, const LevelDesc &L, const double *x, const
                        double *partial);                                                          // :186-193
 void restrict_(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine);
 void prolongate(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse);
+// coarse = restrict(fine) on coarse inner cells and coarse_err = 0 on the whole coarse array
+void restrict_fill(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine,
+                   double *coarse_err);
+// fine += prolongate(coarse) on the cells the prolongation writes
+void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse);
+
+// ---- fused operator + boundary fill, for idempotent fills (device.h BCView) ----
+void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src);
+void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, double *dst, const double *src,
+               const double *b, double ax, double ay, double az, double beta, double dsign, double w);
+void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, double *out, const double *x,
+                    const double *b);
+void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, double *dst,
+                       const double *fine, const double *coarse);
+void fill_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *f);
 
 // ---- reductions ----
 int partial_count(const LevelDesc &L);  // number of partial slots a level-wide kernel writes

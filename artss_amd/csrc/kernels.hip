@@ -12,6 +12,7 @@
 // a single workgroup adds in a fixed order, so results are deterministic run
 // to run (their rounding differs from the serial left-to-right sum).
 #include "kernels.h"
+#include "stencil.h"
 
 #include <cmath>
 
@@ -42,9 +43,7 @@ __device__ inline bool inner_cell(const LevelDesc &L, int &i, int &j, int &k) {
     return i <= L.n[0] && j <= L.n[1] && !in_obstacle(L, i, j, k);
 }
 
-__device__ inline long long lin(const LevelDesc &L, long long i, long long j, long long k) {
-    return i + static_cast<long long>(L.N[0]) * (j + static_cast<long long>(L.N[1]) * k);
-}
+using dev::lin;
 
 // workgroup sum of one double per thread (64x4 block), result valid in thread (0,0)
 __device__ inline double block_sum(double v) {
@@ -99,10 +98,7 @@ __global__ void k_bc_faces(const FaceEntry *__restrict__ e, long long n, const F
                            double *__restrict__ f) {
     const long long t = blockIdx.x * 256LL + threadIdx.x;
     if (t >= n) return;
-    const FaceEntry x = e[t];
-    const FaceParam q = p[x.param];
-    if (!q.active) return;
-    f[x.idx] = q.sign * f[static_cast<long long>(x.idx) + x.ref] + q.value;
+    dev::face_write(e[t], p, f);
 }
 
 // DomainBoundary.cpp:25-49 for all six patches at once.  Each ghost cell is
@@ -110,40 +106,20 @@ __global__ void k_bc_faces(const FaceEntry *__restrict__ e, long long n, const F
 // of the patches containing it, applied in patch order to the inner cell the
 // chain of reference offsets ends on.
 __global__ void k_bc_domain(LevelDesc L, DomainBC bc, double *__restrict__ f) {
-    const long long Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
-    long long t = blockIdx.x * 256LL + threadIdx.x;
-    long long i, j, k;
-    const long long fx = Ny * Nz, fy = (Nx - 2) * Nz, fz = (Nx - 2) * (Ny - 2);
-    if (t < 2 * fx) {
-        const long long side = t / fx, r = t % fx;
-        i = side ? Nx - 1 : 0;
-        j = r % Ny;
-        k = r / Ny;
-    } else if ((t -= 2 * fx) < 2 * fy) {
-        const long long side = t / fy, r = t % fy;
-        j = side ? Ny - 1 : 0;
-        i = 1 + r % (Nx - 2);
-        k = r / (Nx - 2);
-    } else if ((t -= 2 * fy) < 2 * fz) {
-        const long long side = t / fz, r = t % fz;
-        k = side ? Nz - 1 : 0;
-        i = 1 + r % (Nx - 2);
-        j = 1 + r / (Nx - 2);
-    } else {
+    dev::domain_ghost(L, bc, f, blockIdx.x * 256LL + threadIdx.x);
+}
+
+// Obstacle faces and domain ghosts in one launch, for a boundary whose
+// writes read no cell that any of its writes changes (so faces and ghosts
+// can run in any order and the fill is idempotent).
+__global__ void k_bc_merged(LevelDesc L, DomainBC bc, const FaceEntry *__restrict__ e, long long n_faces,
+                            const FaceParam *__restrict__ p, double *__restrict__ f) {
+    const long long t = blockIdx.x * 256LL + threadIdx.x;
+    if (t < n_faces) {
+        dev::face_write(e[t], p, f);
         return;
     }
-    const int px = i == 0 ? 0 : (i == Nx - 1 ? 1 : -1);
-    const int py = j == 0 ? 2 : (j == Ny - 1 ? 3 : -1);
-    const int pz = k == 0 ? 4 : (k == Nz - 1 ? 5 : -1);
-    long long ii = i, jj = j, kk = k;
-    if (px >= 0) ii = bc.target[px];
-    if (py >= 0) jj = bc.target[py];
-    if (pz >= 0) kk = bc.target[pz];
-    double v = f[ii + Nx * (jj + Ny * kk)];
-    if (px >= 0) v = bc.sign[px] * v + bc.value[px];
-    if (py >= 0) v = bc.sign[py] * v + bc.value[py];
-    if (pz >= 0) v = bc.sign[pz] * v + bc.value[pz];
-    f[i + Nx * (j + Ny * k)] = v;
+    if (bc.active) dev::domain_ghost(L, bc, f, t - n_faces);
 }
 
 // ------------------------------------------------------------------ advection
@@ -252,10 +228,23 @@ __global__ void k_jacobi_mg(LevelDesc L, double ax, double ay, double az, double
     int i, j, k;
     if (!inner_cell(L, i, j, k)) return;
     const long long idx = lin(L, i, j, k);
-    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
-    const double out_h = beta * (dsign * b[idx] + ax * (in[idx + 1] + in[idx - 1]) +
-                                 ay * (in[idx + nj] + in[idx - nj]) + az * (in[idx + nk] + in[idx - nk]));
-    out[idx] = (1 - w) * in[idx] + w * out_h;
+    out[idx] = dev::jacobi_mg(L, ax, ay, az, beta, dsign, w, in, b, idx);
+}
+
+// First sweep of a Jacobi smoothing with the smoother's initial copy folded
+// in (JacobiDiffuse.cpp:85-99 + :292-300): every cell of dst is written,
+// inner cells with the update of src, all others with src itself.
+__global__ void k_jacobi_mg_first(LevelDesc L, double ax, double ay, double az, double beta, double dsign,
+                                  double w, double *__restrict__ dst, const double *__restrict__ src,
+                                  const double *__restrict__ b) {
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    if (i >= L.N[0] || j >= L.N[1]) return;
+    const long long idx = lin(L, i, j, k);
+    const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
+                       !in_obstacle(L, i, j, k);
+    dst[idx] = inner ? dev::jacobi_mg(L, ax, ay, az, beta, dsign, w, src, b, idx) : src[idx];
 }
 
 // ------------------------------------------------------------------ colored Gauss-Seidel
@@ -312,12 +301,7 @@ __global__ void k_update_norm(LevelDesc L, DiffCoef c, const double *__restrict_
 }
 
 // ------------------------------------------------------------------ multigrid
-__device__ inline double laplace(const LevelDesc &L, const double *__restrict__ x, long long i) {
-    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
-    const double rdx2 = 1. / (L.h[0] * L.h[0]), rdy2 = 1. / (L.h[1] * L.h[1]), rdz2 = 1. / (L.h[2] * L.h[2]);
-    return (rdx2 * (x[i - 1] - 2 * x[i] + x[i + 1]) + rdy2 * (x[i - nj] - 2 * x[i] + x[i + nj]) +
-            rdz2 * (x[i - nk] - 2 * x[i] + x[i + nk]));
-}
+using dev::laplace;
 
 // VCycleMG.cpp:346-354
 __global__ void k_residuum(LevelDesc L, double *__restrict__ out, const double *__restrict__ x,
@@ -346,32 +330,183 @@ __global__ void k_residual_partials(LevelDesc L, const double *__restrict__ x, c
 __global__ void k_restrict(LevelDesc Lc, LevelDesc Lf, double *__restrict__ out, const double *__restrict__ in) {
     int i, j, k;
     if (!inner_cell(Lc, i, j, k)) return;
-    const long long i2 = 2LL * i, j2 = 2LL * j, k2 = 2LL * k;
-    out[lin(Lc, i, j, k)] = 0.125 * (in[lin(Lf, i2 - 1, j2 - 1, k2 - 1)] + in[lin(Lf, i2, j2 - 1, k2 - 1)] +
-                                     in[lin(Lf, i2 - 1, j2, k2 - 1)] + in[lin(Lf, i2, j2, k2 - 1)] +
-                                     in[lin(Lf, i2 - 1, j2 - 1, k2)] + in[lin(Lf, i2, j2 - 1, k2)] +
-                                     in[lin(Lf, i2 - 1, j2, k2)] + in[lin(Lf, i2, j2, k2)]);
+    out[lin(Lc, i, j, k)] = dev::restrict_cell(Lf, in, i, j, k);
 }
 
-// VCycleMG.cpp:466-546: eight fine cells per coarse inner cell, weights
-// (27, 9, 9, 9, 3, 3, 3, 1) / 64 toward the neighbours on the fine cell's side.
+// VCycleMG.cpp:466-546: eight fine cells per coarse inner cell, in the
+// reference's write order
 __global__ void k_prolongate(LevelDesc Lc, LevelDesc Lf, double *__restrict__ out, const double *__restrict__ in) {
     int i, j, k;
     if (!inner_cell(Lc, i, j, k)) return;
     const long long idx = lin(Lc, i, j, k);
-    const long long ni = 1, nj = Lc.N[0], nk = static_cast<long long>(Lc.N[0]) * Lc.N[1];
-    const double c0 = in[idx];
+#pragma unroll
+    for (int q = 0; q < 8; q++) out[dev::prolong_target(Lf, i, j, k, q)] = dev::prolong_value(Lc, in, idx, q);
+}
+
+// Restriction (VCycleMG.cpp:404-421) fused with the reset of the coarse
+// error (VCycleMG.cpp:250): one thread per coarse cell of the full array.
+__global__ void k_restrict_fill(LevelDesc Lc, LevelDesc Lf, double *__restrict__ out, const double *__restrict__ in,
+                                double *__restrict__ err) {
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    if (i >= Lc.N[0] || j >= Lc.N[1]) return;
+    const long long idx = lin(Lc, i, j, k);
+    err[idx] = 0;
+    if (i >= 1 && j >= 1 && k >= 1 && i <= Lc.n[0] && j <= Lc.n[1] && k <= Lc.n[2] && !in_obstacle(Lc, i, j, k))
+        out[idx] = dev::restrict_cell(Lf, in, i, j, k);
+}
+
+// Prolongation added straight into the fine error (VCycleMG.cpp:266-270
+// without the intermediate field), for boundaries whose fill overwrites
+// every non-inner cell the addition would touch.
+__global__ void k_prolongate_add(LevelDesc Lc, LevelDesc Lf, double *__restrict__ out, const double *__restrict__ in) {
+    int i, j, k;
+    if (!inner_cell(Lc, i, j, k)) return;
+    const long long idx = lin(Lc, i, j, k);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        // reference order of the eight writes: (2i,2j,2k), (2i,2j,2k-1), (2i,2j-1,2k), (2i,2j-1,2k-1),
-        // (2i-1,2j,2k), (2i-1,2j,2k-1), (2i-1,2j-1,2k), (2i-1,2j-1,2k-1)
-        const int di = (q & 4) ? -1 : 1, dj = (q & 2) ? -1 : 1, dk = (q & 1) ? -1 : 1;
-        const long long oi = di * ni, oj = dj * nj, ok = dk * nk;
-        const double val = 0.015625 * (27 * c0 + 9 * in[idx + oi] + 9 * in[idx + oj] + 9 * in[idx + ok] +
-                                       3 * in[idx + oi + oj] + 3 * in[idx + oi + ok] + 3 * in[idx + oj + ok] +
-                                       in[idx + oi + oj + ok]);
-        out[lin(Lf, 2LL * i - (q & 4 ? 1 : 0), 2LL * j - (q & 2 ? 1 : 0), 2LL * k - (q & 1 ? 1 : 0))] = val;
+        const long long t = dev::prolong_target(Lf, i, j, k, q);
+        out[t] = out[t] + dev::prolong_value(Lc, in, idx, q);
     }
+}
+
+// ------------------------------------------------------------------ fused V-cycle kernels
+// Kernels that finish an operator and the boundary fill that follows it in
+// one launch.  They need an idempotent fill (BCView): every filled value is
+// a function of cells the fill does not write, so the fill's work items run
+// in extra workgroups of the same launch, each re-evaluating the operator at
+// the one cell it reads (the same expression, hence the same bits).
+
+// The extra workgroups are appended as additional z-slices of the grid.
+inline dim3 with_items(dim3 g, long long items) {
+    const long long per_slice = static_cast<long long>(g.x) * g.y * BX * BY;
+    return dim3(g.x, g.y, g.z + static_cast<unsigned>((items + per_slice - 1) / per_slice));
+}
+__device__ inline long long item_index(int main_z) {
+    return ((static_cast<long long>(blockIdx.z - main_z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) *
+               (BX * BY) + threadIdx.y * BX + threadIdx.x;
+}
+__device__ inline bool is_inner(const LevelDesc &L, long long c, int &i, int &j, int &k) {
+    const long long Nx = L.N[0], Ny = L.N[1];
+    k = static_cast<int>(c / (Nx * Ny));
+    j = static_cast<int>((c / Nx) % Ny);
+    i = static_cast<int>(c % Nx);
+    return i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k);
+}
+
+// dst = src with the fill applied (copy of the whole array + fill in one launch)
+__global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict__ dst,
+                          const double *__restrict__ src) {
+    if (static_cast<int>(blockIdx.z) >= main_z) {
+        dev::bc_item(L, V, item_index(main_z), dst, [&](long long c) { return src[c]; });
+        return;
+    }
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    if (i >= L.N[0] || j >= L.N[1]) return;
+    const long long idx = lin(L, i, j, k);
+    if (!dev::written(V, idx)) dst[idx] = src[idx];
+}
+
+// One Jacobi sweep of the multigrid smoother and the fill after it
+// (JacobiDiffuse.cpp:292-300 + VCycleMG.cpp:713).  first = 1: the smoother's
+// initial copy is folded in, every cell of dst is written (inner cells with
+// the update of src, filled cells by the fill, all others with src);
+// first = 0: cells that are neither inner nor filled keep dst's own values.
+__global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int first, double ax, double ay, double az,
+                            double beta, double dsign, double w, double *__restrict__ dst,
+                            const double *__restrict__ src, const double *__restrict__ b) {
+    if (static_cast<int>(blockIdx.z) >= main_z) {
+        dev::bc_item(L, V, item_index(main_z), dst, [&](long long c) {
+            int ci, cj, ck;
+            if (is_inner(L, c, ci, cj, ck)) return dev::jacobi_mg(L, ax, ay, az, beta, dsign, w, src, b, c);
+            return first ? src[c] : dst[c];
+        });
+        return;
+    }
+    int i, j, k;
+    if (first) {
+        i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+        j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+        k = static_cast<int>(blockIdx.z);
+        if (i >= L.N[0] || j >= L.N[1]) return;
+    } else {
+        i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+        j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+        k = 1 + static_cast<int>(blockIdx.z);
+        if (i > L.n[0] || j > L.n[1]) return;
+    }
+    const long long idx = lin(L, i, j, k);
+    const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
+                       !in_obstacle(L, i, j, k);
+    if (inner) dst[idx] = dev::jacobi_mg(L, ax, ay, az, beta, dsign, w, src, b, idx);
+    else if (first && !dev::written(V, idx)) dst[idx] = src[idx];
+}
+
+// Residuum (VCycleMG.cpp:346-354) and the obstacle-face part of the fill
+// after it (VCycleMG.cpp:245); the ghosts of the residuum are never read.
+__global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, double *__restrict__ out,
+                                 const double *__restrict__ x, const double *__restrict__ b) {
+    if (static_cast<int>(blockIdx.z) >= main_z) {
+        const long long t = item_index(main_z);
+        if (t >= V.n_faces) return;
+        dev::bc_item(L, V, t, out, [&](long long c) {
+            int ci, cj, ck;
+            if (is_inner(L, c, ci, cj, ck)) return b[c] - laplace(L, x, c);
+            return out[c];
+        });
+        return;
+    }
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    out[idx] = b[idx] - laplace(L, x, idx);
+}
+
+// The correction of the ascent (VCycleMG.cpp:266-270) and the two fills at
+// the start of the next smoothing, written out of place: dst = fine +
+// prolongate(coarse) on the fine cells, with the fill applied.  Cells the
+// prolongation does not reach get fine + 0.0, as the reference's whole-array
+// addition of a zero correction gives.
+__device__ inline double corrected(const LevelDesc &Lc, const LevelDesc &Lf, const double *__restrict__ fine,
+                                   const double *__restrict__ coarse, long long c) {
+    int i, j, k;
+    {
+        const long long Nx = Lf.N[0], Ny = Lf.N[1];
+        k = static_cast<int>(c / (Nx * Ny));
+        j = static_cast<int>((c / Nx) % Ny);
+        i = static_cast<int>(c % Nx);
+    }
+    double p = 0.0;
+    if (i >= 1 && j >= 1 && k >= 1) {
+        const int ci = (i + 1) / 2, cj = (j + 1) / 2, ck = (k + 1) / 2;
+        if (ci <= Lc.n[0] && cj <= Lc.n[1] && ck <= Lc.n[2] && !in_obstacle(Lc, ci, cj, ck)) {
+            const int q = ((i & 1) ? 4 : 0) | ((j & 1) ? 2 : 0) | ((k & 1) ? 1 : 0);
+            p = dev::prolong_value(Lc, coarse, lin(Lc, ci, cj, ck), q);
+        }
+    }
+    return fine[c] + p;
+}
+__global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, double *__restrict__ dst,
+                                    const double *__restrict__ fine, const double *__restrict__ coarse) {
+    if (static_cast<int>(blockIdx.z) >= main_z) {
+        dev::bc_item(Lf, V, item_index(main_z), dst,
+                     [&](long long c) { return corrected(Lc, Lf, fine, coarse, c); });
+        return;
+    }
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    if (i >= Lf.N[0] || j >= Lf.N[1]) return;
+    const long long idx = lin(Lf, i, j, k);
+    if (!dev::written(V, idx)) dst[idx] = corrected(Lc, Lf, fine, coarse, idx);
+}
+
+// One idempotent fill in a single launch (faces, ghosts, dependent ghosts)
+__global__ void k_fill_bc(LevelDesc L, BCView V, double *__restrict__ f) {
+    dev::bc_item(L, V, blockIdx.x * 256LL + threadIdx.x, f, [&](long long c) { return f[c]; });
 }
 
 __global__ void k_reduce_sum(const double *__restrict__ partial, int n, double *__restrict__ out) {
@@ -487,9 +622,15 @@ void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p
 }
 void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f) {
     if (!bc.active) return;
-    const long long Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
-    const long long n = 2 * Ny * Nz + 2 * (Nx - 2) * Nz + 2 * (Nx - 2) * (Ny - 2);
+    const long long n = dev::ghost_count(L);
     k_bc_domain<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, bc, f);
+}
+
+void bc_merged(hipStream_t s, const LevelDesc &L, const DomainBC &bc, const FaceEntry *e, long long n_faces,
+               const FaceParam *p, double *f) {
+    const long long n = n_faces + (bc.active ? dev::ghost_count(L) : 0);
+    if (n <= 0) return;
+    k_bc_merged<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, bc, e, n_faces, p, f);
 }
 
 void sl_advect(hipStream_t s, const LevelDesc &L, double dt, double *out, const double *in, const double *u,
@@ -506,6 +647,11 @@ void jacobi_diffuse_step(hipStream_t s, const LevelDesc &L, double *out, const d
 void jacobi_step_mg(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b, double ax,
                     double ay, double az, double beta, double dsign, double w) {
     k_jacobi_mg<<<inner_grid(L), block(), 0, s>>>(L, ax, ay, az, beta, dsign, w, out, in, b);
+}
+
+void jacobi_mg_first(hipStream_t s, const LevelDesc &L, double *dst, const double *src, const double *b, double ax,
+                     double ay, double az, double beta, double dsign, double w) {
+    k_jacobi_mg_first<<<full_grid(L), block(), 0, s>>>(L, ax, ay, az, beta, dsign, w, dst, src, b);
 }
 
 void cgs_color(hipStream_t s, const LevelDesc &L, int color, double *out, const double *b, double ax, double ay,
@@ -532,6 +678,41 @@ void restrict_(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *
 }
 void prolongate(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse) {
     k_prolongate<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, fine, coarse);
+}
+
+void restrict_fill(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine,
+                   double *coarse_err) {
+    k_restrict_fill<<<full_grid(Lc), block(), 0, s>>>(Lc, Lf, coarse, fine, coarse_err);
+}
+void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse) {
+    k_prolongate_add<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, fine, coarse);
+}
+
+void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src) {
+    const dim3 g = full_grid(L);
+    k_copy_bc<<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), dst, src);
+}
+void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, double *dst, const double *src,
+               const double *b, double ax, double ay, double az, double beta, double dsign, double w) {
+    const dim3 g = first ? full_grid(L) : inner_grid(L);
+    k_jacobi_bc<<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), first ? 1 : 0, ax,
+                                                                     ay, az, beta, dsign, w, dst, src, b);
+}
+void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, double *out, const double *x,
+                    const double *b) {
+    const dim3 g = inner_grid(L);
+    k_residuum_faces<<<with_items(g, V.n_faces), block(), 0, s>>>(L, V, static_cast<int>(g.z), out, x, b);
+}
+void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, double *dst,
+                       const double *fine, const double *coarse) {
+    const dim3 g = full_grid(Lf);
+    k_prolongate_add_bc<<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z), dst,
+                                                                             fine, coarse);
+}
+void fill_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *f) {
+    const long long n = dev::bc_items(V);
+    if (n <= 0) return;
+    k_fill_bc<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, V, f);
 }
 
 int partial_count(const LevelDesc &L) {

@@ -1,0 +1,210 @@
+// Kernels of the Krylov-accelerated pressure solve (krylov.cpp).
+//
+// GCR with one previous direction, right-preconditioned by a V-cycle: per
+// iteration w = A z (A: the discrete Laplacian with the linear part of the
+// boundary fill), four dot products, one coefficient kernel and one fused
+// update.  FP64 throughout; the preconditioned direction z may be FP32.
+// Dot products are per-workgroup partial sums added by one workgroup in a
+// fixed order, so runs are deterministic.
+#include <cmath>
+
+#include "krylov_kernels.h"
+#include "stencil.h"
+
+namespace artss {
+namespace kk {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+inline dim3 inner_grid(const LevelDesc &L) {
+    return dim3((L.n[0] + BX - 1) / BX, (L.n[1] + BY - 1) / BY, L.n[2]);
+}
+inline dim3 full_grid(const LevelDesc &L) {
+    return dim3((L.N[0] + BX - 1) / BX, (L.N[1] + BY - 1) / BY, L.N[2]);
+}
+
+__device__ inline int block_id() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+__device__ inline int block_count() { return gridDim.x * gridDim.y * gridDim.z; }
+
+// workgroup sums of Q values per thread, written to partial[q * blocks + block]
+template <int Q>
+__device__ inline void write_partials(double (&v)[Q], double *partial) {
+    __shared__ double wsum[Q][BY];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        double x = v[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (threadIdx.x == 0) wsum[q][threadIdx.y] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            double s = 0.0;
+            for (int w = 0; w < BY; w++) s += wsum[q][w];
+            partial[q * block_count() + block_id()] = s;
+        }
+    }
+}
+
+__device__ inline bool inner_at(const LevelDesc &L, int &i, int &j, int &k) {
+    i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    k = 1 + static_cast<int>(blockIdx.z);
+    return i <= L.n[0] && j <= L.n[1] && !in_obstacle(L, i, j, k);
+}
+
+template <typename T>
+__device__ inline double lap(const LevelDesc &L, const T *__restrict__ x, long long i) {
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const double rdx2 = 1. / (L.h[0] * L.h[0]), rdy2 = 1. / (L.h[1] * L.h[1]), rdz2 = 1. / (L.h[2] * L.h[2]);
+    const double c = x[i];
+    return rdx2 * (double(x[i - 1]) - 2 * c + double(x[i + 1])) + rdy2 * (double(x[i - nj]) - 2 * c + double(x[i + nj])) +
+           rdz2 * (double(x[i - nk]) - 2 * c + double(x[i + nk]));
+}
+
+// r = b - lap(x) on inner cells, partial (r, r)
+__global__ void k_residual_vec(LevelDesc L, double *__restrict__ r, const double *__restrict__ x,
+                               const double *__restrict__ b, double *__restrict__ partial) {
+    int i, j, k;
+    double v[1] = {0.0};
+    if (inner_at(L, i, j, k)) {
+        const long long idx = dev::lin(L, i, j, k);
+        const double x_ = b[idx] - dev::laplace(L, x, idx);
+        r[idx] = x_;
+        v[0] = x_ * x_;
+    }
+    write_partials<1>(v, partial);
+}
+
+// w = lap(z) on inner cells; partials (w,w), (w,wp), (r,w), (r,wp)
+template <typename Tz>
+__global__ void k_lap_dots(LevelDesc L, double *__restrict__ w, const Tz *__restrict__ z,
+                           const double *__restrict__ wp, const double *__restrict__ r, double *__restrict__ partial) {
+    int i, j, k;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (inner_at(L, i, j, k)) {
+        const long long idx = dev::lin(L, i, j, k);
+        const double x = lap(L, z, idx);
+        w[idx] = x;
+        const double p = wp[idx], q = r[idx];
+        v[0] = x * x;
+        v[1] = x * p;
+        v[2] = q * x;
+        v[3] = q * p;
+    }
+    write_partials<4>(v, partial);
+}
+
+// coef[0] = beta, coef[1] = 1/|w - beta wp|, coef[2] = alpha
+__global__ void k_gcr_coef(const double *__restrict__ partial, int blocks, double *__restrict__ coef) {
+    __shared__ double s[4][16];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < blocks; t += 1024)
+        for (int q = 0; q < 4; q++) acc[q] += partial[q * blocks + t];
+    for (int q = 0; q < 4; q++) {
+        double x = acc[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((threadIdx.x & 63) == 0) s[q][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S[4];
+        for (int q = 0; q < 4; q++) {
+            S[q] = 0.0;
+            for (int w = 0; w < 16; w++) S[q] += s[q][w];
+        }
+        double beta = S[1];
+        double nw2 = S[0] - beta * beta;
+        if (!(nw2 > 1e-6 * S[0])) {  // w (almost) parallel to wp: restart without it
+            beta = 0.0;
+            nw2 = S[0];
+        }
+        const double inv = nw2 > 0.0 ? 1.0 / sqrt(nw2) : 0.0;
+        coef[0] = beta;
+        coef[1] = inv;
+        coef[2] = (S[2] - beta * S[3]) * inv;
+    }
+}
+
+// z' = (z - beta zp)/|w'|, w' = (w - beta wp)/|w'|; x += alpha z'; r -= alpha w';
+// zp = z', wp = w' over the whole array; partial (r, r)
+template <typename Tz>
+__global__ void k_gcr_update(LevelDesc L, const double *__restrict__ coef, double *__restrict__ x,
+                             double *__restrict__ r, const Tz *__restrict__ z, double *__restrict__ zp,
+                             const double *__restrict__ w, double *__restrict__ wp, double *__restrict__ partial) {
+    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = static_cast<int>(blockIdx.z);
+    double v[1] = {0.0};
+    if (i < L.N[0] && j < L.N[1]) {
+        const long long idx = dev::lin(L, i, j, k);
+        const double beta = coef[0], inv = coef[1], alpha = coef[2];
+        const double zn = (double(z[idx]) - beta * zp[idx]) * inv;
+        const double wn = (w[idx] - beta * wp[idx]) * inv;
+        x[idx] += alpha * zn;
+        const double rn = r[idx] - alpha * wn;
+        r[idx] = rn;
+        zp[idx] = zn;
+        wp[idx] = wn;
+        v[0] = rn * rn;
+    }
+    write_partials<1>(v, partial);
+}
+
+__global__ void k_sum(const double *__restrict__ partial, int n, double *__restrict__ out) {
+    __shared__ double s[16];
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += partial[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < 16; w++) r += s[w];
+        out[0] = r;
+    }
+}
+
+}  // namespace
+
+int blocks_inner(const LevelDesc &L) {
+    const dim3 g = inner_grid(L);
+    return static_cast<int>(g.x * g.y * g.z);
+}
+int blocks_full(const LevelDesc &L) {
+    const dim3 g = full_grid(L);
+    return static_cast<int>(g.x * g.y * g.z);
+}
+
+void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x, const double *b, double *partial) {
+    k_residual_vec<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, r, x, b, partial);
+}
+template <typename Tz>
+void lap_dots(hipStream_t s, const LevelDesc &L, double *w, const Tz *z, const double *wp, const double *r,
+              double *partial) {
+    k_lap_dots<Tz><<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, w, z, wp, r, partial);
+}
+template void lap_dots<double>(hipStream_t, const LevelDesc &, double *, const double *, const double *,
+                               const double *, double *);
+template void lap_dots<float>(hipStream_t, const LevelDesc &, double *, const float *, const double *,
+                              const double *, double *);
+void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef) {
+    k_gcr_coef<<<1, 1024, 0, s>>>(partial, blocks, coef);
+}
+template <typename Tz>
+void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, double *zp,
+                const double *w, double *wp, double *partial) {
+    k_gcr_update<Tz><<<full_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, w, wp, partial);
+}
+template void gcr_update<double>(hipStream_t, const LevelDesc &, const double *, double *, double *, const double *,
+                                 double *, const double *, double *, double *);
+template void gcr_update<float>(hipStream_t, const LevelDesc &, const double *, double *, double *, const float *,
+                                double *, const double *, double *, double *);
+void sum(hipStream_t s, const double *partial, int n, double *out) { k_sum<<<1, 1024, 0, s>>>(partial, n, out); }
+
+}  // namespace kk
+}  // namespace artss

@@ -210,12 +210,18 @@ void Solver::pressure_step(double t) {
         m_counters.vcycles = m_pres->last_cycles();
         return;
     }
+    if (m_pmode.method == 1) {
+        if (!m_gcr) m_gcr = std::make_unique<GcrPressure>(*m_domain, *m_pres);
+        double r = 0.0;
+        m_counters.vcycles = m_gcr->solve(f.p, f.rhs, m_pmode.tolerance, m_pmode.max_cycles, &r);
+        m_counters.residual = r;
+        return;
+    }
     double r = m_pres->residual(f.p, f.rhs);
     int cycles = 0;
     while (r > m_pmode.tolerance && cycles < m_pmode.max_cycles) {
-        m_pres->vcycle(f.p, f.rhs);
+        r = m_pres->vcycle_residual(f.p, f.rhs);
         cycles++;
-        r = m_pres->residual(f.p, f.rhs);
     }
     m_counters.vcycles = cycles;
     m_counters.residual = r;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "operators.h"
+#include "krylov.h"
 #include "vcycle.h"
 
 namespace artss {
@@ -44,6 +45,9 @@ struct PressureMode {
     bool to_tolerance = false;
     double tolerance = 1e-8;
     int max_cycles = 200;
+    // with to_tolerance: 0 V-cycles on their own (the reference's
+    // iteration), 1 V-cycles as the preconditioner of GCR (krylov.h)
+    int method = 0;
 };
 
 class Solver {
@@ -58,6 +62,7 @@ class Solver {
     void step(double t);
     const StepCounters &counters() const { return m_counters; }
     void set_pressure_mode(const PressureMode &m) { m_pmode = m; }
+    const PressureMode &pressure_mode() const { return m_pmode; }
     VCycleMG &pressure_solver() { return *m_pres; }
 
   private:
@@ -74,6 +79,7 @@ class Solver {
     std::unique_ptr<IDiffusion> m_dif_temp;
     std::unique_ptr<ITurbulence> m_turb;
     std::unique_ptr<VCycleMG> m_pres;
+    std::unique_ptr<GcrPressure> m_gcr;
     std::unique_ptr<ExplicitEulerSource> m_src_vel, m_src_temp;
     std::unique_ptr<GaussFunction> m_gauss;
     int m_kind = 0;  // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver

@@ -1,6 +1,9 @@
 #include "vcycle.h"
 
 #include <cmath>
+#include <cstring>
+
+#include "mg_tail.h"
 
 namespace artss {
 
@@ -24,9 +27,123 @@ VCycleMG::VCycleMG(Domain &d, const MultigridSettings &s) : IPressure(d), m_s(s)
         m_err1[l + 1] = std::make_unique<Field>(d, P, 0.0, l + 1);
         m_tmp[l + 1] = std::make_unique<Field>(d, P, 0.0, l + 1);
     }
+    if (d.device() >= 0) build_tail();
 }
 
-VCycleMG::~VCycleMG() = default;
+VCycleMG::~VCycleMG() {
+    drop_graphs();
+    if (m_plan_dev) (void)hipFree(m_plan_dev);
+}
+
+void VCycleMG::set_schedule(bool fused, long long tail_cells) {
+    drop_graphs();
+    m_fused = fused;
+    m_tail_cells = tail_cells < 0 ? 0 : tail_cells;
+    build_tail();
+}
+
+void VCycleMG::set_graphs(bool on) {
+    drop_graphs();
+    m_use_graphs = on;
+}
+
+void VCycleMG::drop_graphs() {
+    for (CycleGraph &g : m_graph) {
+        if (g.exec) {
+            (void)hipStreamSynchronize(m_domain.stream());
+            (void)hipGraphExecDestroy(g.exec);
+        }
+        g = CycleGraph();
+    }
+}
+
+// Capture (once per out/b pair) and replay one V-cycle,
+// optionally followed by the residual norm.  The capture records exactly
+// the launches vcycle_fused issues eagerly; the buffer exchanges it makes on
+// the host cancel out over one V-cycle, so every replay sees the pointers
+// the capture saw.
+void VCycleMG::launch_graph(int which, Field &out, const Field &b) {
+    Domain &d = m_domain;
+    CycleGraph &g = m_graph[which];
+    if (!g.exec || g.out != out.data || g.b != b.data) {
+        if (g.exec) {
+            ARTSS_HIP(hipStreamSynchronize(d.stream()));
+            ARTSS_HIP(hipGraphExecDestroy(g.exec));
+        }
+        g = CycleGraph();
+        ARTSS_HIP(hipStreamBeginCapture(d.stream(), hipStreamCaptureModeRelaxed));
+        try {
+            vcycle_fused(out, b);
+            if (which == 1) residual_async(out, b);
+        } catch (...) {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(d.stream(), &broken);
+            if (broken) (void)hipGraphDestroy(broken);
+            throw;
+        }
+        hipGraph_t graph = nullptr;
+        ARTSS_HIP(hipStreamEndCapture(d.stream(), &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        ARTSS_HIP(e);
+        g.out = out.data;
+        g.b = b.data;
+    }
+    ARTSS_HIP(hipGraphLaunch(g.exec, d.stream()));
+}
+
+// Choose the single-workgroup levels and upload their plan.  The kernel
+// covers levels first..n_level with first <= n_level - 1 (it always
+// includes the coarse solve); levels whose arrays exceed tail_cells, or
+// whose boundary fill needs more waves than the plan holds, stay on the
+// multi-workgroup path.
+void VCycleMG::build_tail() {
+    Domain &d = m_domain;
+    const int L = m_s.n_level;
+    m_tail_first = L;
+    if (!m_fused || L == 0 || m_tail_cells == 0) return;
+    int f = L;
+    while (f > 0) {
+        const int l = f - 1;
+        if (d.level(l).size > m_tail_cells || d.face_waves(l, P) > kMaxTailWaves) break;
+        f = l;
+    }
+    if (d.level(L).size > m_tail_cells || d.face_waves(L, P) > kMaxTailWaves) f = L;
+    if (f > L - 1 || L - f + 1 > kMaxTailLevels) return;
+    TailPlan plan;
+    std::memset(&plan, 0, sizeof(plan));
+    plan.first = f;
+    plan.last = L;
+    plan.n_relax = m_s.n_relax;
+    plan.cgs = m_cgs ? 1 : 0;
+    plan.w = m_s.smoother.w;
+    plan.dsign = kDsign;
+    plan.max_iter = m_s.smoother.max_iter;
+    plan.tol = m_s.smoother.tol_res;
+    const LevelDesc &Ls = d.level(L - 1);
+    plan.solve_smooths = (Ls.N[0] <= 4 && Ls.N[1] <= 4) ? 1 : 0;
+    for (int l = f; l <= L; l++) {
+        TailLevel &T = plan.lv[l - f];
+        T.L = d.level(l);
+        T.bc = d.domain_bc(l, P);
+        T.faces = d.faces_dev();
+        T.params = d.params_dev(l, P);
+        T.n_waves = d.face_waves(l, P);
+        for (int w = 0; w < T.n_waves; w++) {
+            T.wave_start[w] = d.face_wave_offset(l, P, w);
+            T.wave_start[w + 1] = T.wave_start[w] + d.face_wave_count(l, P, w);
+        }
+        coeffs(l, &T.ax, &T.ay, &T.az, &T.beta);
+        T.err1 = m_err1[l]->data;
+        T.tmp = m_tmp[l]->data;
+        T.res1 = l > 0 ? m_res1[l]->data : nullptr;
+        T.res0 = l < L ? m_res0[l]->data : nullptr;
+        T.err0 = l < L ? m_err0[l]->data : nullptr;
+    }
+    if (!m_plan_dev) ARTSS_HIP(hipMalloc(&m_plan_dev, sizeof(TailPlan)));
+    ARTSS_HIP(hipMemcpy(m_plan_dev, &plan, sizeof(TailPlan), hipMemcpyHostToDevice));
+    m_tail_first = f;
+}
 
 void VCycleMG::coeffs(int level, double *ax, double *ay, double *az, double *beta) const {
     const LevelDesc &L = m_domain.level(level);
@@ -45,6 +162,23 @@ double VCycleMG::level_residual(const Field &out, const Field &b, int level) {
 }
 
 double VCycleMG::residual(const Field &out, const Field &b) { return level_residual(out, b, 0); }
+
+void VCycleMG::residual_async(const Field &out, const Field &b) {
+    const LevelDesc &L = m_domain.level(0);
+    m_domain.timed("residual", 0,
+                   [&] { k::residual_partials(m_domain.stream(), L, out.data, b.data, m_domain.partial_buffer()); });
+    m_domain.reduce_async(m_domain.partial_buffer(), k::partial_count(L));
+}
+
+double VCycleMG::vcycle_residual(Field &out, const Field &b) {
+    if (graphable()) {
+        launch_graph(1, out, b);
+        ARTSS_HIP(hipStreamSynchronize(m_domain.stream()));
+        return std::sqrt(m_domain.reduced_value());
+    }
+    vcycle(out, b);
+    return residual(out, b);
+}
 
 // VCycleMG.cpp:685-723.  The smoother swaps data pointers after each sweep;
 // with an even number of sweeps the returned iterate is the one before the
@@ -153,16 +287,201 @@ void VCycleMG::solve(Field &out, Field &tmp, const Field &b, int level) {
     else solve_jacobi(out, tmp, b, level);
 }
 
-// VCycleMG::VCycleMultigrid :215-288.  residuum1[0] is the right-hand side
-// itself (the reference copies b there and never writes it), and the copy
-// of error0[0] into out at :254 is dead (out is overwritten at :282).
 void VCycleMG::vcycle(Field &out, const Field &b) {
-    Domain &d = m_domain;
-    const int L = m_s.n_level;
-    if (L == 0) {
+    if (m_s.n_level == 0) {
         solve(out, *m_tmp[0], b, 0);
         return;
     }
+    if (graphable()) launch_graph(0, out, b);
+    else if (m_fused) vcycle_fused(out, b);
+    else vcycle_plain(out, b);
+}
+
+void VCycleMG::vcycle_eager(Field &out, const Field &b) {
+    if (m_s.n_level == 0) solve(out, *m_tmp[0], b, 0);
+    else if (m_fused) vcycle_fused(out, b);
+    else vcycle_plain(out, b);
+}
+
+// Boundary fill of a pressure-type field on `level`: one launch when the
+// fill is simple, the reference's face waves + ghosts otherwise.
+void VCycleMG::bc(double *f, int level) {
+    Domain &d = m_domain;
+    if (d.bc_simple(level, P)) {
+        d.apply_boundary_merged(f, P, level);
+    } else if (d.bc_idempotent(level, P)) {
+        d.timed("bc_merged", level, [&] { k::fill_bc(d.stream(), d.level(level), d.bc_view(level, P), f); });
+    } else {
+        d.apply_boundary(f, P, level);
+    }
+}
+
+// Levels on the fully fused path: weighted-Jacobi smoother with at least one
+// effective sweep and an idempotent fill.  There every sweep carries the
+// fill after it, and a smoothing starts from tmp and ends in err1 (the
+// effective sweep count is odd), so no buffers are exchanged.
+bool VCycleMG::fast_level(int level) const {
+    const int n = m_s.n_relax;
+    const int effective = (n % 2 == 0 && n > 0) ? n - 1 : n;
+    return !m_cgs && effective > 0 && m_domain.bc_idempotent(level, P);
+}
+
+// Smoothing of err1[level] whose initial iterate (with the fill applied)
+// is in tmp[level]: sweep 1 tmp -> err1 with the copy folded in, then
+// alternating, the last one into err1.
+void VCycleMG::smooth_fast(int level, const double *b) {
+    Domain &d = m_domain;
+    double ax, ay, az, beta;
+    coeffs(level, &ax, &ay, &az, &beta);
+    const LevelDesc &Ld = d.level(level);
+    const BCView V = d.bc_view(level, P);
+    const int n = m_s.n_relax;
+    const int effective = (n % 2 == 0 && n > 0) ? n - 1 : n;
+    double *src = m_tmp[level]->data, *dst = m_err1[level]->data;
+    for (int i = 0; i < effective; i++) {
+        d.timed("jacobi_mg", level, [&] {
+            k::jacobi_bc(d.stream(), Ld, V, i == 0, dst, src, b, ax, ay, az, beta, kDsign, m_s.smoother.w);
+        });
+        std::swap(src, dst);
+    }
+}
+
+// VCycleMG::Smooth on err1[level] with launches folded:
+//  - the smoother's own fill right after Smooth's is dropped when the fill is
+//    simple (applying it twice equals applying it once);
+//  - Jacobi: the copy tmp <- out and the first sweep out <- J(tmp) become one
+//    kernel writing J(out) on inner cells and out elsewhere into tmp; the
+//    sweeps then alternate between the two buffers and the odd count leaves
+//    the result in tmp, so the two Field objects trade buffers.  Both buffers
+//    hold exactly what the reference leaves in them, with the roles exchanged.
+void VCycleMG::smooth_fused(int level, const double *b) {
+    Domain &d = m_domain;
+    Field &out = *m_err1[level];
+    Field &tmp = *m_tmp[level];
+    bc(out.data, level);
+    if (!d.bc_idempotent(level, P)) d.apply_boundary(out.data, P, level);
+    double ax, ay, az, beta;
+    coeffs(level, &ax, &ay, &az, &beta);
+    const LevelDesc &Ld = d.level(level);
+    if (m_cgs) {
+        for (int i = 0; i < m_s.n_relax; i++) {
+            for (int color = 0; color < 2; color++)
+                d.timed("cgs_mg", level, [&] {
+                    k::cgs_color(d.stream(), Ld, color, out.data, b, ax, ay, az, beta, kDsign, m_s.smoother.w, 0.0,
+                                 nullptr, 0.0);
+                });
+            bc(out.data, level);
+        }
+        return;
+    }
+    const int n = m_s.n_relax;
+    const int effective = (n % 2 == 0 && n > 0) ? n - 1 : n;  // odd, or 0
+    if (effective == 0) return;
+    d.timed("jacobi_mg", level, [&] {
+        k::jacobi_mg_first(d.stream(), Ld, tmp.data, out.data, b, ax, ay, az, beta, kDsign, m_s.smoother.w);
+    });
+    bc(tmp.data, level);
+    double *src = tmp.data, *dst = out.data;
+    for (int i = 1; i < effective; i++) {
+        d.timed("jacobi_mg", level, [&] {
+            k::jacobi_step_mg(d.stream(), Ld, dst, src, b, ax, ay, az, beta, kDsign, m_s.smoother.w);
+        });
+        bc(dst, level);
+        std::swap(src, dst);
+    }
+    Field::swap(out, tmp);  // the last sweep wrote into tmp's buffer
+}
+
+// VCycleMG::VCycleMultigrid with the fused schedule.  Besides the folds of
+// smooth_fused, exact by construction:
+//  - the fill of residuum1 after Restrict (:248) is dropped: only inner cells
+//    of residuum1 are ever read;
+//  - for a simple fill, the ghost part of the fill of residuum0 (:245) is
+//    dropped (Restrict reads no ghost), Restrict and the reset of the coarse
+//    error share one kernel, Prolongate + fill + correction (:266-270) become
+//    one kernel adding into error1 (every cell the fill of error0 writes is
+//    overwritten by the fill at the start of the next Smooth/Solve before it
+//    is read), and the final fill of `out` (:283) is dropped (out already
+//    satisfies it);
+//  - the levels from tail_first down run in one single-workgroup kernel.
+void VCycleMG::vcycle_fused(Field &out, const Field &b) {
+    Domain &d = m_domain;
+    const int L = m_s.n_level;
+    const int f = m_tail_first;
+    // the buffer the smoothing of `level` starts from
+    auto start = [&](int level) -> Field & { return level < f && fast_level(level) ? *m_tmp[level] : *m_err1[level]; };
+    if (f > 0 && fast_level(0)) {
+        // err1 <- out and the two fills of Smooth fold into tmp <- fill(out)
+        d.timed("copy_bc", 0, [&] { k::copy_bc(d.stream(), d.level(0), d.bc_view(0, P), m_tmp[0]->data, out.data); });
+    } else {
+        m_err1[0]->copy_data(out);
+    }
+    for (int level = 0; level < f; ++level) {
+        const double *rhs = level == 0 ? b.data : m_res1[level]->data;
+        // at levels > 0 the smoothing starts from zero, which every fill
+        // (all values are zero below level 0) maps to zero: fills elided
+        if (fast_level(level)) smooth_fast(level, rhs);
+        else smooth_fused(level, rhs);
+        if (d.bc_idempotent(level, P)) {
+            d.timed("residuum", level, [&] {
+                k::residuum_faces(d.stream(), d.level(level), d.bc_view(level, P), m_res0[level]->data,
+                                  m_err1[level]->data, rhs);
+            });
+        } else {
+            d.timed("residuum", level, [&] {
+                k::residuum(d.stream(), d.level(level), m_res0[level]->data, m_err1[level]->data, rhs);
+            });
+            d.apply_boundary(m_res0[level]->data, P, level);
+        }
+        d.timed("restrict", level, [&] {
+            k::restrict_fill(d.stream(), d.level(level + 1), d.level(level), m_res1[level + 1]->data,
+                             m_res0[level]->data, start(level + 1).data);
+        });
+    }
+    if (f < L) {
+        d.timed("mg_tail", f, [&] { mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data); });
+    }
+    for (int level = f; level > 0; --level) {
+        const int fine = level - 1;
+        const double *rhs = fine == 0 ? b.data : m_res1[fine]->data;
+        if (level != L && fast_level(fine)) {
+            d.timed("prolongate", fine, [&] {
+                k::prolongate_add_bc(d.stream(), d.level(level), d.level(fine), d.bc_view(fine, P),
+                                     m_tmp[fine]->data, m_err1[fine]->data, m_err1[level]->data);
+            });
+            smooth_fast(fine, rhs);
+            continue;
+        }
+        if (d.bc_idempotent(fine, P)) {
+            d.timed("prolongate", fine, [&] {
+                k::prolongate_add(d.stream(), d.level(level), d.level(fine), m_err1[fine]->data,
+                                  m_err1[level]->data);
+            });
+        } else {
+            d.timed("prolongate", fine, [&] {
+                k::prolongate(d.stream(), d.level(level), d.level(fine), m_err0[fine]->data, m_err1[level]->data);
+            });
+            d.apply_boundary(m_err0[fine]->data, P, fine);
+            *m_err1[fine] += *m_err0[fine];
+        }
+        if (level == L) {
+            Field rhs_view(d, P, fine, const_cast<double *>(rhs));
+            solve(*m_err1[fine], *m_tmp[fine], rhs_view, fine);
+        } else {
+            smooth_fused(fine, rhs);
+        }
+    }
+    out.copy_data(*m_err1[0]);
+    if (!d.bc_idempotent(0, P)) d.apply_boundary(out.data, out.get_type(), 0);
+}
+
+// VCycleMG::VCycleMultigrid :215-288, one launch per reference operator.
+// residuum1[0] is the right-hand side itself (the reference copies b there
+// and never writes it), and the copy of error0[0] into out at :254 is dead
+// (out is overwritten at :282).
+void VCycleMG::vcycle_plain(Field &out, const Field &b) {
+    Domain &d = m_domain;
+    const int L = m_s.n_level;
     m_err1[0]->copy_data(out);
     for (int level = 0; level < L; ++level) {
         const Field &rhs = level == 0 ? b : *m_res1[level];

@@ -2,8 +2,10 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
+#include "mg_plan.h"
 #include "operators.h"
 
 namespace artss {
@@ -20,10 +22,52 @@ class VCycleMG : public IPressure {
     // ||b - lap_h(out)||_2 over the level-0 inner cells (VCycleMG.cpp:186-195)
     double residual(const Field &out, const Field &b);
     void vcycle(Field &out, const Field &b);
+    // one V-cycle followed by the residual norm, one host round trip
+    double vcycle_residual(Field &out, const Field &b);
+    // one V-cycle issued launch by launch (never from a graph), for callers
+    // that capture it into graphs of their own
+    void vcycle_eager(Field &out, const Field &b);
 
     int last_cycles() const { return m_cycles; }
 
+    // Launch schedule (results are identical either way):
+    //  fused = false: one launch per reference operator, host-side coarse
+    //                 solve loop (a readback per iteration);
+    //  fused = true:  copies and boundary fills that cannot change the result
+    //                 dropped or folded into neighbouring kernels, and every
+    //                 level whose array has at most `tail_cells` cells run by
+    //                 one single-workgroup kernel (0: no such kernel).
+    void set_schedule(bool fused, long long tail_cells);
+    bool fused() const { return m_fused; }
+    int tail_first() const { return m_tail_first; }
+    // With the fused schedule and the single-workgroup kernel in use, a
+    // V-cycle has no host round trip; it is then captured once into a HIP
+    // graph (per output/right-hand-side pair) and replayed.
+    void set_graphs(bool on);
+    // graphs need the fused schedule with the single-workgroup kernel (no
+    // host round trip inside a V-cycle); while a kernel timer is armed the
+    // V-cycle runs eagerly so that every launch can be bracketed
+    bool graphable() const {
+        return m_fused && m_use_graphs && m_tail_first < m_s.n_level && m_domain.timer_tag().empty();
+    }
+
   private:
+    struct CycleGraph {
+        hipGraphExec_t exec = nullptr;
+        double *out = nullptr;
+        const double *b = nullptr;
+    };
+
+    void launch_graph(int which, Field &out, const Field &b);
+    void drop_graphs();
+    void residual_async(const Field &out, const Field &b);
+    void vcycle_plain(Field &out, const Field &b);
+    void vcycle_fused(Field &out, const Field &b);
+    void smooth_fused(int level, const double *b);
+    void smooth_fast(int level, const double *b);
+    bool fast_level(int level) const;
+    void bc(double *f, int level);
+    void build_tail();
     void smooth(Field &out, Field &tmp, const Field &b, int level);
     void solve(Field &out, Field &tmp, const Field &b, int level);
     void smooth_jacobi(Field &out, Field &tmp, const Field &b, int level);
@@ -39,6 +83,12 @@ class VCycleMG : public IPressure {
     // per-level scratch, VCycleMG.cpp:50-98 (residuum1[0] aliases b)
     std::vector<std::unique_ptr<Field>> m_res0, m_res1, m_err0, m_err1, m_tmp;
     int m_cycles = 0;
+    bool m_fused = true;
+    long long m_tail_cells = 20000;
+    int m_tail_first = 0;          // first level of the single-workgroup kernel; n_level: none
+    TailPlan *m_plan_dev = nullptr;
+    bool m_use_graphs = true;
+    CycleGraph m_graph[2];  // [0]: V-cycle, [1]: V-cycle + residual
 };
 
 }  // namespace artss

@@ -113,6 +113,16 @@ int artss_mg_pressure(artss_mg *mg, double *p, const double *rhs, double t, int 
 /* one VCycleMG::VCycleMultigrid (:215) and the residual norm of :186-195 */
 int artss_mg_vcycle(artss_mg *mg, double *p, const double *rhs);
 int artss_mg_residual(artss_mg *mg, const double *p, const double *rhs, double *residual);
+/* Launch schedule of the V-cycle (no reference counterpart: the reference has
+ * one).  mode 0: one launch per reference operator; mode 1: launches that
+ * cannot change the result folded or dropped, and the levels whose arrays
+ * hold at most tail_cells cells (default 20000; 0 = none) run in one
+ * single-workgroup kernel; mode 2 (default): mode 1, with each V-cycle
+ * captured once into a HIP graph and replayed.  Results are bit-identical in
+ * every mode. */
+int artss_mg_set_schedule(artss_mg *mg, int mode, long long tail_cells);
+/* first level run by the single-workgroup kernel (n_level: none) */
+int artss_mg_tail_level(artss_mg *mg, int *first);
 
 /* ---- solver: SolverController (solver/SolverController.cpp:26) + TimeIntegration ---- */
 int artss_solver_create(const char *xml_text, int device, artss_solver **out);
@@ -128,6 +138,15 @@ int artss_solver_field(artss_solver *s, const char *name, double **dev, size_t *
 int artss_solver_counters(artss_solver *s, int *diffusion4, int *vcycles, double *residual);
 /* 0: reference pressure schedule (VCycleMG::pressure); 1: V-cycles until the residual <= tolerance */
 int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tolerance, int max_cycles);
+/* With the residual target of artss_solver_set_pressure_mode (no reference
+ * counterpart): method 0 iterates V-cycles on their own (the reference's
+ * iteration); method 1 uses one V-cycle per iteration as the preconditioner
+ * of a GCR iteration with one kept direction, which meets the same target
+ * ||rhs - lap p||_2 in far fewer V-cycles when the V-cycle alone stalls
+ * (obstacles).  The final residual is the true one in both. */
+int artss_solver_set_pressure_method(artss_solver *s, int method);
+/* artss_mg_set_schedule for the solver's own pressure solver */
+int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cells);
 
 /* ---- live kernel timing (no reference counterpart; the reference only
  * prints "Global Time", TimeIntegration.cpp:165-183) ----
