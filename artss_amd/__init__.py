@@ -72,7 +72,7 @@ def lib():
         "artss_mg_vcycle": (C.c_int, [vp, vp, vp]),
         "artss_mg_residual": (C.c_int, [vp, vp, vp, dp]),
         "artss_mg_set_schedule": (C.c_int, [vp, C.c_int, C.c_longlong]),
-        "artss_mg_tail_level": (C.c_int, [vp, ip]),
+        "artss_mg_tail_info": (C.c_int, [vp, ip, ip]),
         "artss_solver_create": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp)]),
         "artss_solver_destroy": (C.c_int, [vp]),
         "artss_solver_domain": (C.c_int, [vp, C.POINTER(vp)]),
@@ -82,7 +82,8 @@ def lib():
         "artss_solver_counters": (C.c_int, [vp, ip, ip, dp]),
         "artss_solver_set_pressure_mode": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
         "artss_solver_set_mg_schedule": (C.c_int, [vp, C.c_int, C.c_longlong]),
-        "artss_solver_set_pressure_method": (C.c_int, [vp, C.c_int]),
+        "artss_solver_set_pressure_method": (C.c_int, [vp, C.c_int, C.c_int]),
+        "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
         "artss_profile_arm": (C.c_int, [vp, C.c_char_p]),
         "artss_profile_read": (C.c_int, [vp, C.POINTER(C.c_longlong), dp]),
     }
@@ -290,16 +291,20 @@ class Multigrid:
         _check(self.dom.L.artss_mg_residual(self.h, p.ptr, rhs.ptr, C.byref(r)))
         return r.value
 
-    def set_schedule(self, mode=2, tail_cells=20000):
+    def set_schedule(self, mode=2, tail_cells=4096):
         """mode 0: one launch per reference operator; 1: folded launches and a
         single-workgroup kernel for levels of <= tail_cells cells; 2: as 1,
         replayed from a captured HIP graph.  Results are bit-identical."""
         _check(self.dom.L.artss_mg_set_schedule(self.h, int(mode), int(tail_cells)))
 
     def tail_level(self):
-        f = C.c_int()
-        _check(self.dom.L.artss_mg_tail_level(self.h, C.byref(f)))
-        return f.value
+        return self.tail_info()[0]
+
+    def tail_info(self):
+        """(first level of the single-workgroup kernel, LDS bytes it keeps level arrays in)"""
+        f, b = C.c_int(), C.c_int()
+        _check(self.dom.L.artss_mg_tail_info(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value
 
 
 class Solver:
@@ -355,9 +360,17 @@ class Solver:
     def set_pressure_mode(self, to_tolerance, tolerance=1e-8, max_cycles=200):
         _check(self.L.artss_solver_set_pressure_mode(self.h, 1 if to_tolerance else 0, tolerance, max_cycles))
 
-    def set_pressure_method(self, method):
-        """0: V-cycles on their own; 1: V-cycle-preconditioned GCR (residual target only)."""
-        _check(self.L.artss_solver_set_pressure_method(self.h, int(method)))
+    def write_dat(self, directory):
+        """The reference's final output: <name>.dat and <name>_obstacle.dat per used field."""
+        n = C.c_int()
+        _check(self.L.artss_solver_write_dat(self.h, str(directory).encode(), C.byref(n)))
+        return n.value
 
-    def set_mg_schedule(self, mode=2, tail_cells=20000):
+    def set_pressure_method(self, method, coarse_max_iter=0):
+        """0: V-cycles on their own; 1: V-cycle-preconditioned GCR; 2: GCR with an FP32/FP64
+        mixed-precision V-cycle (residual target only); coarse_max_iter > 0 caps the
+        preconditioner's coarse solve in method 2."""
+        _check(self.L.artss_solver_set_pressure_method(self.h, int(method), int(coarse_max_iter)))
+
+    def set_mg_schedule(self, mode=2, tail_cells=4096):
         _check(self.L.artss_solver_set_mg_schedule(self.h, int(mode), int(tail_cells)))

@@ -424,10 +424,11 @@ int artss_mg_set_schedule(artss_mg *mg, int mode, long long tail_cells) {
     });
 }
 
-int artss_mg_tail_level(artss_mg *mg, int *first) {
+int artss_mg_tail_info(artss_mg *mg, int *first, int *lds_bytes) {
     return guard([&] {
         require(mg && first, "null argument");
         *first = mg->mg->tail_first();
+        if (lds_bytes) *lds_bytes = mg->mg->tail_lds_bytes();
         return ARTSS_OK;
     });
 }
@@ -528,11 +529,21 @@ int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tol
     });
 }
 
-int artss_solver_set_pressure_method(artss_solver *s, int method) {
+int artss_solver_write_dat(artss_solver *s, const char *directory, int *files) {
     return guard([&] {
-        require(s && method >= 0 && method <= 1, "bad argument");
+        require(s && directory, "null argument");
+        const int n = s->solver->write_dat(directory);
+        if (files) *files = n;
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max_iter) {
+    return guard([&] {
+        require(s && method >= 0 && method <= 2 && coarse_max_iter >= 0, "bad argument");
         PressureMode m = s->solver->pressure_mode();
         m.method = method;
+        m.coarse_max_iter = coarse_max_iter;
         s->solver->set_pressure_mode(m);
         return ARTSS_OK;
     });

@@ -14,6 +14,7 @@ namespace k {
 // ---- element-wise over the whole array (Field.h:64-135) ----
 void copy(hipStream_t s, double *dst, const double *src, long long n);
 void fill(hipStream_t s, double *dst, double v, long long n);
+void fill32(hipStream_t s, float *dst, float v, long long n);
 void scale(hipStream_t s, double *dst, double v, long long n);           // dst *= v
 void add(hipStream_t s, double *dst, const double *src, long long n);    // dst += src
 void scale_to(hipStream_t s, double *dst, const double *src, double v, long long n);  // dst = src * v
@@ -60,20 +61,27 @@ void residual_partials(hipStream_t s, const LevelDesc &L, const double *x, const
 void restrict_(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine);
 void prolongate(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse);
 // coarse = restrict(fine) on coarse inner cells and coarse_err = 0 on the whole coarse array
-void restrict_fill(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine,
-                   double *coarse_err);
+// (instantiated for double/double/double, float/float/float and float in -> double out)
+template <typename Tin, typename Tout, typename Terr>
+void restrict_fill(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, Tout *coarse, const Tin *fine,
+                   Terr *coarse_err);
 // fine += prolongate(coarse) on the cells the prolongation writes
 void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse);
 
 // ---- fused operator + boundary fill, for idempotent fills (device.h BCView) ----
+// Iterates of type T (double: the reference's arithmetic; float: the mixed
+// V-cycle), right-hand sides / coarse errors of type TB / Tc.
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src);
-void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, double *dst, const double *src,
-               const double *b, double ax, double ay, double az, double beta, double dsign, double w);
-void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, double *out, const double *x,
-                    const double *b);
-void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, double *dst,
-                       const double *fine, const double *coarse);
-void fill_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *f);
+template <typename T, typename TB>
+void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
+               double ax, double ay, double az, double beta, double dsign, double w);
+template <typename T, typename TB>
+void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, T *out, const T *x, const TB *b);
+template <typename Tf, typename Tc>
+void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, Tf *dst,
+                       const Tf *fine, const Tc *coarse);
+template <typename T>
+void fill_bc(hipStream_t s, const LevelDesc &L, const BCView &V, T *f);
 
 // ---- reductions ----
 int partial_count(const LevelDesc &L);  // number of partial slots a level-wide kernel writes

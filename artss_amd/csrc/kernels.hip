@@ -67,6 +67,9 @@ __device__ inline void write_partial(double *partial, double v) {
 __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long long n) {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i];
 }
+__global__ void k_fill32(float *__restrict__ d, float v, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = v;
+}
 __global__ void k_fill(double *__restrict__ d, double v, long long n) {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = v;
 }
@@ -345,8 +348,9 @@ __global__ void k_prolongate(LevelDesc Lc, LevelDesc Lf, double *__restrict__ ou
 
 // Restriction (VCycleMG.cpp:404-421) fused with the reset of the coarse
 // error (VCycleMG.cpp:250): one thread per coarse cell of the full array.
-__global__ void k_restrict_fill(LevelDesc Lc, LevelDesc Lf, double *__restrict__ out, const double *__restrict__ in,
-                                double *__restrict__ err) {
+template <typename Tin, typename Tout, typename Terr>
+__global__ void k_restrict_fill(LevelDesc Lc, LevelDesc Lf, Tout *__restrict__ out, const Tin *__restrict__ in,
+                                Terr *__restrict__ err) {
     const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
     const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
     const int k = static_cast<int>(blockIdx.z);
@@ -354,7 +358,7 @@ __global__ void k_restrict_fill(LevelDesc Lc, LevelDesc Lf, double *__restrict__
     const long long idx = lin(Lc, i, j, k);
     err[idx] = 0;
     if (i >= 1 && j >= 1 && k >= 1 && i <= Lc.n[0] && j <= Lc.n[1] && k <= Lc.n[2] && !in_obstacle(Lc, i, j, k))
-        out[idx] = dev::restrict_cell(Lf, in, i, j, k);
+        out[idx] = static_cast<Tout>(dev::restrict_t<Tin>(Lf, in, i, j, k));
 }
 
 // Prolongation added straight into the fine error (VCycleMG.cpp:266-270
@@ -415,13 +419,13 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict_
 // initial copy is folded in, every cell of dst is written (inner cells with
 // the update of src, filled cells by the fill, all others with src);
 // first = 0: cells that are neither inner nor filled keep dst's own values.
-__global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int first, double ax, double ay, double az,
-                            double beta, double dsign, double w, double *__restrict__ dst,
-                            const double *__restrict__ src, const double *__restrict__ b) {
+template <typename T, typename TB>
+__global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int first, T ax, T ay, T az, T beta, T dsign, T w,
+                            T *__restrict__ dst, const T *__restrict__ src, const TB *__restrict__ b) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
-        dev::bc_item(L, V, item_index(main_z), dst, [&](long long c) {
+        dev::bc_item(L, V, item_index(main_z), dst, [&](long long c) -> T {
             int ci, cj, ck;
-            if (is_inner(L, c, ci, cj, ck)) return dev::jacobi_mg(L, ax, ay, az, beta, dsign, w, src, b, c);
+            if (is_inner(L, c, ci, cj, ck)) return dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, c);
             return first ? src[c] : dst[c];
         });
         return;
@@ -441,20 +445,21 @@ __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int first, double
     const long long idx = lin(L, i, j, k);
     const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
                        !in_obstacle(L, i, j, k);
-    if (inner) dst[idx] = dev::jacobi_mg(L, ax, ay, az, beta, dsign, w, src, b, idx);
+    if (inner) dst[idx] = dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, idx);
     else if (first && !dev::written(V, idx)) dst[idx] = src[idx];
 }
 
 // Residuum (VCycleMG.cpp:346-354) and the obstacle-face part of the fill
 // after it (VCycleMG.cpp:245); the ghosts of the residuum are never read.
-__global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, double *__restrict__ out,
-                                 const double *__restrict__ x, const double *__restrict__ b) {
+template <typename T, typename TB>
+__global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, T *__restrict__ out, const T *__restrict__ x,
+                                 const TB *__restrict__ b) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
         const long long t = item_index(main_z);
         if (t >= V.n_faces) return;
-        dev::bc_item(L, V, t, out, [&](long long c) {
+        dev::bc_item(L, V, t, out, [&](long long c) -> T {
             int ci, cj, ck;
-            if (is_inner(L, c, ci, cj, ck)) return b[c] - laplace(L, x, c);
+            if (is_inner(L, c, ci, cj, ck)) return static_cast<T>(b[c]) - dev::laplace_t<T>(L, x, c);
             return out[c];
         });
         return;
@@ -462,7 +467,7 @@ __global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, double *__re
     int i, j, k;
     if (!inner_cell(L, i, j, k)) return;
     const long long idx = lin(L, i, j, k);
-    out[idx] = b[idx] - laplace(L, x, idx);
+    out[idx] = static_cast<T>(b[idx]) - dev::laplace_t<T>(L, x, idx);
 }
 
 // The correction of the ascent (VCycleMG.cpp:266-270) and the two fills at
@@ -470,8 +475,9 @@ __global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, double *__re
 // prolongate(coarse) on the fine cells, with the fill applied.  Cells the
 // prolongation does not reach get fine + 0.0, as the reference's whole-array
 // addition of a zero correction gives.
-__device__ inline double corrected(const LevelDesc &Lc, const LevelDesc &Lf, const double *__restrict__ fine,
-                                   const double *__restrict__ coarse, long long c) {
+template <typename Tf, typename Tc>
+__device__ inline Tf corrected(const LevelDesc &Lc, const LevelDesc &Lf, const Tf *__restrict__ fine,
+                               const Tc *__restrict__ coarse, long long c) {
     int i, j, k;
     {
         const long long Nx = Lf.N[0], Ny = Lf.N[1];
@@ -479,21 +485,22 @@ __device__ inline double corrected(const LevelDesc &Lc, const LevelDesc &Lf, con
         j = static_cast<int>((c / Nx) % Ny);
         i = static_cast<int>(c % Nx);
     }
-    double p = 0.0;
+    Tf p = 0.0;
     if (i >= 1 && j >= 1 && k >= 1) {
         const int ci = (i + 1) / 2, cj = (j + 1) / 2, ck = (k + 1) / 2;
         if (ci <= Lc.n[0] && cj <= Lc.n[1] && ck <= Lc.n[2] && !in_obstacle(Lc, ci, cj, ck)) {
             const int q = ((i & 1) ? 4 : 0) | ((j & 1) ? 2 : 0) | ((k & 1) ? 1 : 0);
-            p = dev::prolong_value(Lc, coarse, lin(Lc, ci, cj, ck), q);
+            p = static_cast<Tf>(dev::prolong_t<Tc>(Lc, coarse, lin(Lc, ci, cj, ck), q));
         }
     }
     return fine[c] + p;
 }
-__global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, double *__restrict__ dst,
-                                    const double *__restrict__ fine, const double *__restrict__ coarse) {
+template <typename Tf, typename Tc>
+__global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, Tf *__restrict__ dst,
+                                    const Tf *__restrict__ fine, const Tc *__restrict__ coarse) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
         dev::bc_item(Lf, V, item_index(main_z), dst,
-                     [&](long long c) { return corrected(Lc, Lf, fine, coarse, c); });
+                     [&](long long c) { return corrected<Tf, Tc>(Lc, Lf, fine, coarse, c); });
         return;
     }
     const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
@@ -501,11 +508,12 @@ __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int ma
     const int k = static_cast<int>(blockIdx.z);
     if (i >= Lf.N[0] || j >= Lf.N[1]) return;
     const long long idx = lin(Lf, i, j, k);
-    if (!dev::written(V, idx)) dst[idx] = corrected(Lc, Lf, fine, coarse, idx);
+    if (!dev::written(V, idx)) dst[idx] = corrected<Tf, Tc>(Lc, Lf, fine, coarse, idx);
 }
 
 // One idempotent fill in a single launch (faces, ghosts, dependent ghosts)
-__global__ void k_fill_bc(LevelDesc L, BCView V, double *__restrict__ f) {
+template <typename T>
+__global__ void k_fill_bc(LevelDesc L, BCView V, T *__restrict__ f) {
     dev::bc_item(L, V, blockIdx.x * 256LL + threadIdx.x, f, [&](long long c) { return f[c]; });
 }
 
@@ -605,6 +613,7 @@ void copy(hipStream_t s, double *dst, const double *src, long long n) {
     k_copy<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n);
 }
 void fill(hipStream_t s, double *dst, double v, long long n) { k_fill<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
+void fill32(hipStream_t s, float *dst, float v, long long n) { k_fill32<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
 void scale(hipStream_t s, double *dst, double v, long long n) { k_scale<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
 void add(hipStream_t s, double *dst, const double *src, long long n) {
     k_add<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n);
@@ -680,10 +689,17 @@ void prolongate(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double 
     k_prolongate<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, fine, coarse);
 }
 
-void restrict_fill(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *coarse, const double *fine,
-                   double *coarse_err) {
-    k_restrict_fill<<<full_grid(Lc), block(), 0, s>>>(Lc, Lf, coarse, fine, coarse_err);
+template <typename Tin, typename Tout, typename Terr>
+void restrict_fill(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, Tout *coarse, const Tin *fine,
+                   Terr *coarse_err) {
+    k_restrict_fill<Tin, Tout, Terr><<<full_grid(Lc), block(), 0, s>>>(Lc, Lf, coarse, fine, coarse_err);
 }
+template void restrict_fill<double, double, double>(hipStream_t, const LevelDesc &, const LevelDesc &, double *,
+                                                    const double *, double *);
+template void restrict_fill<float, float, float>(hipStream_t, const LevelDesc &, const LevelDesc &, float *,
+                                                 const float *, float *);
+template void restrict_fill<float, double, double>(hipStream_t, const LevelDesc &, const LevelDesc &, double *,
+                                                   const float *, double *);
 void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse) {
     k_prolongate_add<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, fine, coarse);
 }
@@ -692,28 +708,56 @@ void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, co
     const dim3 g = full_grid(L);
     k_copy_bc<<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), dst, src);
 }
-void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, double *dst, const double *src,
-               const double *b, double ax, double ay, double az, double beta, double dsign, double w) {
+template <typename T, typename TB>
+void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
+               double ax, double ay, double az, double beta, double dsign, double w) {
     const dim3 g = first ? full_grid(L) : inner_grid(L);
-    k_jacobi_bc<<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), first ? 1 : 0, ax,
-                                                                     ay, az, beta, dsign, w, dst, src, b);
+    k_jacobi_bc<T, TB><<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(
+        L, V, static_cast<int>(g.z), first ? 1 : 0, static_cast<T>(ax), static_cast<T>(ay), static_cast<T>(az),
+        static_cast<T>(beta), static_cast<T>(dsign), static_cast<T>(w), dst, src, b);
 }
-void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, double *out, const double *x,
-                    const double *b) {
+template void jacobi_bc<double, double>(hipStream_t, const LevelDesc &, const BCView &, bool, double *,
+                                        const double *, const double *, double, double, double, double, double,
+                                        double);
+template void jacobi_bc<float, double>(hipStream_t, const LevelDesc &, const BCView &, bool, float *, const float *,
+                                       const double *, double, double, double, double, double, double);
+template void jacobi_bc<float, float>(hipStream_t, const LevelDesc &, const BCView &, bool, float *, const float *,
+                                      const float *, double, double, double, double, double, double);
+
+template <typename T, typename TB>
+void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, T *out, const T *x, const TB *b) {
     const dim3 g = inner_grid(L);
-    k_residuum_faces<<<with_items(g, V.n_faces), block(), 0, s>>>(L, V, static_cast<int>(g.z), out, x, b);
+    k_residuum_faces<T, TB><<<with_items(g, V.n_faces), block(), 0, s>>>(L, V, static_cast<int>(g.z), out, x, b);
 }
-void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, double *dst,
-                       const double *fine, const double *coarse) {
+template void residuum_faces<double, double>(hipStream_t, const LevelDesc &, const BCView &, double *,
+                                             const double *, const double *);
+template void residuum_faces<float, double>(hipStream_t, const LevelDesc &, const BCView &, float *, const float *,
+                                            const double *);
+template void residuum_faces<float, float>(hipStream_t, const LevelDesc &, const BCView &, float *, const float *,
+                                           const float *);
+
+template <typename Tf, typename Tc>
+void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, Tf *dst,
+                       const Tf *fine, const Tc *coarse) {
     const dim3 g = full_grid(Lf);
-    k_prolongate_add_bc<<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z), dst,
-                                                                             fine, coarse);
+    k_prolongate_add_bc<Tf, Tc><<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z),
+                                                                                     dst, fine, coarse);
 }
-void fill_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *f) {
+template void prolongate_add_bc<double, double>(hipStream_t, const LevelDesc &, const LevelDesc &, const BCView &,
+                                                double *, const double *, const double *);
+template void prolongate_add_bc<float, double>(hipStream_t, const LevelDesc &, const LevelDesc &, const BCView &,
+                                               float *, const float *, const double *);
+template void prolongate_add_bc<float, float>(hipStream_t, const LevelDesc &, const LevelDesc &, const BCView &,
+                                              float *, const float *, const float *);
+
+template <typename T>
+void fill_bc(hipStream_t s, const LevelDesc &L, const BCView &V, T *f) {
     const long long n = dev::bc_items(V);
     if (n <= 0) return;
-    k_fill_bc<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, V, f);
+    k_fill_bc<T><<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, V, f);
 }
+template void fill_bc<double>(hipStream_t, const LevelDesc &, const BCView &, double *);
+template void fill_bc<float>(hipStream_t, const LevelDesc &, const BCView &, float *);
 
 int partial_count(const LevelDesc &L) {
     const dim3 g = inner_grid(L);

@@ -7,8 +7,11 @@
 
 namespace artss {
 
-GcrPressure::GcrPressure(Domain &d, VCycleMG &preconditioner)
-    : m_domain(d), m_pre(preconditioner), m_r(d, P), m_z(d, P), m_w(d, P), m_zp(d, P), m_wp(d, P) {
+GcrPressure::GcrPressure(Domain &d, VCycleMG &preconditioner, bool mixed, int coarse_max_iter)
+    : m_domain(d), m_pre(preconditioner), m_mixed(mixed), m_coarse_max_iter(coarse_max_iter), m_r(d, P), m_z(d, P), m_w(d, P), m_zp(d, P), m_wp(d, P) {
+    if (m_mixed && !m_pre.mixed_supported())
+        throw config_error("the mixed-precision pressure solve needs the weighted-Jacobi smoother, boundary fills "
+                           "without chained obstacle faces and a coarse-level kernel starting below level 0");
     const LevelDesc &L = d.level(0);
     const int blocks = std::max(kk::blocks_inner(L), kk::blocks_full(L));
     ARTSS_HIP(hipMalloc(&m_partial, sizeof(double) * 4 * static_cast<size_t>(blocks)));
@@ -33,14 +36,25 @@ void GcrPressure::iteration(Field &p) {
     Domain &d = m_domain;
     const LevelDesc &L = d.level(0);
     hipStream_t s = d.stream();
-    m_z.set_value(0.0);
-    m_pre.vcycle_eager(m_z, m_r);
-    d.apply_boundary_linear(m_z.data, P, 0);
-    d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_w.data, m_z.data, m_wp.data, m_r.data, m_partial); });
-    kk::gcr_coef(s, m_partial, kk::blocks_inner(L), m_coef);
-    d.timed("gcr_update", 0, [&] {
-        kk::gcr_update(s, L, m_coef, p.data, m_r.data, m_z.data, m_zp.data, m_w.data, m_wp.data, m_partial);
-    });
+    if (m_mixed) {
+        // the FP32 result already carries the linear part of the fill
+        m_pre.vcycle_mixed(m_r.data, m_coarse_max_iter);
+        const float *z = m_pre.mixed_result();
+        d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_w.data, z, m_wp.data, m_r.data, m_partial); });
+        kk::gcr_coef(s, m_partial, kk::blocks_inner(L), m_coef);
+        d.timed("gcr_update", 0, [&] {
+            kk::gcr_update(s, L, m_coef, p.data, m_r.data, z, m_zp.data, m_w.data, m_wp.data, m_partial);
+        });
+    } else {
+        m_z.set_value(0.0);
+        m_pre.vcycle_eager(m_z, m_r);
+        d.apply_boundary_linear(m_z.data, P, 0);
+        d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_w.data, m_z.data, m_wp.data, m_r.data, m_partial); });
+        kk::gcr_coef(s, m_partial, kk::blocks_inner(L), m_coef);
+        d.timed("gcr_update", 0, [&] {
+            kk::gcr_update(s, L, m_coef, p.data, m_r.data, m_z.data, m_zp.data, m_w.data, m_wp.data, m_partial);
+        });
+    }
     kk::sum(s, m_partial, kk::blocks_full(L), m_sum);
     ARTSS_HIP(hipMemcpyAsync(m_sum_host, m_sum, sizeof(double), hipMemcpyDeviceToHost, s));
 }
@@ -85,6 +99,7 @@ double GcrPressure::true_residual(Field &p, const Field &b) {
 
 int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, double *residual) {
     Domain &d = m_domain;
+    if (m_mixed) m_pre.prepare_mixed();
     const LevelDesc &L = d.level(0);
     hipStream_t s = d.stream();
     auto restart = [&]() {

@@ -19,7 +19,12 @@ namespace artss {
 
 class GcrPressure {
   public:
-    GcrPressure(Domain &d, VCycleMG &preconditioner);
+    // mixed: the preconditioner's V-cycle runs its upper levels in FP32
+    // (VCycleMG::vcycle_mixed); the iteration, residuals and norms stay FP64
+    // coarse_max_iter (mixed only, > 0): cap of the preconditioner's coarse
+    // Jacobi solve; the preconditioner need not solve its coarse problem to
+    // the reference's tolerance because GCR corrects what it misses
+    GcrPressure(Domain &d, VCycleMG &preconditioner, bool mixed = false, int coarse_max_iter = 0);
     ~GcrPressure();
     GcrPressure(const GcrPressure &) = delete;
     GcrPressure &operator=(const GcrPressure &) = delete;
@@ -35,6 +40,8 @@ class GcrPressure {
 
     Domain &m_domain;
     VCycleMG &m_pre;
+    bool m_mixed;
+    int m_coarse_max_iter;
     Field m_r, m_z, m_w, m_zp, m_wp;
     double *m_partial = nullptr;       // 4 sums x blocks
     double *m_coef = nullptr;          // device: beta, 1/|w'|, alpha

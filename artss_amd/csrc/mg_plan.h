@@ -26,6 +26,10 @@ struct TailLevel {
     long long wave_start[kMaxTailWaves + 1];
     double ax, ay, az, beta;          // smoother coefficients
     double *err1, *tmp, *res0, *res1, *err0;
+    // Offsets (in doubles) into the kernel's dynamic LDS when the level's
+    // arrays live there, -1 when they stay in HBM.  Order: err1, tmp, res0,
+    // res1, err0.  At the first level res1 holds a copy of the right-hand side.
+    int lds_off[5];
 };
 
 struct TailPlan {
@@ -38,6 +42,8 @@ struct TailPlan {
     int max_iter;     // coarse solve: iteration cap
     double tol;       // coarse solve: residual target
     int solve_smooths; // the coarse solve degenerates to a smoothing (Nx <= 4 and Ny <= 4)
+    int lds_doubles;   // dynamic LDS size (doubles) for level arrays; 0: all in HBM
+    double *io_err1;   // HBM error of the first level, written back when it lives in LDS
     TailLevel lv[kMaxTailLevels];  // index: level - first
 };
 

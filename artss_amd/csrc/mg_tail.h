@@ -10,6 +10,11 @@ namespace artss {
 // Runs VCycleMG's descent and ascent over levels plan->first .. plan->last
 // (device-resident plan) with `b_first` as the right-hand side of the first
 // level; one workgroup, enqueued on `s`.
-void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first);
+// max_iter > 0 replaces the coarse solve's iteration cap of the plan
+void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter = 0);
+// dynamic LDS (bytes) a workgroup of the kernel can use for level arrays
+int mg_tail_lds_limit();
+// allow launches with this much dynamic LDS (once, before the first launch)
+void mg_tail_reserve_lds(int lds_bytes);
 
 }  // namespace artss

@@ -310,17 +310,42 @@ __device__ void t_solve(const TailLevel &T, const TailPlan &P, double *out, doub
     else t_solve_jacobi(T, P, out, tmp, b, red);
 }
 
-__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__restrict__ plan, const double *b_first) {
+__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__restrict__ plan, const double *b_first,
+                                                           int max_iter) {
     __shared__ TailPlan P;
     __shared__ double red[kTailThreads / 64 + 1];
+    extern __shared__ double lds[];
     {
         const unsigned long long *src = reinterpret_cast<const unsigned long long *>(plan);
         unsigned long long *dst = reinterpret_cast<unsigned long long *>(&P);
         constexpr int words = sizeof(TailPlan) / sizeof(unsigned long long);
         for (int t = threadIdx.x; t < words; t += kTailThreads) dst[t] = src[t];
         __syncthreads();
+        if (max_iter > 0 && threadIdx.x == 0) P.max_iter = max_iter;
+        __syncthreads();
     }
     const int f = P.first, Lm = P.last;
+    // Level arrays placed in LDS start as the HBM arrays they stand for: zero
+    // wherever no operator writes them (every array is zero-initialised and
+    // those cells are never written), the first level's error and
+    // right-hand side copied in.  Only the first level's error leaves.
+    if (P.lds_doubles > 0) {
+        for (int t = threadIdx.x; t < P.lds_doubles; t += kTailThreads) lds[t] = 0.0;
+        if (threadIdx.x == 0) {
+            for (int l = 0; l <= Lm - f; l++) {
+                TailLevel &T = P.lv[l];
+                double **ptr[5] = {&T.err1, &T.tmp, &T.res0, &T.res1, &T.err0};
+                for (int a = 0; a < 5; a++)
+                    if (T.lds_off[a] >= 0) *ptr[a] = lds + T.lds_off[a];
+            }
+        }
+        __syncthreads();
+        if (P.lv[0].lds_off[0] >= 0) {
+            t_copy(P.lv[0].err1, P.io_err1, static_cast<int>(P.lv[0].L.size));
+            t_copy(P.lv[0].res1, b_first, static_cast<int>(P.lv[0].L.size));
+            b_first = P.lv[0].res1;
+        }
+    }
     // descent, VCycleMG.cpp:228-252
     for (int level = f; level < Lm; ++level) {
         const TailLevel &T = P.lv[level - f];
@@ -344,12 +369,27 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__rest
         if (level == Lm) t_solve(T, P, T.err1, T.tmp, rhs, red);
         else t_smooth(T, P, T.err1, T.tmp, rhs);
     }
+    if (P.lv[0].lds_off[0] >= 0) t_copy(P.io_err1, P.lv[0].err1, static_cast<int>(P.lv[0].L.size));
 }
 
 }  // namespace
 
-void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first) {
-    k_mg_tail<<<1, kTailThreads, 0, s>>>(plan, b_first);
+int mg_tail_lds_limit() {
+    int dev = 0, bytes = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    const int static_bytes = static_cast<int>(sizeof(TailPlan)) + 1024;
+    return bytes > static_bytes ? bytes - static_bytes : 0;
+}
+
+void mg_tail_reserve_lds(int lds_bytes) {
+    if (lds_bytes > 0)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mg_tail),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+}
+
+void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter) {
+    k_mg_tail<<<1, kTailThreads, lds_bytes, s>>>(plan, b_first, max_iter);
 }
 
 }  // namespace artss

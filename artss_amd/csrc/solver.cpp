@@ -1,5 +1,7 @@
 #include "solver.h"
 
+#include <fstream>
+
 #include <cmath>
 
 namespace artss {
@@ -210,8 +212,13 @@ void Solver::pressure_step(double t) {
         m_counters.vcycles = m_pres->last_cycles();
         return;
     }
-    if (m_pmode.method == 1) {
-        if (!m_gcr) m_gcr = std::make_unique<GcrPressure>(*m_domain, *m_pres);
+    if (m_pmode.method == 1 || m_pmode.method == 2) {
+        const bool mixed = m_pmode.method == 2;
+        if (!m_gcr || m_gcr_mixed != mixed || m_gcr_coarse != m_pmode.coarse_max_iter) {
+            m_gcr = std::make_unique<GcrPressure>(*m_domain, *m_pres, mixed, m_pmode.coarse_max_iter);
+            m_gcr_mixed = mixed;
+            m_gcr_coarse = m_pmode.coarse_max_iter;
+        }
         double r = 0.0;
         m_counters.vcycles = m_gcr->solve(f.p, f.rhs, m_pmode.tolerance, m_pmode.max_cycles, &r);
         m_counters.residual = r;
@@ -303,6 +310,33 @@ void Solver::step(double t) {
     else pressure_step(t);
     update_sources(t);
     m_fields->update_data();
+}
+
+int Solver::write_dat(const std::string &dir) {
+    Domain &d = *m_domain;
+    static const char *names[NUMBER_OF_FIELD_TYPES] = {"rho", "u", "v", "w", "p", "T", "nu"};
+    const std::vector<uint64_t> obstacle_cells = d.obstacle_list(0);
+    int files = 0;
+    std::vector<double> host(static_cast<size_t>(d.level(0).size));
+    for (int ft = 0; ft < NUMBER_OF_FIELD_TYPES; ft++) {
+        if (!d.domain_bc(0, ft).active) continue;  // BoundaryDataController::get_used_fields
+        Field *f = m_fields->by_name(names[ft]);
+        if (f == nullptr) continue;  // no such field on this path (rho, nu)
+        ARTSS_HIP(hipMemcpy(host.data(), f->data, sizeof(double) * host.size(), hipMemcpyDeviceToHost));
+        {
+            std::ofstream out(dir + "/" + names[ft] + ".dat");
+            if (!out) throw std::invalid_argument("cannot write " + dir + "/" + names[ft] + ".dat");
+            for (double x : host) out << x << std::endl;
+        }
+        files++;
+        if (!obstacle_cells.empty()) {
+            std::ofstream out(dir + "/" + names[ft] + "_obstacle.dat");
+            if (!out) throw std::invalid_argument("cannot write " + dir + "/" + names[ft] + "_obstacle.dat");
+            for (uint64_t c : obstacle_cells) out << c << ";" << host[c] << std::endl;
+            files++;
+        }
+    }
+    return files;
 }
 
 }  // namespace artss

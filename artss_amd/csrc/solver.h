@@ -46,8 +46,10 @@ struct PressureMode {
     double tolerance = 1e-8;
     int max_cycles = 200;
     // with to_tolerance: 0 V-cycles on their own (the reference's
-    // iteration), 1 V-cycles as the preconditioner of GCR (krylov.h)
+    // iteration), 1 V-cycles as the preconditioner of GCR (krylov.h),
+    // 2 as 1 with the V-cycle's upper levels in FP32
     int method = 0;
+    int coarse_max_iter = 0;  // method 2: cap of the preconditioner's coarse solve (0: the XML's)
 };
 
 class Solver {
@@ -65,6 +67,13 @@ class Solver {
     const PressureMode &pressure_mode() const { return m_pmode; }
     VCycleMG &pressure_solver() { return *m_pres; }
 
+    // Analysis::save_variables_in_file (analysis/Analysis.cpp:360-408): for
+    // every field type with domain boundary data, `<dir>/<name>.dat` with one
+    // value per cell in the default ostream format, and, with obstacles,
+    // `<dir>/<name>_obstacle.dat` with "index;value" per obstacle cell.
+    // Returns the number of files written.
+    int write_dat(const std::string &dir);
+
   private:
     void do_step_ns(double t);
     void do_step_nstempturb(double t);
@@ -80,6 +89,8 @@ class Solver {
     std::unique_ptr<ITurbulence> m_turb;
     std::unique_ptr<VCycleMG> m_pres;
     std::unique_ptr<GcrPressure> m_gcr;
+    bool m_gcr_mixed = false;
+    int m_gcr_coarse = 0;
     std::unique_ptr<ExplicitEulerSource> m_src_vel, m_src_temp;
     std::unique_ptr<GaussFunction> m_gauss;
     int m_kind = 0;  // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver

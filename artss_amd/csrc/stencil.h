@@ -18,22 +18,34 @@ __device__ inline long long lin(const LevelDesc &L, long long i, long long j, lo
 }
 
 // VCycleMG.cpp:346-354 (Residuum) and :186-193 (residual norm): discrete
-// Laplacian with 1/h^2 formed in place, as the reference does.
-__device__ inline double laplace(const LevelDesc &L, const double *__restrict__ x, long long i) {
+// Laplacian with 1/h^2 formed in place, as the reference does (in T).
+template <typename T>
+__device__ inline T laplace_t(const LevelDesc &L, const T *__restrict__ x, long long i) {
     const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
-    const double rdx2 = 1. / (L.h[0] * L.h[0]), rdy2 = 1. / (L.h[1] * L.h[1]), rdz2 = 1. / (L.h[2] * L.h[2]);
+    const T rdx2 = static_cast<T>(1. / (L.h[0] * L.h[0])), rdy2 = static_cast<T>(1. / (L.h[1] * L.h[1])),
+            rdz2 = static_cast<T>(1. / (L.h[2] * L.h[2]));
     return (rdx2 * (x[i - 1] - 2 * x[i] + x[i + 1]) + rdy2 * (x[i - nj] - 2 * x[i] + x[i + nj]) +
             rdz2 * (x[i - nk] - 2 * x[i] + x[i + nk]));
 }
+__device__ inline double laplace(const LevelDesc &L, const double *__restrict__ x, long long i) {
+    return laplace_t<double>(L, x, i);
+}
 
-// JacobiDiffuse.cpp:292-300: one weighted Jacobi update of cell idx
+// JacobiDiffuse.cpp:292-300: one weighted Jacobi update of cell idx, in the
+// arithmetic of the iterate's type T (double: the reference's bits; float:
+// the mixed-precision V-cycle); the right-hand side may be stored wider.
+template <typename T, typename TB>
+__device__ inline T jacobi_t(const LevelDesc &L, T ax, T ay, T az, T beta, T dsign, T w, const T *__restrict__ in,
+                             const TB *__restrict__ b, long long idx) {
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const T out_h = beta * (dsign * static_cast<T>(b[idx]) + ax * (in[idx + 1] + in[idx - 1]) +
+                            ay * (in[idx + nj] + in[idx - nj]) + az * (in[idx + nk] + in[idx - nk]));
+    return (1 - w) * in[idx] + w * out_h;
+}
 __device__ inline double jacobi_mg(const LevelDesc &L, double ax, double ay, double az, double beta, double dsign,
                                    double w, const double *__restrict__ in, const double *__restrict__ b,
                                    long long idx) {
-    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
-    const double out_h = beta * (dsign * b[idx] + ax * (in[idx + 1] + in[idx - 1]) +
-                                 ay * (in[idx + nj] + in[idx - nj]) + az * (in[idx + nk] + in[idx - nk]));
-    return (1 - w) * in[idx] + w * out_h;
+    return jacobi_t<double, double>(L, ax, ay, az, beta, dsign, w, in, b, idx);
 }
 
 // ColoredGaussSeidelDiffuse.cpp:316-341, constant coefficients, in place
@@ -49,26 +61,35 @@ __device__ inline double cgs_mg(const LevelDesc &L, double ax, double ay, double
 }
 
 // VCycleMG.cpp:404-421: mean of the eight fine cells of coarse cell (i,j,k)
+template <typename T>
+__device__ inline T restrict_t(const LevelDesc &Lf, const T *__restrict__ in, long long i, long long j, long long k) {
+    const long long i2 = 2 * i, j2 = 2 * j, k2 = 2 * k;
+    return static_cast<T>(0.125) *
+           (in[lin(Lf, i2 - 1, j2 - 1, k2 - 1)] + in[lin(Lf, i2, j2 - 1, k2 - 1)] + in[lin(Lf, i2 - 1, j2, k2 - 1)] +
+            in[lin(Lf, i2, j2, k2 - 1)] + in[lin(Lf, i2 - 1, j2 - 1, k2)] + in[lin(Lf, i2, j2 - 1, k2)] +
+            in[lin(Lf, i2 - 1, j2, k2)] + in[lin(Lf, i2, j2, k2)]);
+}
 __device__ inline double restrict_cell(const LevelDesc &Lf, const double *__restrict__ in, long long i, long long j,
                                        long long k) {
-    const long long i2 = 2 * i, j2 = 2 * j, k2 = 2 * k;
-    return 0.125 * (in[lin(Lf, i2 - 1, j2 - 1, k2 - 1)] + in[lin(Lf, i2, j2 - 1, k2 - 1)] +
-                    in[lin(Lf, i2 - 1, j2, k2 - 1)] + in[lin(Lf, i2, j2, k2 - 1)] +
-                    in[lin(Lf, i2 - 1, j2 - 1, k2)] + in[lin(Lf, i2, j2 - 1, k2)] +
-                    in[lin(Lf, i2 - 1, j2, k2)] + in[lin(Lf, i2, j2, k2)]);
+    return restrict_t<double>(Lf, in, i, j, k);
 }
 
 // VCycleMG.cpp:466-546: value of fine cell number q (0..7) of coarse cell idx,
 // weights (27, 9, 9, 9, 3, 3, 3, 1) / 64 toward the neighbours on that
 // fine cell's side.  q's bits select the -1 side along x (4), y (2), z (1);
 // q = 0..7 is the reference's order of the eight writes.
-__device__ inline double prolong_value(const LevelDesc &Lc, const double *__restrict__ in, long long idx, int q) {
+template <typename T>
+__device__ inline T prolong_t(const LevelDesc &Lc, const T *__restrict__ in, long long idx, int q) {
     const long long ni = 1, nj = Lc.N[0], nk = static_cast<long long>(Lc.N[0]) * Lc.N[1];
     const int di = (q & 4) ? -1 : 1, dj = (q & 2) ? -1 : 1, dk = (q & 1) ? -1 : 1;
     const long long oi = di * ni, oj = dj * nj, ok = dk * nk;
-    const double c0 = in[idx];
-    return 0.015625 * (27 * c0 + 9 * in[idx + oi] + 9 * in[idx + oj] + 9 * in[idx + ok] + 3 * in[idx + oi + oj] +
-                       3 * in[idx + oi + ok] + 3 * in[idx + oj + ok] + in[idx + oi + oj + ok]);
+    const T c0 = in[idx];
+    return static_cast<T>(0.015625) * (27 * c0 + 9 * in[idx + oi] + 9 * in[idx + oj] + 9 * in[idx + ok] +
+                                       3 * in[idx + oi + oj] + 3 * in[idx + oi + ok] + 3 * in[idx + oj + ok] +
+                                       in[idx + oi + oj + ok]);
+}
+__device__ inline double prolong_value(const LevelDesc &Lc, const double *__restrict__ in, long long idx, int q) {
+    return prolong_t<double>(Lc, in, idx, q);
 }
 __device__ inline long long prolong_target(const LevelDesc &Lf, long long i, long long j, long long k, int q) {
     return lin(Lf, 2 * i - (q & 4 ? 1 : 0), 2 * j - (q & 2 ? 1 : 0), 2 * k - (q & 1 ? 1 : 0));
@@ -123,15 +144,15 @@ __device__ inline long long ghost_source(const LevelDesc &L, const DomainBC &bc,
     const long long kk = (k == 0 || k == Nz - 1) ? bc.target[k == 0 ? 4 : 5] : k;
     return ii + Nx * (jj + Ny * kk);
 }
-__device__ inline double ghost_chain(const LevelDesc &L, const DomainBC &bc, long long i, long long j, long long k,
-                                     double v) {
+template <typename T = double>
+__device__ inline T ghost_chain(const LevelDesc &L, const DomainBC &bc, long long i, long long j, long long k, T v) {
     const long long Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
     const int px = i == 0 ? 0 : (i == Nx - 1 ? 1 : -1);
     const int py = j == 0 ? 2 : (j == Ny - 1 ? 3 : -1);
     const int pz = k == 0 ? 4 : (k == Nz - 1 ? 5 : -1);
-    if (px >= 0) v = bc.sign[px] * v + bc.value[px];
-    if (py >= 0) v = bc.sign[py] * v + bc.value[py];
-    if (pz >= 0) v = bc.sign[pz] * v + bc.value[pz];
+    if (px >= 0) v = static_cast<T>(bc.sign[px]) * v + static_cast<T>(bc.value[px]);
+    if (py >= 0) v = static_cast<T>(bc.sign[py]) * v + static_cast<T>(bc.value[py]);
+    if (pz >= 0) v = static_cast<T>(bc.sign[pz]) * v + static_cast<T>(bc.value[pz]);
     return v;
 }
 
@@ -143,12 +164,13 @@ __device__ inline bool written(const BCView &V, long long c) { return (V.wbits[c
 // the value cell c has once the producing operator is done, for any cell c
 // the fill does not write; every item reads only such cells, so all items
 // are independent and the result equals the reference's sequential fill.
-template <typename Val>
-__device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t, double *dst, const Val &val) {
+template <typename T, typename Val>
+__device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t, T *dst, const Val &val) {
     if (t < V.n_faces) {
         const FaceEntry e = V.faces[t];
         const FaceParam q = V.params[e.param];
-        if (q.active) dst[e.idx] = q.sign * val(static_cast<long long>(e.idx) + e.ref) + q.value;
+        if (q.active)
+            dst[e.idx] = static_cast<T>(q.sign) * val(static_cast<long long>(e.idx) + e.ref) + static_cast<T>(q.value);
         return;
     }
     t -= V.n_faces;
@@ -158,7 +180,7 @@ __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t,
         const long long s = ghost_source(L, V.bc, i, j, k);
         if (written(V, s)) return;  // produced by a dependent item
         dst[i + static_cast<long long>(L.N[0]) * (j + static_cast<long long>(L.N[1]) * k)] =
-            ghost_chain(L, V.bc, i, j, k, val(s));
+            ghost_chain<T>(L, V.bc, i, j, k, val(s));
         return;
     }
     t -= V.n_ghosts;
@@ -166,12 +188,12 @@ __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t,
         const DepGhost d = V.deps[t];
         const FaceEntry e = V.faces[d.face];
         const FaceParam q = V.params[e.param];
-        const double fv = q.sign * val(static_cast<long long>(e.idx) + e.ref) + q.value;
+        const T fv = static_cast<T>(q.sign) * val(static_cast<long long>(e.idx) + e.ref) + static_cast<T>(q.value);
         const long long g = d.idx, Nx = L.N[0], Ny = L.N[1];
         k = g / (Nx * Ny);
         j = (g / Nx) % Ny;
         i = g % Nx;
-        dst[g] = ghost_chain(L, V.bc, i, j, k, fv);
+        dst[g] = ghost_chain<T>(L, V.bc, i, j, k, fv);
     }
 }
 __device__ __host__ inline long long bc_items(const BCView &V) { return V.n_faces + V.n_ghosts + V.n_deps; }

@@ -32,11 +32,108 @@ VCycleMG::VCycleMG(Domain &d, const MultigridSettings &s) : IPressure(d), m_s(s)
 
 VCycleMG::~VCycleMG() {
     drop_graphs();
+    free_mixed();
     if (m_plan_dev) (void)hipFree(m_plan_dev);
+}
+
+bool VCycleMG::mixed_supported() const {
+    if (!m_fused || m_tail_first < 1 || m_tail_first >= m_s.n_level) return false;
+    for (int l = 0; l < m_tail_first; l++)
+        if (!fast_level(l)) return false;
+    return true;
+}
+
+void VCycleMG::free_mixed() {
+    for (auto *v : {&m_err1f, &m_tmpf, &m_res0f, &m_res1f}) {
+        for (float *p : *v)
+            if (p) (void)hipFree(p);
+        v->clear();
+    }
+}
+
+void VCycleMG::alloc_mixed() {
+    if (!m_err1f.empty()) return;
+    const int f = m_tail_first;
+    auto alloc = [&](int level) {
+        float *p = nullptr;
+        const size_t bytes = sizeof(float) * static_cast<size_t>(m_domain.level(level).size);
+        ARTSS_HIP(hipMalloc(&p, bytes));
+        ARTSS_HIP(hipMemset(p, 0, bytes));
+        return p;
+    };
+    for (int l = 0; l < f; l++) {
+        m_err1f.push_back(alloc(l));
+        m_tmpf.push_back(alloc(l));
+        m_res0f.push_back(alloc(l));
+        m_res1f.push_back(l > 0 ? alloc(l) : nullptr);
+    }
+}
+
+// smooth_fast in FP32: tmpf -> err1f -> ... -> err1f, linear part of the fill
+void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32) {
+    Domain &d = m_domain;
+    double ax, ay, az, beta;
+    coeffs(level, &ax, &ay, &az, &beta);
+    const LevelDesc &Ld = d.level(level);
+    const BCView V = d.bc_view(level, P, true);
+    const int n = m_s.n_relax;
+    const int effective = (n % 2 == 0 && n > 0) ? n - 1 : n;
+    float *src = m_tmpf[level], *dst = m_err1f[level];
+    for (int i = 0; i < effective; i++) {
+        d.timed("jacobi_mg32", level, [&] {
+            if (b64) k::jacobi_bc<float, double>(d.stream(), Ld, V, i == 0, dst, src, b64, ax, ay, az, beta, kDsign,
+                                                 m_s.smoother.w);
+            else k::jacobi_bc<float, float>(d.stream(), Ld, V, i == 0, dst, src, b32, ax, ay, az, beta, kDsign,
+                                            m_s.smoother.w);
+        });
+        std::swap(src, dst);
+    }
+}
+
+void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
+    if (!mixed_supported()) throw std::logic_error("mixed-precision V-cycle needs the fused fast path and the tail");
+    alloc_mixed();
+    Domain &d = m_domain;
+    const int f = m_tail_first;
+    hipStream_t s = d.stream();
+    k::fill32(s, m_tmpf[0], 0.0f, d.level(0).size);
+    for (int level = 0; level < f; ++level) {
+        const BCView V = d.bc_view(level, P, true);
+        const double *b64 = level == 0 ? r : nullptr;
+        const float *b32 = level == 0 ? nullptr : m_res1f[level];
+        smooth_fast_f(level, b64, b32);
+        d.timed("residuum32", level, [&] {
+            if (b64) k::residuum_faces<float, double>(s, d.level(level), V, m_res0f[level], m_err1f[level], b64);
+            else k::residuum_faces<float, float>(s, d.level(level), V, m_res0f[level], m_err1f[level], b32);
+        });
+        d.timed("restrict32", level, [&] {
+            if (level + 1 < f)
+                k::restrict_fill<float, float, float>(s, d.level(level + 1), d.level(level), m_res1f[level + 1],
+                                                      m_res0f[level], m_tmpf[level + 1]);
+            else
+                k::restrict_fill<float, double, double>(s, d.level(level + 1), d.level(level), m_res1[level + 1]->data,
+                                                        m_res0f[level], m_err1[level + 1]->data);
+        });
+    }
+    d.timed("mg_tail", f, [&] { mg_tail(s, m_plan_dev, m_res1[f]->data, m_tail_lds_bytes, coarse_max_iter); });
+    for (int level = f; level > 0; --level) {
+        const int fine = level - 1;
+        const BCView V = d.bc_view(fine, P, true);
+        d.timed("prolongate32", fine, [&] {
+            if (level == f)
+                k::prolongate_add_bc<float, double>(s, d.level(level), d.level(fine), V, m_tmpf[fine], m_err1f[fine],
+                                                    m_err1[level]->data);
+            else
+                k::prolongate_add_bc<float, float>(s, d.level(level), d.level(fine), V, m_tmpf[fine], m_err1f[fine],
+                                                   m_err1f[level]);
+        });
+        smooth_fast_f(fine, fine == 0 ? r : nullptr, fine == 0 ? nullptr : m_res1f[fine]);
+    }
 }
 
 void VCycleMG::set_schedule(bool fused, long long tail_cells) {
     drop_graphs();
+    free_mixed();
     m_fused = fused;
     m_tail_cells = tail_cells < 0 ? 0 : tail_cells;
     build_tail();
@@ -139,7 +236,29 @@ void VCycleMG::build_tail() {
         T.res1 = l > 0 ? m_res1[l]->data : nullptr;
         T.res0 = l < L ? m_res0[l]->data : nullptr;
         T.err0 = l < L ? m_err0[l]->data : nullptr;
+        for (int a = 0; a < 5; a++) T.lds_off[a] = -1;
     }
+    // LDS placement, coarsest level first, whole levels, while they fit
+    plan.io_err1 = m_err1[f]->data;
+    const long long budget = mg_tail_lds_limit() / static_cast<long long>(sizeof(double));
+    long long used = 0;
+    for (int l = L; l >= f; l--) {
+        TailLevel &T = plan.lv[l - f];
+        const bool has[5] = {true, true, l < L, true, l < L};
+        long long need = 0;
+        for (int a = 0; a < 5; a++)
+            if (has[a]) need += d.level(l).size;
+        if (used + need > budget) break;
+        for (int a = 0; a < 5; a++)
+            if (has[a]) {
+                T.lds_off[a] = static_cast<int>(used);
+                used += d.level(l).size;
+            }
+    }
+    // a level in LDS needs every coarser one there too (the loop stops at the first misfit)
+    plan.lds_doubles = static_cast<int>(used);
+    m_tail_lds_bytes = static_cast<int>(used * sizeof(double));
+    mg_tail_reserve_lds(m_tail_lds_bytes);
     if (!m_plan_dev) ARTSS_HIP(hipMalloc(&m_plan_dev, sizeof(TailPlan)));
     ARTSS_HIP(hipMemcpy(m_plan_dev, &plan, sizeof(TailPlan), hipMemcpyHostToDevice));
     m_tail_first = f;
@@ -439,7 +558,9 @@ void VCycleMG::vcycle_fused(Field &out, const Field &b) {
         });
     }
     if (f < L) {
-        d.timed("mg_tail", f, [&] { mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data); });
+        d.timed("mg_tail", f, [&] {
+            mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes);
+        });
     }
     for (int level = f; level > 0; --level) {
         const int fine = level - 1;

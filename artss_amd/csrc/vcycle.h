@@ -40,10 +40,22 @@ class VCycleMG : public IPressure {
     void set_schedule(bool fused, long long tail_cells);
     bool fused() const { return m_fused; }
     int tail_first() const { return m_tail_first; }
+    int tail_lds_bytes() const { return m_tail_lds_bytes; }
     // With the fused schedule and the single-workgroup kernel in use, a
     // V-cycle has no host round trip; it is then captured once into a HIP
     // graph (per output/right-hand-side pair) and replayed.
     void set_graphs(bool on);
+    // Mixed precision (used as the preconditioner of GcrPressure): one
+    // V-cycle for A z = r from z = 0 with the linear part of the fill, the
+    // levels above the single-workgroup kernel in FP32 (smoothing, residuum,
+    // restriction, prolongation), the coarse levels and their solve in FP64.
+    // The result stays in mixed_result() (FP32, level 0) until the next call.
+    bool mixed_supported() const;
+    void prepare_mixed() { alloc_mixed(); }  // allocates; call before capturing vcycle_mixed
+    // coarse_max_iter > 0 caps the coarse solve's Jacobi iterations
+    void vcycle_mixed(const double *r, int coarse_max_iter = 0);
+    float *mixed_result() const { return m_err1f.empty() ? nullptr : m_err1f[0]; }
+
     // graphs need the fused schedule with the single-workgroup kernel (no
     // host round trip inside a V-cycle); while a kernel timer is armed the
     // V-cycle runs eagerly so that every launch can be bracketed
@@ -84,9 +96,15 @@ class VCycleMG : public IPressure {
     std::vector<std::unique_ptr<Field>> m_res0, m_res1, m_err0, m_err1, m_tmp;
     int m_cycles = 0;
     bool m_fused = true;
-    long long m_tail_cells = 20000;
+    long long m_tail_cells = 4096;
     int m_tail_first = 0;          // first level of the single-workgroup kernel; n_level: none
     TailPlan *m_plan_dev = nullptr;
+    int m_tail_lds_bytes = 0;
+    // FP32 level arrays of the mixed V-cycle (levels above the tail)
+    std::vector<float *> m_err1f, m_tmpf, m_res0f, m_res1f;
+    void alloc_mixed();
+    void free_mixed();
+    void smooth_fast_f(int level, const double *b64, const float *b32);
     bool m_use_graphs = true;
     CycleGraph m_graph[2];  // [0]: V-cycle, [1]: V-cycle + residual
 };

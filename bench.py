@@ -63,9 +63,11 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the reference CPU sample (0: skip)")
     ap.add_argument("--mg-schedule", choices=("graph", "fused", "plain"), default="graph",
                     help="V-cycle launch schedule (bit-identical results)")
-    ap.add_argument("--pressure", choices=("vcycle", "gcr"), default="vcycle",
+    ap.add_argument("--pressure", choices=("vcycle", "gcr", "mixed"), default="vcycle",
                     help="residual-target iteration: V-cycles alone (reference) or V-cycle-preconditioned GCR")
-    ap.add_argument("--tail-cells", type=int, default=20000,
+    ap.add_argument("--coarse-iterations", type=int, default=0,
+                    help="mixed: cap of the preconditioner's coarse Jacobi solve (0: the XML's max_iter)")
+    ap.add_argument("--tail-cells", type=int, default=4096,
                     help="levels of at most this many cells run in one single-workgroup kernel")
     ap.add_argument("--device", type=int, default=0)
     return ap.parse_args()
@@ -106,7 +108,7 @@ def main():
     solver.set_mg_schedule({"plain": 0, "fused": 1, "graph": 2}[args.mg_schedule], args.tail_cells)
     if not args.reference_schedule:
         solver.set_pressure_mode(True, args.tolerance, 500)
-        solver.set_pressure_method({"vcycle": 0, "gcr": 1}[args.pressure])
+        solver.set_pressure_method({"vcycle": 0, "gcr": 1, "mixed": 2}[args.pressure], args.coarse_iterations)
     with open(args.config) as fh:
         dt = float(fh.read().split("<dt>")[1].split("</dt>")[0])
 
@@ -163,9 +165,12 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: plume from rest, timed steps %d..%d" % (first, first + args.steps - 1),
         "config": {"workload": "plume_256x128x128", "grid": list(info["n"]), "inner_cells": inner,
-                   "path": "FP64, " + ("V-cycles alone (reference iteration)" if args.pressure == "vcycle"
-                                       else "V-cycle-preconditioned GCR"),
+                   "path": {"vcycle": "FP64, V-cycles alone (reference iteration)",
+                            "gcr": "FP64, V-cycle-preconditioned GCR",
+                            "mixed": "GCR in FP64 preconditioned by an FP32 V-cycle with FP64 coarse levels"}[
+                       args.pressure],
                    "mg_schedule": args.mg_schedule, "tail_cells": args.tail_cells,
+                   "coarse_iterations": args.coarse_iterations,
                    "pressure": ("reference V-cycle schedule" if args.reference_schedule else
                                 "V-cycles until residual <= %g" % args.tolerance),
                    "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(

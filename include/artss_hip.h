@@ -116,13 +116,14 @@ int artss_mg_residual(artss_mg *mg, const double *p, const double *rhs, double *
 /* Launch schedule of the V-cycle (no reference counterpart: the reference has
  * one).  mode 0: one launch per reference operator; mode 1: launches that
  * cannot change the result folded or dropped, and the levels whose arrays
- * hold at most tail_cells cells (default 20000; 0 = none) run in one
+ * hold at most tail_cells cells (default 4096; 0 = none) run in one
  * single-workgroup kernel; mode 2 (default): mode 1, with each V-cycle
  * captured once into a HIP graph and replayed.  Results are bit-identical in
  * every mode. */
 int artss_mg_set_schedule(artss_mg *mg, int mode, long long tail_cells);
-/* first level run by the single-workgroup kernel (n_level: none) */
-int artss_mg_tail_level(artss_mg *mg, int *first);
+/* first level run by the single-workgroup kernel (n_level: none) and the
+ * LDS it uses for the level arrays it keeps on chip (bytes, may be NULL) */
+int artss_mg_tail_info(artss_mg *mg, int *first, int *lds_bytes);
 
 /* ---- solver: SolverController (solver/SolverController.cpp:26) + TimeIntegration ---- */
 int artss_solver_create(const char *xml_text, int device, artss_solver **out);
@@ -134,6 +135,10 @@ int artss_solver_step(artss_solver *s, double t);
 int artss_solver_run(artss_solver *s, long first_step, long count, double *seconds);
 /* device pointer of a named field: u v w p rhs T nu_t kappa_t f_x f_y f_z S_T T_ambient u0 ... */
 int artss_solver_field(artss_solver *s, const char *name, double **dev, size_t *n);
+/* Analysis::save_variables_in_file (analysis/Analysis.cpp:360-408): <name>.dat (one value per cell,
+ * default ostream format) and <name>_obstacle.dat ("index;value") for every field type with domain
+ * boundary data, into `directory`; *files = number of files written (may be NULL) */
+int artss_solver_write_dat(artss_solver *s, const char *directory, int *files);
 /* last step: diffusion iterations (u, v, w, T), V-cycles, final pressure residual (-1 if not measured) */
 int artss_solver_counters(artss_solver *s, int *diffusion4, int *vcycles, double *residual);
 /* 0: reference pressure schedule (VCycleMG::pressure); 1: V-cycles until the residual <= tolerance */
@@ -143,8 +148,13 @@ int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tol
  * iteration); method 1 uses one V-cycle per iteration as the preconditioner
  * of a GCR iteration with one kept direction, which meets the same target
  * ||rhs - lap p||_2 in far fewer V-cycles when the V-cycle alone stalls
- * (obstacles).  The final residual is the true one in both. */
-int artss_solver_set_pressure_method(artss_solver *s, int method);
+ * (obstacles); method 2 is method 1 with the V-cycle's levels above its
+ * single-workgroup coarse levels in FP32 (smoothing, residuum, restriction,
+ * prolongation), the coarse levels, the GCR iteration, residuals and norms
+ * in FP64; with method 2, coarse_max_iter > 0 caps the preconditioner's
+ * coarse Jacobi solve (0: the XML's max_iter).  The final residual is the
+ * true FP64 one in every method. */
+int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max_iter);
 /* artss_mg_set_schedule for the solver's own pressure solver */
 int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cells);
 

@@ -19,7 +19,7 @@ OBST = [(0.375, 0.625, 0.0, 0.375, 0.625, 0.8125)]
 
 def _run(xml, method, mode, tol, steps):
     s = artss_amd.Solver(xml)
-    s.set_mg_schedule(mode, 20000)
+    s.set_mg_schedule(mode, 4096)
     s.set_pressure_mode(True, tol, 400)
     s.set_pressure_method(method)
     counts = []
@@ -30,30 +30,32 @@ def _run(xml, method, mode, tol, steps):
 
 
 @pytest.mark.parametrize("obstacles", [OBST, []], ids=["obstacle", "open"])
-def test_gcr_meets_target_with_fewer_vcycles(obstacles):
+@pytest.mark.parametrize("method", [1, 2], ids=["gcr_fp64", "gcr_mixed"])
+def test_gcr_meets_target_with_fewer_vcycles(obstacles, method):
     xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=4, steps=4, dt=0.01, obstacles=obstacles, hrr=3.0,
                       source_x=1.0, open_top=True)
     tol = 1e-9
     f_v, c_v = _run(xml, 0, 2, tol, 4)
-    f_g, c_g = _run(xml, 1, 2, tol, 4)
+    f_g, c_g = _run(xml, method, 2, tol, 4)
     cyc_v = [c["vcycles"] for c in c_v]
     cyc_g = [c["vcycles"] for c in c_g]
-    print("V-cycles alone", cyc_v, "GCR", cyc_g, "residuals", [c["residual"] for c in c_g])
+    errs = {f: float(np.abs(f_g[f] - f_v[f]).max() / max(np.abs(f_v[f]).max(), 1e-30)) for f in f_v}
+    print("V-cycles alone", cyc_v, "GCR", cyc_g, "residuals", [c["residual"] for c in c_g], "rel err", errs)
     for c in c_g:
         assert 0 <= c["residual"] <= tol
     assert sum(cyc_g) <= sum(cyc_v)
     if obstacles:
         assert sum(cyc_g) * 2 < sum(cyc_v)
     for f in f_v:
-        scale = max(np.abs(f_v[f]).max(), 1e-30)
-        assert np.abs(f_g[f] - f_v[f]).max() / scale < 1e-5, f
+        assert errs[f] < 1e-5, f
 
 
-def test_gcr_graph_replay_matches_eager():
+@pytest.mark.parametrize("method", [1, 2], ids=["gcr_fp64", "gcr_mixed"])
+def test_gcr_graph_replay_matches_eager(method):
     xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=4, steps=3, dt=0.01, obstacles=OBST, hrr=3.0,
                       source_x=1.0, open_top=True)
-    f1, c1 = _run(xml, 1, 1, 1e-9, 3)
-    f2, c2 = _run(xml, 1, 2, 1e-9, 3)
+    f1, c1 = _run(xml, method, 1, 1e-9, 3)
+    f2, c2 = _run(xml, method, 2, 1e-9, 3)
     assert c1 == c2
     for f in f1:
         assert np.array_equal(f1[f], f2[f]), f

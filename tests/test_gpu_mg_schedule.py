@@ -65,7 +65,8 @@ def _run(xml, mode, tail_cells, p0, rhs, dt):
         mg.vcycle(gp, grhs)
     states.append(gp.download())
     res = mg.residual(gp, grhs)
-    tail = mg.tail_level()
+    tail, lds = mg.tail_info()
+    print("first tail level", tail, "LDS bytes", lds)
     return counts, states, res, tail
 
 
