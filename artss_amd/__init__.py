@@ -84,6 +84,21 @@ def lib():
         "artss_solver_set_mg_schedule": (C.c_int, [vp, C.c_int, C.c_longlong]),
         "artss_solver_set_pressure_method": (C.c_int, [vp, C.c_int, C.c_int]),
         "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
+        "artss_solver_set_warm_start": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "artss_solver_warm_sample": (C.c_int, [vp, vp, vp, ip]),
+        "artss_solver_initial_residual": (C.c_int, [vp, dp]),
+        "artss_buffer_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
+        "artss_buffer_free": (C.c_int, [vp, vp]),
+        "artss_buffer_upload": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "artss_buffer_download": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "artss_unet_create": (C.c_int, [vp, C.c_ulonglong, C.POINTER(vp)]),
+        "artss_unet_destroy": (C.c_int, [vp]),
+        "artss_unet_param_count": (C.c_int, [C.POINTER(C.c_size_t)]),
+        "artss_unet_set_params": (C.c_int, [vp, vp]),
+        "artss_unet_get_params": (C.c_int, [vp, vp]),
+        "artss_unet_forward": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp]),
+        "artss_unet_gradients": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.POINTER(C.c_float)]),
+        "artss_unet_adam_step": (C.c_int, [vp, C.c_float]),
         "artss_profile_arm": (C.c_int, [vp, C.c_char_p]),
         "artss_profile_read": (C.c_int, [vp, C.POINTER(C.c_longlong), dp]),
     }
@@ -355,7 +370,9 @@ class Solver:
         vc = C.c_int()
         r = C.c_double()
         _check(self.L.artss_solver_counters(self.h, d, C.byref(vc), C.byref(r)))
-        return dict(diffusion=list(d), vcycles=vc.value, residual=r.value)
+        r0 = C.c_double()
+        _check(self.L.artss_solver_initial_residual(self.h, C.byref(r0)))
+        return dict(diffusion=list(d), vcycles=vc.value, residual=r.value, initial_residual=r0.value)
 
     def set_pressure_mode(self, to_tolerance, tolerance=1e-8, max_cycles=200):
         _check(self.L.artss_solver_set_pressure_mode(self.h, 1 if to_tolerance else 0, tolerance, max_cycles))
@@ -372,5 +389,102 @@ class Solver:
         preconditioner's coarse solve in method 2."""
         _check(self.L.artss_solver_set_pressure_method(self.h, int(method), int(coarse_max_iter)))
 
+    def set_warm_start(self, mode, net=None, record=False):
+        """Start of the GCR pressure iteration: 0 the previous p, 1 linear extrapolation in
+        time, 2 extrapolation plus the correction of the warm-start UNet `net`; `record`
+        keeps each step's network input and target for warm_sample()."""
+        self._warm_net = net  # the solver holds a pointer to it
+        _check(self.L.artss_solver_set_warm_start(self.h, int(mode), net.h if net is not None else None,
+                                                  1 if record else 0))
+
+    def warm_sample(self, x_buf, t_buf):
+        """Copies the last recorded (input, target) into Buffers of shapes (Z, Y, X, 2) and
+        (Z, Y, X, 1), float32; False when nothing is recorded yet."""
+        ok = C.c_int()
+        _check(self.L.artss_solver_warm_sample(self.h, x_buf.ptr, t_buf.ptr, C.byref(ok)))
+        return bool(ok.value)
+
     def set_mg_schedule(self, mode=2, tail_cells=4096):
         _check(self.L.artss_solver_set_mg_schedule(self.h, int(mode), int(tail_cells)))
+
+
+class Buffer:
+    """Raw device memory on a domain's device holding a numpy array's bytes."""
+
+    def __init__(self, dom, array):
+        self.dom = dom
+        a = np.ascontiguousarray(array)
+        self.dtype, self.shape, self.nbytes = a.dtype, a.shape, a.nbytes
+        h = C.c_void_p()
+        _check(dom.L.artss_buffer_alloc(dom.h, a.nbytes, C.byref(h)))
+        self.ptr = h
+        self.upload(a)
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(self.dom.L.artss_buffer_upload(self.dom.h, self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def download(self):
+        a = np.empty(self.shape, self.dtype)
+        _check(self.dom.L.artss_buffer_download(self.dom.h, a.ctypes.data_as(C.c_void_p), self.ptr, a.nbytes))
+        return a
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.dom.L.artss_buffer_free(self.dom.h, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class UNet:
+    """Warm-start U-Net (BF16 matrix-core convolutions) on a domain's stream.
+    Volumes are (Z, Y, X, C) float32 arrays, sides multiples of 8."""
+
+    def __init__(self, dom, seed=0):
+        self.dom = dom
+        h = C.c_void_p()
+        _check(dom.L.artss_unet_create(dom.h, seed, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.dom.L.artss_unet_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @staticmethod
+    def param_count():
+        n = C.c_size_t()
+        _check(lib().artss_unet_param_count(C.byref(n)))
+        return n.value
+
+    def set_params(self, flat):
+        a = np.ascontiguousarray(flat, dtype=np.float32)
+        assert a.size == self.param_count()
+        _check(self.dom.L.artss_unet_set_params(self.h, a.ctypes.data_as(C.c_void_p)))
+
+    def get_params(self):
+        a = np.empty(self.param_count(), np.float32)
+        _check(self.dom.L.artss_unet_get_params(self.h, a.ctypes.data_as(C.c_void_p)))
+        return a
+
+    def forward(self, x_buf, y_buf):
+        Z, Y, X = x_buf.shape[:3]
+        _check(self.dom.L.artss_unet_forward(self.h, X, Y, Z, x_buf.ptr, y_buf.ptr))
+
+    def gradients(self, x_buf, t_buf, want_grads=True):
+        Z, Y, X = x_buf.shape[:3]
+        loss = C.c_float()
+        g = np.empty(self.param_count(), np.float32) if want_grads else None
+        _check(self.dom.L.artss_unet_gradients(self.h, X, Y, Z, x_buf.ptr, t_buf.ptr,
+                                               g.ctypes.data_as(C.c_void_p) if want_grads else None,
+                                               C.byref(loss)))
+        return loss.value, g
+
+    def adam_step(self, lr):
+        _check(self.dom.L.artss_unet_adam_step(self.h, lr))

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "solver.h"
+#include "unet.h"
 
 using namespace artss;
 
@@ -19,6 +20,11 @@ struct artss_domain {
 struct artss_mg {
     artss_domain *d;
     std::unique_ptr<VCycleMG> mg;
+};
+
+struct artss_unet {
+    artss_domain *d;
+    std::unique_ptr<UNet> net;
 };
 
 struct artss_solver {
@@ -549,12 +555,152 @@ int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max
     });
 }
 
+int artss_solver_set_warm_start(artss_solver *s, int mode, artss_unet *net, int record) {
+    return guard([&] {
+        require(s, "null argument");
+        if (net) require(dom(net->d).stream() == s->solver->domain().stream(),
+                         "the network must be created on the solver's domain");
+        WarmStart w;
+        w.mode = mode;
+        w.net = net ? net->net.get() : nullptr;
+        w.record = record != 0;
+        s->solver->set_warm_start(w);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_warm_sample(artss_solver *s, float *x, float *t, int *available) {
+    return guard([&] {
+        require(s && x && t && available, "null argument");
+        *available = s->solver->warm_sample(x, t) ? 1 : 0;
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_initial_residual(artss_solver *s, double *residual) {
+    return guard([&] {
+        require(s && residual, "null argument");
+        *residual = s->solver->counters().initial_residual;
+        return ARTSS_OK;
+    });
+}
+
 int artss_solver_set_mg_schedule(artss_solver *s, int mode, long long tail_cells) {
     return guard([&] {
         require(s && tail_cells >= 0 && mode >= 0 && mode <= 2, "bad argument");
         s->solver->pressure_solver().set_schedule(mode != 0, tail_cells);
         s->solver->pressure_solver().set_graphs(mode == 2);
         finish(s->solver->domain());
+        return ARTSS_OK;
+    });
+}
+
+int artss_buffer_alloc(artss_domain *d, size_t bytes, void **dev) {
+    return guard([&] {
+        require(dev && bytes > 0, "bad argument");
+        dom(d);
+        void *p = nullptr;
+        ARTSS_HIP(hipMalloc(&p, bytes));
+        ARTSS_HIP(hipMemset(p, 0, bytes));
+        *dev = p;
+        return ARTSS_OK;
+    });
+}
+
+int artss_buffer_free(artss_domain *d, void *dev) {
+    return guard([&] {
+        dom(d);
+        if (dev) ARTSS_HIP(hipFree(dev));
+        return ARTSS_OK;
+    });
+}
+
+int artss_buffer_upload(artss_domain *d, void *dev, const void *host, size_t bytes) {
+    return guard([&] {
+        require(dev && host, "null argument");
+        Domain &D = dom(d);
+        ARTSS_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, D.stream()));
+        ARTSS_HIP(hipStreamSynchronize(D.stream()));
+        return ARTSS_OK;
+    });
+}
+
+int artss_buffer_download(artss_domain *d, void *host, const void *dev, size_t bytes) {
+    return guard([&] {
+        require(dev && host, "null argument");
+        Domain &D = dom(d);
+        ARTSS_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, D.stream()));
+        ARTSS_HIP(hipStreamSynchronize(D.stream()));
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_create(artss_domain *d, unsigned long long seed, artss_unet **out) {
+    return guard([&] {
+        require(out, "null argument");
+        Domain &D = dom(d);
+        auto u = std::make_unique<artss_unet>();
+        u->d = d;
+        u->net = std::make_unique<UNet>(D.stream(), seed);
+        *out = u.release();
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_destroy(artss_unet *u) {
+    return guard([&] {
+        require(u, "null argument");
+        delete u;
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_param_count(size_t *n) {
+    return guard([&] {
+        require(n, "null argument");
+        *n = UNet::param_count();
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_set_params(artss_unet *u, const float *host) {
+    return guard([&] {
+        require(u && host, "null argument");
+        u->net->set_params(host);
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_get_params(artss_unet *u, float *host) {
+    return guard([&] {
+        require(u && host, "null argument");
+        u->net->get_params(host);
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_forward(artss_unet *u, int X, int Y, int Z, const float *x, float *y) {
+    return guard([&] {
+        require(u && x && y, "null argument");
+        u->net->forward(X, Y, Z, x, y);
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_gradients(artss_unet *u, int X, int Y, int Z, const float *x, const float *target, float *grads,
+                         float *loss) {
+    return guard([&] {
+        require(u && x && target, "null argument");
+        const float l = u->net->gradients(X, Y, Z, x, target, grads);
+        if (loss) *loss = l;
+        return ARTSS_OK;
+    });
+}
+
+int artss_unet_adam_step(artss_unet *u, float lr) {
+    return guard([&] {
+        require(u && lr > 0, "bad argument");
+        u->net->adam_step(lr);
         return ARTSS_OK;
     });
 }

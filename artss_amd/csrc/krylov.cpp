@@ -8,7 +8,8 @@
 namespace artss {
 
 GcrPressure::GcrPressure(Domain &d, VCycleMG &preconditioner, bool mixed, int coarse_max_iter)
-    : m_domain(d), m_pre(preconditioner), m_mixed(mixed), m_coarse_max_iter(coarse_max_iter), m_r(d, P), m_z(d, P), m_w(d, P), m_zp(d, P), m_wp(d, P) {
+    : m_domain(d), m_pre(preconditioner), m_mixed(mixed), m_coarse_max_iter(coarse_max_iter), m_r(d, P), m_z(d, P), m_zp(d, P),
+      m_wp(d, P) {
     if (m_mixed && !m_pre.mixed_supported())
         throw config_error("the mixed-precision pressure solve needs the weighted-Jacobi smoother, boundary fills "
                            "without chained obstacle faces and a coarse-level kernel starting below level 0");
@@ -18,6 +19,10 @@ GcrPressure::GcrPressure(Domain &d, VCycleMG &preconditioner, bool mixed, int co
     ARTSS_HIP(hipMalloc(&m_coef, sizeof(double) * 4));
     ARTSS_HIP(hipMalloc(&m_sum, sizeof(double)));
     ARTSS_HIP(hipHostMalloc(&m_sum_host, sizeof(double)));
+    ARTSS_HIP(hipMalloc(&m_zp32, sizeof(float) * static_cast<size_t>(L.size)));
+    ARTSS_HIP(hipMalloc(&m_wp32, sizeof(float) * static_cast<size_t>(L.size)));
+    ARTSS_HIP(hipMemset(m_zp32, 0, sizeof(float) * static_cast<size_t>(L.size)));
+    ARTSS_HIP(hipMemset(m_wp32, 0, sizeof(float) * static_cast<size_t>(L.size)));
 }
 
 GcrPressure::~GcrPressure() {
@@ -27,6 +32,8 @@ GcrPressure::~GcrPressure() {
     (void)hipFree(m_coef);
     (void)hipFree(m_sum);
     (void)hipHostFree(m_sum_host);
+    (void)hipFree(m_zp32);
+    (void)hipFree(m_wp32);
 }
 
 // One iteration: z = M r (one V-cycle from zero), the linear part of the
@@ -37,22 +44,23 @@ void GcrPressure::iteration(Field &p) {
     const LevelDesc &L = d.level(0);
     hipStream_t s = d.stream();
     if (m_mixed) {
-        // the FP32 result already carries the linear part of the fill
+        // the FP32 result already carries the linear part of the fill; the
+        // kept direction is stored in FP32 as well
         m_pre.vcycle_mixed(m_r.data, m_coarse_max_iter);
         const float *z = m_pre.mixed_result();
-        d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_w.data, z, m_wp.data, m_r.data, m_partial); });
+        d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, z, m_wp32, m_r.data, m_partial); });
         kk::gcr_coef(s, m_partial, kk::blocks_inner(L), m_coef);
         d.timed("gcr_update", 0, [&] {
-            kk::gcr_update(s, L, m_coef, p.data, m_r.data, z, m_zp.data, m_w.data, m_wp.data, m_partial);
+            kk::gcr_update(s, L, m_coef, p.data, m_r.data, z, m_zp32, m_wp32, m_partial);
         });
     } else {
         m_z.set_value(0.0);
         m_pre.vcycle_eager(m_z, m_r);
         d.apply_boundary_linear(m_z.data, P, 0);
-        d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_w.data, m_z.data, m_wp.data, m_r.data, m_partial); });
+        d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_z.data, m_wp.data, m_r.data, m_partial); });
         kk::gcr_coef(s, m_partial, kk::blocks_inner(L), m_coef);
         d.timed("gcr_update", 0, [&] {
-            kk::gcr_update(s, L, m_coef, p.data, m_r.data, m_z.data, m_zp.data, m_w.data, m_wp.data, m_partial);
+            kk::gcr_update(s, L, m_coef, p.data, m_r.data, m_z.data, m_zp.data, m_wp.data, m_partial);
         });
     }
     kk::sum(s, m_partial, kk::blocks_full(L), m_sum);
@@ -109,10 +117,13 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
         ARTSS_HIP(hipMemcpyAsync(m_sum_host, m_sum, sizeof(double), hipMemcpyDeviceToHost, s));
         m_zp.set_value(0.0);
         m_wp.set_value(0.0);
+        k::fill32(s, m_zp32, 0.0f, L.size);
+        k::fill32(s, m_wp32, 0.0f, L.size);
         ARTSS_HIP(hipStreamSynchronize(s));
         return std::sqrt(*m_sum_host);
     };
     double res = restart();
+    m_initial = res;
     int it = 0;
     bool confirmed_once = false;
     while (true) {

@@ -32,6 +32,8 @@ class GcrPressure {
     // iterate until ||b - lap p||_2 <= tol (or max_iter); returns the
     // iterations (V-cycles) taken, *residual the final true residual
     int solve(Field &p, const Field &b, double tol, int max_iter, double *residual);
+    // ||b - lap p||_2 at the start of the last solve
+    double initial_residual() const { return m_initial; }
 
   private:
     void iteration(Field &p);          // enqueue one GCR iteration
@@ -42,11 +44,13 @@ class GcrPressure {
     VCycleMG &m_pre;
     bool m_mixed;
     int m_coarse_max_iter;
-    Field m_r, m_z, m_w, m_zp, m_wp;
+    Field m_r, m_z, m_zp, m_wp;
+    float *m_zp32 = nullptr, *m_wp32 = nullptr;  // kept direction of the mixed iteration
     double *m_partial = nullptr;       // 4 sums x blocks
     double *m_coef = nullptr;          // device: beta, 1/|w'|, alpha
     double *m_sum = nullptr;           // device scalar
     double *m_sum_host = nullptr;      // pinned
+    double m_initial = -1.0;
     hipGraphExec_t m_graph = nullptr;
     double *m_graph_p = nullptr;
     const double *m_graph_b = nullptr;

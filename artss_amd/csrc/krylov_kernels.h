@@ -14,18 +14,40 @@ int blocks_full(const LevelDesc &L);   // ... of the whole-array kernels
 
 // r = b - lap(x) on inner cells (x must carry its boundary fill); 1 sum: (r, r)
 void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x, const double *b, double *partial);
-// w = lap(z) on inner cells; 4 sums: (w,w), (w,wp), (r,w), (r,wp)
-template <typename Tz>
-void lap_dots(hipStream_t s, const LevelDesc &L, double *w, const Tz *z, const double *wp, const double *r,
-              double *partial);
+// w = lap(z) on inner cells (not stored); 4 sums: (w,w), (w,wp), (r,w), (r,wp)
+template <typename Tz, typename Tw>
+void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tw *wp, const double *r, double *partial);
 // GCR coefficients from the 4 sums: coef[0] beta, coef[1] 1/|w - beta wp|, coef[2] alpha
 void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef);
-// the GCR step over the whole array; 1 sum: (r, r)
-template <typename Tz>
-void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, double *zp,
-                const double *w, double *wp, double *partial);
+// the GCR step over the whole array (w = lap(z) recomputed); 1 sum: (r, r).
+// The kept direction (zp, wp) is stored as Tw.
+template <typename Tz, typename Tw>
+void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, Tw *zp,
+                Tw *wp, double *partial);
 // out[0] = fixed-order sum of n partials
 void sum(hipStream_t s, const double *partial, int n, double *out);
+
+}  // namespace kk
+}  // namespace artss
+
+namespace artss {
+namespace kk {
+
+// ---- warm start of the pressure solve (solver.cpp), p_n the previous
+// solution (carrying its fill), p_prev the one before, d = p_n - p_prev,
+// s = rms(d) over the fluid cells of level 0, inner box volumes x fastest
+// *sum = sum of d^2 over the fluid cells
+void warm_sum(hipStream_t s, const LevelDesc &L, const double *p, const double *p_prev, double *partial,
+              double *sum);
+// network input [v][2]: c (b - lap p) / s and d / s, zero off the fluid
+void warm_pack(hipStream_t s, const LevelDesc &L, const double *p, const double *b, const double *p_prev,
+               const double *sum, double fluid_cells, double c, float *x);
+// p += d (+ s y[v] when y is given) on the fluid cells; nothing when d == 0
+void warm_apply(hipStream_t s, const LevelDesc &L, const double *sum, double fluid_cells, const float *y,
+                const double *p_prev, double *p);
+// training target [v]: (p_new - p_n - d) / s, zero off the fluid
+void warm_target(hipStream_t s, const LevelDesc &L, const double *sum, double fluid_cells, const double *p_new,
+                 const double *p_start, const double *p_prev, float *t);
 
 }  // namespace kk
 }  // namespace artss

@@ -80,16 +80,16 @@ __global__ void k_residual_vec(LevelDesc L, double *__restrict__ r, const double
     write_partials<1>(v, partial);
 }
 
-// w = lap(z) on inner cells; partials (w,w), (w,wp), (r,w), (r,wp)
-template <typename Tz>
-__global__ void k_lap_dots(LevelDesc L, double *__restrict__ w, const Tz *__restrict__ z,
-                           const double *__restrict__ wp, const double *__restrict__ r, double *__restrict__ partial) {
+// w = lap(z) on inner cells (not stored: the update recomputes it);
+// partials (w,w), (w,wp), (r,w), (r,wp)
+template <typename Tz, typename Tw>
+__global__ void k_lap_dots(LevelDesc L, const Tz *__restrict__ z, const Tw *__restrict__ wp,
+                           const double *__restrict__ r, double *__restrict__ partial) {
     int i, j, k;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (inner_at(L, i, j, k)) {
         const long long idx = dev::lin(L, i, j, k);
         const double x = lap(L, z, idx);
-        w[idx] = x;
         const double p = wp[idx], q = r[idx];
         v[0] = x * x;
         v[1] = x * p;
@@ -130,26 +130,30 @@ __global__ void k_gcr_coef(const double *__restrict__ partial, int blocks, doubl
     }
 }
 
-// z' = (z - beta zp)/|w'|, w' = (w - beta wp)/|w'|; x += alpha z'; r -= alpha w';
-// zp = z', wp = w' over the whole array; partial (r, r)
-template <typename Tz>
+// w = lap(z) on inner cells, 0 elsewhere; z' = (z - beta zp)/|w'|,
+// w' = (w - beta wp)/|w'|; x += alpha z'; r -= alpha w'; zp = z', wp = w'
+// over the whole array; partial (r, r)
+template <typename Tz, typename Tw>
 __global__ void k_gcr_update(LevelDesc L, const double *__restrict__ coef, double *__restrict__ x,
-                             double *__restrict__ r, const Tz *__restrict__ z, double *__restrict__ zp,
-                             const double *__restrict__ w, double *__restrict__ wp, double *__restrict__ partial) {
+                             double *__restrict__ r, const Tz *__restrict__ z, Tw *__restrict__ zp,
+                             Tw *__restrict__ wp, double *__restrict__ partial) {
     const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
     const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
     const int k = static_cast<int>(blockIdx.z);
     double v[1] = {0.0};
     if (i < L.N[0] && j < L.N[1]) {
         const long long idx = dev::lin(L, i, j, k);
+        const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
+                           !in_obstacle(L, i, j, k);
+        const double w = inner ? lap(L, z, idx) : 0.0;
         const double beta = coef[0], inv = coef[1], alpha = coef[2];
-        const double zn = (double(z[idx]) - beta * zp[idx]) * inv;
-        const double wn = (w[idx] - beta * wp[idx]) * inv;
+        const double zn = (double(z[idx]) - beta * double(zp[idx])) * inv;
+        const double wn = (w - beta * double(wp[idx])) * inv;
         x[idx] += alpha * zn;
         const double rn = r[idx] - alpha * wn;
         r[idx] = rn;
-        zp[idx] = zn;
-        wp[idx] = wn;
+        zp[idx] = static_cast<Tw>(zn);
+        wp[idx] = static_cast<Tw>(wn);
         v[0] = rn * rn;
     }
     write_partials<1>(v, partial);
@@ -169,7 +173,86 @@ __global__ void k_sum(const double *__restrict__ partial, int n, double *__restr
     }
 }
 
+// ---- warm start
+__global__ void k_warm_partials(LevelDesc L, const double *__restrict__ p, const double *__restrict__ p_prev,
+                                double *__restrict__ partial) {
+    int i, j, k;
+    double v[1] = {0.0};
+    if (inner_at(L, i, j, k)) {
+        const long long idx = dev::lin(L, i, j, k);
+        const double d = p[idx] - p_prev[idx];
+        v[0] = d * d;
+    }
+    write_partials<1>(v, partial);
+}
+// 1 / rms of the increment over the fluid cells, 0 when there is none
+__device__ inline double inv_rms(const double *sum, double n) { return *sum > 0.0 ? 1.0 / sqrt(*sum / n) : 0.0; }
+__device__ inline long long box_index(const LevelDesc &L, int i, int j, int k) {
+    return (i - 1) + static_cast<long long>(L.n[0]) * ((j - 1) + static_cast<long long>(L.n[1]) * (k - 1));
+}
+__global__ void k_warm_pack(LevelDesc L, const double *__restrict__ p, const double *__restrict__ b,
+                            const double *__restrict__ p_prev, const double *__restrict__ sum, double n,
+                            double c, float *__restrict__ x) {
+    const int i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = 1 + static_cast<int>(blockIdx.z);
+    if (i > L.n[0] || j > L.n[1]) return;
+    float r = 0.f, d = 0.f;
+    if (!in_obstacle(L, i, j, k)) {
+        const long long idx = dev::lin(L, i, j, k);
+        const double q = inv_rms(sum, n);
+        r = static_cast<float>(c * (b[idx] - dev::laplace(L, p, idx)) * q);
+        d = static_cast<float>((p[idx] - p_prev[idx]) * q);
+    }
+    const long long v = box_index(L, i, j, k);
+    x[2 * v] = r;
+    x[2 * v + 1] = d;
+}
+__global__ void k_warm_apply(LevelDesc L, const double *__restrict__ sum, double n, const float *__restrict__ y,
+                             const double *__restrict__ p_prev, double *__restrict__ p) {
+    int i, j, k;
+    if (!inner_at(L, i, j, k)) return;
+    if (!(*sum > 0.0)) return;
+    const long long idx = dev::lin(L, i, j, k);
+    double v = p[idx] - p_prev[idx];
+    if (y != nullptr) v += sqrt(*sum / n) * static_cast<double>(y[box_index(L, i, j, k)]);
+    p[idx] += v;
+}
+__global__ void k_warm_target(LevelDesc L, const double *__restrict__ sum, double n,
+                              const double *__restrict__ p_new, const double *__restrict__ p_start,
+                              const double *__restrict__ p_prev, float *__restrict__ t) {
+    const int i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
+    const int j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
+    const int k = 1 + static_cast<int>(blockIdx.z);
+    if (i > L.n[0] || j > L.n[1]) return;
+    float a = 0.f;
+    if (!in_obstacle(L, i, j, k)) {
+        const long long idx = dev::lin(L, i, j, k);
+        const double e = (p_new[idx] - p_start[idx]) - (p_start[idx] - p_prev[idx]);
+        a = static_cast<float>(e * inv_rms(sum, n));
+    }
+    t[box_index(L, i, j, k)] = a;
+}
+
 }  // namespace
+
+void warm_sum(hipStream_t s, const LevelDesc &L, const double *p, const double *p_prev, double *partial,
+              double *sum) {
+    k_warm_partials<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, p, p_prev, partial);
+    k_sum<<<1, 1024, 0, s>>>(partial, blocks_inner(L), sum);
+}
+void warm_pack(hipStream_t s, const LevelDesc &L, const double *p, const double *b, const double *p_prev,
+               const double *sum, double fluid_cells, double c, float *x) {
+    k_warm_pack<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, p, b, p_prev, sum, fluid_cells, c, x);
+}
+void warm_apply(hipStream_t s, const LevelDesc &L, const double *sum, double fluid_cells, const float *y,
+                const double *p_prev, double *p) {
+    k_warm_apply<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, sum, fluid_cells, y, p_prev, p);
+}
+void warm_target(hipStream_t s, const LevelDesc &L, const double *sum, double fluid_cells, const double *p_new,
+                 const double *p_start, const double *p_prev, float *t) {
+    k_warm_target<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, sum, fluid_cells, p_new, p_start, p_prev, t);
+}
 
 int blocks_inner(const LevelDesc &L) {
     const dim3 g = inner_grid(L);
@@ -183,27 +266,26 @@ int blocks_full(const LevelDesc &L) {
 void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x, const double *b, double *partial) {
     k_residual_vec<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, r, x, b, partial);
 }
-template <typename Tz>
-void lap_dots(hipStream_t s, const LevelDesc &L, double *w, const Tz *z, const double *wp, const double *r,
-              double *partial) {
-    k_lap_dots<Tz><<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, w, z, wp, r, partial);
+template <typename Tz, typename Tw>
+void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tw *wp, const double *r, double *partial) {
+    k_lap_dots<Tz, Tw><<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, z, wp, r, partial);
 }
-template void lap_dots<double>(hipStream_t, const LevelDesc &, double *, const double *, const double *,
-                               const double *, double *);
-template void lap_dots<float>(hipStream_t, const LevelDesc &, double *, const float *, const double *,
-                              const double *, double *);
+template void lap_dots<double, double>(hipStream_t, const LevelDesc &, const double *, const double *,
+                                       const double *, double *);
+template void lap_dots<float, float>(hipStream_t, const LevelDesc &, const float *, const float *, const double *,
+                                     double *);
 void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef) {
     k_gcr_coef<<<1, 1024, 0, s>>>(partial, blocks, coef);
 }
-template <typename Tz>
-void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, double *zp,
-                const double *w, double *wp, double *partial) {
-    k_gcr_update<Tz><<<full_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, w, wp, partial);
+template <typename Tz, typename Tw>
+void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, Tw *zp,
+                Tw *wp, double *partial) {
+    k_gcr_update<Tz, Tw><<<full_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, wp, partial);
 }
-template void gcr_update<double>(hipStream_t, const LevelDesc &, const double *, double *, double *, const double *,
-                                 double *, const double *, double *, double *);
-template void gcr_update<float>(hipStream_t, const LevelDesc &, const double *, double *, double *, const float *,
-                                double *, const double *, double *, double *);
+template void gcr_update<double, double>(hipStream_t, const LevelDesc &, const double *, double *, double *,
+                                         const double *, double *, double *, double *);
+template void gcr_update<float, float>(hipStream_t, const LevelDesc &, const double *, double *, double *,
+                                       const float *, float *, float *, double *);
 void sum(hipStream_t s, const double *partial, int n, double *out) { k_sum<<<1, 1024, 0, s>>>(partial, n, out); }
 
 }  // namespace kk

@@ -1,8 +1,11 @@
 #include "solver.h"
 
 #include <fstream>
+#include <utility>
 
 #include <cmath>
+
+#include "krylov_kernels.h"
 
 namespace artss {
 
@@ -154,6 +157,88 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
 
 Solver::~Solver() {
     if (m_domain) (void)hipStreamSynchronize(m_domain->stream());
+    for (void *q : {static_cast<void *>(m_p_prev), static_cast<void *>(m_p_start),
+                    static_cast<void *>(m_warm_partial), static_cast<void *>(m_warm_sum),
+                    static_cast<void *>(m_warm_x), static_cast<void *>(m_warm_y), static_cast<void *>(m_warm_t)})
+        if (q) (void)hipFree(q);
+}
+
+void Solver::set_warm_start(const WarmStart &w) {
+    if (w.mode < 0 || w.mode > 2) throw config_error("warm start mode must be 0, 1 or 2");
+    if (w.mode == 2 && w.net == nullptr) throw config_error("the learned warm start needs a network");
+    const LevelDesc &L = m_domain->level(0);
+    const bool net = w.mode == 2 || w.record;
+    if (net && (L.n[0] % 8 || L.n[1] % 8 || L.n[2] % 8))
+        throw config_error("the warm-start network needs inner box sides that are multiples of 8");
+    const size_t n = static_cast<size_t>(L.size);
+    const size_t v = static_cast<size_t>(L.n[0]) * L.n[1] * L.n[2];
+    if (!m_p_prev) {
+        ARTSS_HIP(hipMalloc(&m_p_prev, sizeof(double) * n));
+        ARTSS_HIP(hipMalloc(&m_p_start, sizeof(double) * n));
+        ARTSS_HIP(hipMalloc(&m_warm_partial, sizeof(double) * static_cast<size_t>(kk::blocks_inner(L))));
+        ARTSS_HIP(hipMalloc(&m_warm_sum, sizeof(double)));
+    }
+    if (net && !m_warm_x) {
+        ARTSS_HIP(hipMalloc(&m_warm_x, sizeof(float) * 2 * v));
+        ARTSS_HIP(hipMalloc(&m_warm_y, sizeof(float) * v));
+        ARTSS_HIP(hipMalloc(&m_warm_t, sizeof(float) * v));
+    }
+    // the previous solution stays valid across a change of mode when it
+    // was being kept
+    const bool kept = m_warm.mode != 0 || m_warm.record;
+    m_warm = w;
+    if (!kept) m_warm_history = 0;
+    if (!w.record) m_warm_sample = false;
+}
+
+bool Solver::warm_sample(float *x, float *t) {
+    if (!m_warm_sample) return false;
+    const LevelDesc &L = m_domain->level(0);
+    const size_t v = static_cast<size_t>(L.n[0]) * L.n[1] * L.n[2];
+    hipStream_t s = m_domain->stream();
+    ARTSS_HIP(hipMemcpyAsync(x, m_warm_x, sizeof(float) * 2 * v, hipMemcpyDeviceToDevice, s));
+    ARTSS_HIP(hipMemcpyAsync(t, m_warm_t, sizeof(float) * v, hipMemcpyDeviceToDevice, s));
+    ARTSS_HIP(hipStreamSynchronize(s));
+    return true;
+}
+
+// p_n is saved; with a previous solution p is moved to the warm start and
+// the network input is kept for recording
+void Solver::warm_begin() {
+    Domain &d = *m_domain;
+    FieldController &f = *m_fields;
+    const LevelDesc &L = d.level(0);
+    hipStream_t s = d.stream();
+    ARTSS_HIP(hipMemcpyAsync(m_p_start, f.p.data, sizeof(double) * static_cast<size_t>(L.size),
+                             hipMemcpyDeviceToDevice, s));
+    if (m_warm_history == 0) return;
+    const double fluid = static_cast<double>(d.inner_cells(0));
+    kk::warm_sum(s, L, f.p.data, m_p_prev, m_warm_partial, m_warm_sum);
+    const bool net = m_warm.mode == 2 || m_warm.record;
+    if (net) {
+        const double c = 1.0 / (2.0 / (L.h[0] * L.h[0]) + 2.0 / (L.h[1] * L.h[1]) + 2.0 / (L.h[2] * L.h[2]));
+        kk::warm_pack(s, L, f.p.data, f.rhs.data, m_p_prev, m_warm_sum, fluid, c, m_warm_x);
+    }
+    if (m_warm.mode == 2) {
+        d.timed("warm_unet", 0, [&] { m_warm.net->forward(L.n[0], L.n[1], L.n[2], m_warm_x, m_warm_y); });
+        kk::warm_apply(s, L, m_warm_sum, fluid, m_warm_y, m_p_prev, f.p.data);
+    } else if (m_warm.mode == 1) {
+        kk::warm_apply(s, L, m_warm_sum, fluid, nullptr, m_p_prev, f.p.data);
+    }
+}
+
+// the target of this step (p_{n+1} - p_n - d) / s when recording; p_n
+// becomes p_prev of the next solve
+void Solver::warm_end() {
+    Domain &d = *m_domain;
+    const LevelDesc &L = d.level(0);
+    if (m_warm.record && m_warm_history > 0) {
+        kk::warm_target(d.stream(), L, m_warm_sum, static_cast<double>(d.inner_cells(0)), m_fields->p.data,
+                        m_p_start, m_p_prev, m_warm_t);
+        m_warm_sample = true;
+    }
+    std::swap(m_p_prev, m_p_start);
+    m_warm_history++;
 }
 
 // SolverController.cpp:26-43, 95-205
@@ -219,9 +304,13 @@ void Solver::pressure_step(double t) {
             m_gcr_mixed = mixed;
             m_gcr_coarse = m_pmode.coarse_max_iter;
         }
+        const bool warm = m_warm.mode != 0 || m_warm.record;
+        if (warm) warm_begin();
         double r = 0.0;
         m_counters.vcycles = m_gcr->solve(f.p, f.rhs, m_pmode.tolerance, m_pmode.max_cycles, &r);
         m_counters.residual = r;
+        m_counters.initial_residual = m_gcr->initial_residual();
+        if (warm) warm_end();
         return;
     }
     double r = m_pres->residual(f.p, f.rhs);

@@ -10,6 +10,7 @@
 
 #include "operators.h"
 #include "krylov.h"
+#include "unet.h"
 #include "vcycle.h"
 
 namespace artss {
@@ -36,6 +37,7 @@ struct StepCounters {
     int diffusion[4] = {0, 0, 0, 0};  // u, v, w, T
     int vcycles = 0;
     double residual = -1.0;           // final pressure residual when measured
+    double initial_residual = -1.0;   // pressure residual at the start of the iteration (GCR methods)
 };
 
 // How the pressure equation is solved each step.
@@ -50,6 +52,22 @@ struct PressureMode {
     // 2 as 1 with the V-cycle's upper levels in FP32
     int method = 0;
     int coarse_max_iter = 0;  // method 2: cap of the preconditioner's coarse solve (0: the XML's)
+};
+
+// Where the residual-target GCR pressure iteration starts (methods 1, 2),
+// from p_n (the previous step's solution) and p_prev (the one before):
+//   0  p_n, the start the reference's iteration uses
+//   1  p_n + d with d = p_n - p_prev (linear extrapolation in time)
+//   2  p_n + d + s * net(c r / s, d / s), the warm-start U-Net's correction
+//      to the extrapolation, r = rhs - lap p_n, s = rms(d) over the fluid
+//      cells, c = 1 / (2/dx^2 + 2/dy^2 + 2/dz^2)
+// The iteration then runs to the same residual target from any start.
+// `record` keeps the last step's network input and target, (p_{n+1} - p_n -
+// d) / s, for training.
+struct WarmStart {
+    int mode = 0;
+    UNet *net = nullptr;  // mode 2, not owned
+    bool record = false;
 };
 
 class Solver {
@@ -74,7 +92,17 @@ class Solver {
     // Returns the number of files written.
     int write_dat(const std::string &dir);
 
+    // the start of the GCR pressure iteration; throws config_error when
+    // mode 2 or `record` is asked for on an inner box whose sides are not
+    // multiples of 8 (the network halves it three times)
+    void set_warm_start(const WarmStart &w);
+    // copies the last recorded sample into device buffers x [V][2] and t [V]
+    // (V = inner box cells, x fastest); false when none is recorded yet
+    bool warm_sample(float *x, float *t);
+
   private:
+    void warm_begin();
+    void warm_end();
     void do_step_ns(double t);
     void do_step_nstempturb(double t);
     void pressure_step(double t);
@@ -97,6 +125,12 @@ class Solver {
     bool m_add_source = false, m_add_temp_source = false;
     StepCounters m_counters;
     PressureMode m_pmode;
+    WarmStart m_warm;
+    int m_warm_history = 0;  // solves since the warm start was set
+    bool m_warm_sample = false;
+    double *m_p_prev = nullptr, *m_p_start = nullptr;
+    double *m_warm_partial = nullptr, *m_warm_sum = nullptr;
+    float *m_warm_x = nullptr, *m_warm_y = nullptr, *m_warm_t = nullptr;
 };
 
 // initial conditions on the inner list (Functions.cpp) for the supported usr_fct values

@@ -1,0 +1,580 @@
+// Kernels of the pressure warm-start U-Net (unet.{h,cpp}).
+//
+// Tensors are channels-last (voxel-major, channel innermost) at the
+// resolution of their level, without ghost cells.  Activations and weights
+// are BF16 (raw bit patterns, round-to-nearest-even), gradients FP32, and
+// every product accumulates in FP32.  The 3x3x3 convolutions with 16 or more
+// channels on both sides are implicit GEMMs on the gfx950 matrix cores
+// (v_mfma_f32_16x16x16_bf16): a wavefront owns 16 consecutive voxels along
+// x and all output channels; per tap and 16-channel chunk it loads one
+// 16x16 BF16 operand of the input (8 bytes per lane, contiguous in the
+// channel) and one of the weights per output tile.  Operand layouts:
+//   A (16x16, rows m, cols k): lane l holds A[l%16][4*(l/16) .. +3]
+//   B (16x16, rows k, cols n): lane l holds B[4*(l/16) .. +3][l%16]
+//   D (16x16 FP32):            lane l holds D[4*(l/16) .. +3][l%16]
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+
+#include "unet_kernels.h"
+
+namespace artss {
+namespace nn {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float(static_cast<unsigned>(b) << 16); }
+__device__ inline uint16_t f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ inline short f2bfs(float f) { return static_cast<short>(f2bf(f)); }
+
+__device__ inline long long vox(const Dims &D, int x, int y, int z) {
+    return (static_cast<long long>(z) * D.Y + y) * D.X + x;
+}
+
+__device__ inline void tap_offset(int t, int &dx, int &dy, int &dz) {
+    dz = t / 9 - 1;
+    dy = (t / 3) % 3 - 1;
+    dx = t % 3 - 1;
+}
+
+// ---------------------------------------------------------------- forward
+// out[v][co] = act(b[co] + sum_t sum_ci in[v + off_t][ci] * W[t][ci][co]) (+ skip[v][co])
+// Wt is [27][COUT][CIN].  UP: the input lives at half resolution and is read
+// nearest-upsampled.  pre (optional): the pre-activation values.
+template <int CIN, int COUT, bool UP>
+__global__ void __launch_bounds__(256) k_conv_fwd(Dims D, const uint16_t *__restrict__ in,
+                                                  const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
+                                                  uint16_t *__restrict__ out, uint16_t *__restrict__ pre,
+                                                  const uint16_t *__restrict__ skip, int relu) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = (blockIdx.x * 4 + wave) * 16;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x0 >= D.X) return;  // no workgroup barrier in this kernel
+    const int vx = x0 + (lane & 15), kq = lane >> 4;
+    f32x4 acc[COUT / 16];
+#pragma unroll
+    for (int ct = 0; ct < COUT / 16; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 27; t++) {
+        int dx, dy, dz;
+        tap_offset(t, dx, dy, dz);
+        const int xx = vx + dx, yy = y + dy, zz = z + dz;
+        const bool inside = vx < D.X && xx >= 0 && xx < D.X && yy >= 0 && yy < D.Y && zz >= 0 && zz < D.Z;
+        long long src = 0;
+        if (inside) {
+            if (UP) {
+                const Dims C{D.X >> 1, D.Y >> 1, D.Z >> 1};
+                src = vox(C, xx >> 1, yy >> 1, zz >> 1);
+            } else {
+                src = vox(D, xx, yy, zz);
+            }
+        }
+#pragma unroll
+        for (int c0 = 0; c0 < CIN; c0 += 16) {
+            bf16x4 b = bf16x4{0, 0, 0, 0};
+            if (inside) b = *reinterpret_cast<const bf16x4 *>(in + src * CIN + c0 + 4 * kq);
+#pragma unroll
+            for (int ct = 0; ct < COUT / 16; ct++) {
+                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(
+                    Wt + (static_cast<long long>(t) * COUT + ct * 16 + (lane & 15)) * CIN + c0 + 4 * kq);
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
+            }
+        }
+    }
+    if (vx >= D.X) return;
+    const long long v = vox(D, vx, y, z);
+#pragma unroll
+    for (int ct = 0; ct < COUT / 16; ct++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int co = ct * 16 + 4 * kq + e;
+            const float c = acc[ct][e] + bias[co];
+            if (pre) pre[v * COUT + co] = f2bf(c);
+            float r = relu ? fmaxf(c, 0.f) : c;
+            if (skip) r += bf2f(skip[v * COUT + co]);
+            out[v * COUT + co] = f2bf(r);
+        }
+}
+
+// First layer: x is FP32 with CI (2) channels, 16 BF16 outputs with ReLU.
+template <int CI, int CO>
+__global__ void k_conv_in(Dims D, const float *__restrict__ x, const float *__restrict__ W,
+                          const float *__restrict__ bias, uint16_t *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (i >= D.X) return;
+    float acc[CO];
+#pragma unroll
+    for (int c = 0; c < CO; c++) acc[c] = bias[c];
+    for (int t = 0; t < 27; t++) {
+        int dx, dy, dz;
+        tap_offset(t, dx, dy, dz);
+        const int xx = i + dx, yy = y + dy, zz = z + dz;
+        if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
+        const long long s = vox(D, xx, yy, zz);
+#pragma unroll
+        for (int ci = 0; ci < CI; ci++) {
+            const float a = x[s * CI + ci];
+#pragma unroll
+            for (int c = 0; c < CO; c++) acc[c] += a * W[(t * CI + ci) * CO + c];
+        }
+    }
+    const long long v = vox(D, i, y, z);
+#pragma unroll
+    for (int c = 0; c < CO; c++) out[v * CO + c] = f2bf(fmaxf(acc[c], 0.f));
+}
+
+// Last layer: CI BF16 channels -> one FP32 output, no activation.
+template <int CI>
+__global__ void k_conv_out(Dims D, const uint16_t *__restrict__ in, const float *__restrict__ W,
+                           const float *__restrict__ bias, float *__restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yy0 = blockIdx.y, z = blockIdx.z;
+    if (i >= D.X) return;
+    float acc = bias[0];
+    for (int t = 0; t < 27; t++) {
+        int dx, dy, dz;
+        tap_offset(t, dx, dy, dz);
+        const int xx = i + dx, yy = yy0 + dy, zz = z + dz;
+        if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
+        const long long s = vox(D, xx, yy, zz);
+#pragma unroll
+        for (int ci = 0; ci < CI; ci++) acc += bf2f(in[s * CI + ci]) * W[t * CI + ci];
+    }
+    y[vox(D, i, yy0, z)] = acc;
+}
+
+// 2x2x2 mean, D: the output (coarse) dims
+template <int C>
+__global__ void k_down(Dims D, const uint16_t *__restrict__ in, uint16_t *__restrict__ out) {
+    const long long n = static_cast<long long>(D.X) * D.Y * D.Z * C;
+    const Dims F{D.X * 2, D.Y * 2, D.Z * 2};
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x) {
+        const int c = static_cast<int>(q % C);
+        const long long v = q / C;
+        const int x = static_cast<int>(v % D.X), y = static_cast<int>((v / D.X) % D.Y),
+                  z = static_cast<int>(v / (static_cast<long long>(D.X) * D.Y));
+        float s = 0.f;
+        for (int o = 0; o < 8; o++) s += bf2f(in[vox(F, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + (o >> 2)) * C + c]);
+        out[q] = f2bf(s * 0.125f);
+    }
+}
+
+// ---------------------------------------------------------------- backward
+// gin[v][ci] = sum_t sum_co gout[v - off_t][co] * W[t][ci][co] (zero outside);
+// W is [27][CIN][COUT]; gout FP32 (rounded to BF16 as the matrix operand).
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv_dgrad(Dims D, const float *__restrict__ gout,
+                                                    const uint16_t *__restrict__ W, float *__restrict__ gin) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = (blockIdx.x * 4 + wave) * 16;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x0 >= D.X) return;
+    const int vx = x0 + (lane & 15), kq = lane >> 4;
+    f32x4 acc[CIN / 16];
+#pragma unroll
+    for (int ct = 0; ct < CIN / 16; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 27; t++) {
+        int dx, dy, dz;
+        tap_offset(t, dx, dy, dz);
+        const int xx = vx - dx, yy = y - dy, zz = z - dz;
+        const bool inside = vx < D.X && xx >= 0 && xx < D.X && yy >= 0 && yy < D.Y && zz >= 0 && zz < D.Z;
+        const long long src = inside ? vox(D, xx, yy, zz) : 0;
+#pragma unroll
+        for (int c0 = 0; c0 < COUT; c0 += 16) {
+            bf16x4 b = bf16x4{0, 0, 0, 0};
+            if (inside) {
+                const float *g = gout + src * COUT + c0 + 4 * kq;
+                b = bf16x4{f2bfs(g[0]), f2bfs(g[1]), f2bfs(g[2]), f2bfs(g[3])};
+            }
+#pragma unroll
+            for (int ct = 0; ct < CIN / 16; ct++) {
+                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(
+                    W + (static_cast<long long>(t) * CIN + ct * 16 + (lane & 15)) * COUT + c0 + 4 * kq);
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
+            }
+        }
+    }
+    if (vx >= D.X) return;
+    const long long v = vox(D, vx, y, z);
+#pragma unroll
+    for (int ct = 0; ct < CIN / 16; ct++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) gin[v * CIN + ct * 16 + 4 * kq + e] = acc[ct][e];
+}
+
+// gW[t][ci][co] += sum_v in[v + off_t][ci] * gout[v][co] over this
+// workgroup's x-lines (grid: line blocks x 27 taps).  Each wavefront
+// accumulates its lines' 16-voxel chunks as matrix products with the voxels
+// as the reduction index, then adds its tiles to gW with FP32 atomics.
+// UP: the forward read `in` nearest-upsampled from half resolution.
+template <int CIN, int COUT, bool UP>
+__global__ void __launch_bounds__(256) k_conv_wgrad(Dims D, const uint16_t *__restrict__ in,
+                                                    const float *__restrict__ gout, float *__restrict__ gW,
+                                                    int lines_per_block) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = blockIdx.y;
+    int dx, dy, dz;
+    tap_offset(t, dx, dy, dz);
+    const int kq = lane >> 4, m = lane & 15;
+    f32x4 acc[COUT / 16][CIN / 16];
+#pragma unroll
+    for (int a = 0; a < COUT / 16; a++)
+#pragma unroll
+        for (int b = 0; b < CIN / 16; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int lines = D.Y * D.Z;
+    const int chunks = (D.X + 15) / 16;
+    const int first = blockIdx.x * lines_per_block;
+    const int last = min(lines, first + lines_per_block);
+    for (int line = first + wave; line < last; line += 4) {
+        const int y = line % D.Y, z = line / D.Y;
+        for (int ch = 0; ch < chunks; ch++) {
+            // reduction index k = voxel 4*kq + e of this chunk
+            bf16x4 ga[COUT / 16], ib[CIN / 16];
+#pragma unroll
+            for (int a = 0; a < COUT / 16; a++) ga[a] = bf16x4{0, 0, 0, 0};
+#pragma unroll
+            for (int b = 0; b < CIN / 16; b++) ib[b] = bf16x4{0, 0, 0, 0};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int x = ch * 16 + 4 * kq + e;
+                if (x >= D.X) continue;
+                const long long v = vox(D, x, y, z);
+#pragma unroll
+                for (int a = 0; a < COUT / 16; a++) ga[a][e] = f2bfs(gout[v * COUT + a * 16 + m]);
+                const int xx = x + dx, yy = y + dy, zz = z + dz;
+                if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
+                long long s;
+                if (UP) {
+                    const Dims C{D.X >> 1, D.Y >> 1, D.Z >> 1};
+                    s = vox(C, xx >> 1, yy >> 1, zz >> 1);
+                } else {
+                    s = vox(D, xx, yy, zz);
+                }
+#pragma unroll
+                for (int b = 0; b < CIN / 16; b++) ib[b][e] = static_cast<short>(in[s * CIN + b * 16 + m]);
+            }
+#pragma unroll
+            for (int a = 0; a < COUT / 16; a++)
+#pragma unroll
+                for (int b = 0; b < CIN / 16; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga[a], ib[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    // D[m = co][n = ci]: lane holds co = a*16 + 4*kq + e, ci = b*16 + m
+#pragma unroll
+    for (int a = 0; a < COUT / 16; a++)
+#pragma unroll
+        for (int b = 0; b < CIN / 16; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int co = a * 16 + 4 * kq + e, ci = b * 16 + m;
+                atomicAdd(&gW[(static_cast<long long>(t) * CIN + ci) * COUT + co], acc[a][b][e]);
+            }
+}
+
+// weight gradients of the first layer (CI FP32 inputs -> CO outputs):
+// one thread per weight entry (t, ci, co) sums x[v + off_t][ci] * g[v][co]
+// over this workgroup's voxel range, then adds it to gW atomically.
+template <int CI, int CO>
+__global__ void k_conv_in_wgrad(Dims D, const float *__restrict__ x, const float *__restrict__ g,
+                                float *__restrict__ gW, long long voxels_per_block) {
+    const int q = threadIdx.x;
+    if (q >= 27 * CI * CO) return;
+    const int t = q / (CI * CO), ci = (q / CO) % CI, co = q % CO;
+    int dx, dy, dz;
+    tap_offset(t, dx, dy, dz);
+    const long long n = static_cast<long long>(D.X) * D.Y * D.Z;
+    const long long v0 = blockIdx.x * voxels_per_block, v1 = min(n, v0 + voxels_per_block);
+    float acc = 0.f;
+    for (long long v = v0; v < v1; v++) {
+        const int i = static_cast<int>(v % D.X), y = static_cast<int>((v / D.X) % D.Y),
+                  z = static_cast<int>(v / (static_cast<long long>(D.X) * D.Y));
+        const int xx = i + dx, yy = y + dy, zz = z + dz;
+        if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
+        acc += x[vox(D, xx, yy, zz) * CI + ci] * g[v * CO + co];
+    }
+    atomicAdd(&gW[q], acc);
+}
+
+// gradients of the last layer (CI BF16 inputs -> 1 FP32 output, no activation):
+// weights as above, one thread per (t, ci); d in by a separate pass
+template <int CI>
+__global__ void k_conv_out_wgrad(Dims D, const uint16_t *__restrict__ in, const float *__restrict__ gy,
+                                 float *__restrict__ gW, long long voxels_per_block) {
+    const int q = threadIdx.x;
+    if (q >= 27 * CI) return;
+    const int t = q / CI, ci = q % CI;
+    int dx, dy, dz;
+    tap_offset(t, dx, dy, dz);
+    const long long n = static_cast<long long>(D.X) * D.Y * D.Z;
+    const long long v0 = blockIdx.x * voxels_per_block, v1 = min(n, v0 + voxels_per_block);
+    float acc = 0.f;
+    for (long long v = v0; v < v1; v++) {
+        const int i = static_cast<int>(v % D.X), y = static_cast<int>((v / D.X) % D.Y),
+                  z = static_cast<int>(v / (static_cast<long long>(D.X) * D.Y));
+        const int xx = i + dx, yy = y + dy, zz = z + dz;
+        if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
+        acc += bf2f(in[vox(D, xx, yy, zz) * CI + ci]) * gy[v];
+    }
+    atomicAdd(&gW[q], acc);
+}
+template <int CI>
+__global__ void k_conv_out_dgrad(Dims D, const float *__restrict__ W, const float *__restrict__ gy,
+                                 float *__restrict__ gin) {
+    const long long n = static_cast<long long>(D.X) * D.Y * D.Z;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += 256LL * gridDim.x) {
+        const int i = static_cast<int>(v % D.X), y = static_cast<int>((v / D.X) % D.Y),
+                  z = static_cast<int>(v / (static_cast<long long>(D.X) * D.Y));
+        float acc[CI];
+#pragma unroll
+        for (int ci = 0; ci < CI; ci++) acc[ci] = 0.f;
+        for (int t = 0; t < 27; t++) {
+            int dx, dy, dz;
+            tap_offset(t, dx, dy, dz);
+            const int xx = i - dx, yy = y - dy, zz = z - dz;
+            if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
+            const float gs = gy[vox(D, xx, yy, zz)];
+#pragma unroll
+            for (int ci = 0; ci < CI; ci++) acc[ci] += gs * W[t * CI + ci];
+        }
+#pragma unroll
+        for (int ci = 0; ci < CI; ci++) gin[v * CI + ci] = acc[ci];
+    }
+}
+
+// bias gradient: gb[c] += sum over voxels of g[v][c]
+template <int C>
+__global__ void k_bias_grad(long long n, const float *__restrict__ g, float *__restrict__ gb) {
+    __shared__ float part[C];
+    for (int q = threadIdx.x; q < C; q += blockDim.x) part[q] = 0.f;
+    __syncthreads();
+    float acc[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) acc[c] = 0.f;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += 256LL * gridDim.x)
+#pragma unroll
+        for (int c = 0; c < C; c++) acc[c] += g[v * C + c];
+#pragma unroll
+    for (int c = 0; c < C; c++) atomicAdd(&part[c], acc[c]);
+    __syncthreads();
+    for (int q = threadIdx.x; q < C; q += blockDim.x) atomicAdd(&gb[q], part[q]);
+}
+
+// g *= (act > 0): the ReLU mask from stored BF16 activations
+__global__ void k_relu_mask(long long n, float *__restrict__ g, const uint16_t *__restrict__ act) {
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x)
+        if (!(bf2f(act[q]) > 0.f)) g[q] = 0.f;
+}
+
+// dst = src (the skip gradient) ; dst += src
+__global__ void k_copy_f(long long n, float *__restrict__ dst, const float *__restrict__ src) {
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x) dst[q] = src[q];
+}
+
+// coarse gradient of the 2x2x2 mean: gc (coarse dims D) spread /8 onto gf (fine) and added
+template <int C>
+__global__ void k_down_backward_add(Dims D, const float *__restrict__ gc, float *__restrict__ gf) {
+    const Dims F{D.X * 2, D.Y * 2, D.Z * 2};
+    const long long n = static_cast<long long>(F.X) * F.Y * F.Z * C;
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x) {
+        const int c = static_cast<int>(q % C);
+        const long long v = q / C;
+        const int x = static_cast<int>(v % F.X), y = static_cast<int>((v / F.X) % F.Y),
+                  z = static_cast<int>(v / (static_cast<long long>(F.X) * F.Y));
+        gf[q] += 0.125f * gc[vox(D, x >> 1, y >> 1, z >> 1) * C + c];
+    }
+}
+
+// gradient of nearest upsampling: gc[v][c] = sum of the 8 fine children (D: coarse dims)
+template <int C>
+__global__ void k_up_backward(Dims D, const float *__restrict__ gf, float *__restrict__ gc) {
+    const Dims F{D.X * 2, D.Y * 2, D.Z * 2};
+    const long long n = static_cast<long long>(D.X) * D.Y * D.Z * C;
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x) {
+        const int c = static_cast<int>(q % C);
+        const long long v = q / C;
+        const int x = static_cast<int>(v % D.X), y = static_cast<int>((v / D.X) % D.Y),
+                  z = static_cast<int>(v / (static_cast<long long>(D.X) * D.Y));
+        float s = 0.f;
+        for (int o = 0; o < 8; o++) s += gf[vox(F, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + (o >> 2)) * C + c];
+        gc[q] = s;
+    }
+}
+
+// loss partials and dL/dy for the mean squared error over n voxels
+__global__ void k_mse(long long n, const float *__restrict__ y, const float *__restrict__ target,
+                      float *__restrict__ gy, float *__restrict__ loss) {
+    float s = 0.f;
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x) {
+        const float d = y[q] - target[q];
+        gy[q] = 2.f * d / static_cast<float>(n);
+        s += d * d;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, s / static_cast<float>(n));
+}
+
+// Adam on FP32 master weights; refreshes the BF16 copies in both layouts
+// ([t][ci][co] for the data gradient, [t][co][ci] for the forward pass).
+__global__ void k_adam(int n, int ci, int co, float *__restrict__ w, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2, float eps,
+                       float c1, float c2, uint16_t *__restrict__ w_bf, uint16_t *__restrict__ wt_bf) {
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += 256 * gridDim.x) {
+        const float gq = g[q];
+        const float mq = b1 * m[q] + (1.f - b1) * gq;
+        const float vq = b2 * v[q] + (1.f - b2) * gq * gq;
+        m[q] = mq;
+        v[q] = vq;
+        const float wq = w[q] - lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+        w[q] = wq;
+        if (w_bf) {
+            const int t = q / (ci * co), r = q % (ci * co), i = r / co, o = r % co;
+            w_bf[q] = f2bf(wq);
+            wt_bf[(t * co + o) * ci + i] = f2bf(wq);
+        }
+    }
+}
+
+// BF16 copies of FP32 weights in both layouts
+__global__ void k_weights_bf16(int n, int ci, int co, const float *__restrict__ w, uint16_t *__restrict__ w_bf,
+                               uint16_t *__restrict__ wt_bf) {
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += 256 * gridDim.x) {
+        const int t = q / (ci * co), r = q % (ci * co), i = r / co, o = r % co;
+        w_bf[q] = f2bf(w[q]);
+        wt_bf[(t * co + o) * ci + i] = f2bf(w[q]);
+    }
+}
+
+__global__ void k_fill_f(long long n, float *__restrict__ d, float v) {
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x) d[q] = v;
+}
+
+inline unsigned blocks_for(long long n) {
+    const long long b = (n + 255) / 256;
+    return static_cast<unsigned>(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+#define ARTSS_CONV_CASES(X) X(16, 16) X(16, 32) X(32, 64) X(64, 64) X(64, 32) X(32, 16)
+
+void conv_fwd(hipStream_t s, int cin, int cout, bool up, Dims D, const uint16_t *in, const uint16_t *Wt,
+              const float *bias, uint16_t *out, uint16_t *pre, const uint16_t *skip, bool relu) {
+    const dim3 g((D.X + 63) / 64, D.Y, D.Z);
+#define CASE(CI, CO)                                                                                            \
+    if (cin == CI && cout == CO) {                                                                             \
+        if (up) k_conv_fwd<CI, CO, true><<<g, 256, 0, s>>>(D, in, Wt, bias, out, pre, skip, relu ? 1 : 0);    \
+        else k_conv_fwd<CI, CO, false><<<g, 256, 0, s>>>(D, in, Wt, bias, out, pre, skip, relu ? 1 : 0);      \
+        return;                                                                                                \
+    }
+    ARTSS_CONV_CASES(CASE)
+#undef CASE
+    throw std::invalid_argument("conv_fwd: unsupported channel pair");
+}
+
+void conv_dgrad(hipStream_t s, int cin, int cout, Dims D, const float *gout, const uint16_t *W, float *gin) {
+    const dim3 g((D.X + 63) / 64, D.Y, D.Z);
+#define CASE(CI, CO)                                                      \
+    if (cin == CI && cout == CO) {                                       \
+        k_conv_dgrad<CI, CO><<<g, 256, 0, s>>>(D, gout, W, gin);         \
+        return;                                                          \
+    }
+    ARTSS_CONV_CASES(CASE)
+#undef CASE
+    throw std::invalid_argument("conv_dgrad: unsupported channel pair");
+}
+
+void conv_wgrad(hipStream_t s, int cin, int cout, bool up, Dims D, const uint16_t *in, const float *gout,
+                float *gW) {
+    const int lines = D.Y * D.Z;
+    const int per = 16;
+    const dim3 g((lines + per - 1) / per, 27);
+#define CASE(CI, CO)                                                                            \
+    if (cin == CI && cout == CO) {                                                             \
+        if (up) k_conv_wgrad<CI, CO, true><<<g, 256, 0, s>>>(D, in, gout, gW, per);             \
+        else k_conv_wgrad<CI, CO, false><<<g, 256, 0, s>>>(D, in, gout, gW, per);               \
+        return;                                                                                \
+    }
+    ARTSS_CONV_CASES(CASE)
+#undef CASE
+    throw std::invalid_argument("conv_wgrad: unsupported channel pair");
+}
+
+void conv_in(hipStream_t s, Dims D, const float *x, const float *W, const float *bias, uint16_t *out) {
+    k_conv_in<2, 16><<<dim3((D.X + 63) / 64, D.Y, D.Z), 64, 0, s>>>(D, x, W, bias, out);
+}
+void conv_out(hipStream_t s, Dims D, const uint16_t *in, const float *W, const float *bias, float *y) {
+    k_conv_out<16><<<dim3((D.X + 63) / 64, D.Y, D.Z), 64, 0, s>>>(D, in, W, bias, y);
+}
+void conv_in_wgrad(hipStream_t s, Dims D, const float *x, const float *g, float *gW, float *gb) {
+    const long long n = static_cast<long long>(D.X) * D.Y * D.Z;
+    const long long per = 2048;
+    k_conv_in_wgrad<2, 16><<<static_cast<unsigned>((n + per - 1) / per), 27 * 2 * 16, 0, s>>>(D, x, g, gW, per);
+    bias_grad(s, 16, n, g, gb);
+}
+void conv_out_grads(hipStream_t s, Dims D, const uint16_t *in, const float *W, const float *gy, float *gW,
+                    float *gb, float *gin) {
+    const long long n = static_cast<long long>(D.X) * D.Y * D.Z;
+    const long long per = 2048;
+    k_conv_out_wgrad<16><<<static_cast<unsigned>((n + per - 1) / per), 27 * 16, 0, s>>>(D, in, gy, gW, per);
+    k_conv_out_dgrad<16><<<blocks_for(n), 256, 0, s>>>(D, W, gy, gin);
+    bias_grad(s, 1, n, gy, gb);
+}
+void down(hipStream_t s, int c, Dims Dc, const uint16_t *in, uint16_t *out) {
+    const long long n = static_cast<long long>(Dc.X) * Dc.Y * Dc.Z * c;
+    if (c == 16) k_down<16><<<blocks_for(n), 256, 0, s>>>(Dc, in, out);
+    else if (c == 32) k_down<32><<<blocks_for(n), 256, 0, s>>>(Dc, in, out);
+    else if (c == 64) k_down<64><<<blocks_for(n), 256, 0, s>>>(Dc, in, out);
+    else throw std::invalid_argument("down: unsupported channels");
+}
+void down_backward_add(hipStream_t s, int c, Dims Dc, const float *gc, float *gf) {
+    const long long n = static_cast<long long>(Dc.X) * Dc.Y * Dc.Z * 8 * c;
+    if (c == 16) k_down_backward_add<16><<<blocks_for(n), 256, 0, s>>>(Dc, gc, gf);
+    else if (c == 32) k_down_backward_add<32><<<blocks_for(n), 256, 0, s>>>(Dc, gc, gf);
+    else if (c == 64) k_down_backward_add<64><<<blocks_for(n), 256, 0, s>>>(Dc, gc, gf);
+    else throw std::invalid_argument("down_backward_add: unsupported channels");
+}
+void up_backward(hipStream_t s, int c, Dims Dc, const float *gf, float *gc) {
+    const long long n = static_cast<long long>(Dc.X) * Dc.Y * Dc.Z * c;
+    if (c == 16) k_up_backward<16><<<blocks_for(n), 256, 0, s>>>(Dc, gf, gc);
+    else if (c == 32) k_up_backward<32><<<blocks_for(n), 256, 0, s>>>(Dc, gf, gc);
+    else if (c == 64) k_up_backward<64><<<blocks_for(n), 256, 0, s>>>(Dc, gf, gc);
+    else throw std::invalid_argument("up_backward: unsupported channels");
+}
+void bias_grad(hipStream_t s, int c, long long voxels, const float *g, float *gb) {
+    const unsigned b = blocks_for(voxels) > 512 ? 512 : blocks_for(voxels);
+    if (c == 1) k_bias_grad<1><<<b, 256, 0, s>>>(voxels, g, gb);
+    else if (c == 16) k_bias_grad<16><<<b, 256, 0, s>>>(voxels, g, gb);
+    else if (c == 32) k_bias_grad<32><<<b, 256, 0, s>>>(voxels, g, gb);
+    else if (c == 64) k_bias_grad<64><<<b, 256, 0, s>>>(voxels, g, gb);
+    else throw std::invalid_argument("bias_grad: unsupported channels");
+}
+void relu_mask(hipStream_t s, long long n, float *g, const uint16_t *act) {
+    k_relu_mask<<<blocks_for(n), 256, 0, s>>>(n, g, act);
+}
+void copy_f(hipStream_t s, long long n, float *dst, const float *src) {
+    k_copy_f<<<blocks_for(n), 256, 0, s>>>(n, dst, src);
+}
+void fill_f(hipStream_t s, long long n, float *dst, float v) { k_fill_f<<<blocks_for(n), 256, 0, s>>>(n, dst, v); }
+void mse(hipStream_t s, long long n, const float *y, const float *target, float *gy, float *loss) {
+    k_mse<<<blocks_for(n) > 1024 ? 1024 : blocks_for(n), 256, 0, s>>>(n, y, target, gy, loss);
+}
+void adam(hipStream_t s, int n, int ci, int co, float *w, const float *g, float *m, float *v, float lr, float b1,
+          float b2, float eps, float c1, float c2, uint16_t *w_bf, uint16_t *wt_bf) {
+    k_adam<<<blocks_for(n), 256, 0, s>>>(n, ci, co, w, g, m, v, lr, b1, b2, eps, c1, c2, w_bf, wt_bf);
+}
+void weights_bf16(hipStream_t s, int n, int ci, int co, const float *w, uint16_t *w_bf, uint16_t *wt_bf) {
+    k_weights_bf16<<<blocks_for(n), 256, 0, s>>>(n, ci, co, w, w_bf, wt_bf);
+}
+
+}  // namespace nn
+}  // namespace artss

@@ -6,17 +6,22 @@ Gauss-Seidel diffusion, ConstSmagorinsky, Boussinesq buoyancy, Gaussian heat
 source, geometric multigrid pressure) on 256x128x128 cells over 2x1x1 m with
 one box obstacle, from artss_amd/configs/plume_256x128x128.xml.  A step is
 one full time step over all inner cells; the pressure equation is solved
-with V-cycles until ||rhs - lap p||_2 <= 1e-8 every step (plain FP64 path).
+until ||rhs - lap p||_2 <= 1e-8 every step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--pressure mixed|gcr|vcycle]
 
 Prints one JSON line.  `value` is cell-updates/s = inner cells x K / wall
-time of the K timed steps, fields resident in HBM throughout.  `roofline`
-is the dominant kernel's algorithmic bytes per launch over its mean launch
-time (HIP events on the solver's stream, timed region only) against the
-8 TB/s HBM3E peak.  `cpu_baseline` runs the reference's own serial build
-(oracle/_ref, compiled from the reference sources) for a bounded sample
-continuing from the same state on this host.
+time of the K timed steps of the accelerated path (default: GCR with a
+mixed-precision V-cycle), fields resident in HBM throughout.  `plain_fp64`
+runs the plain FP64 path (the reference's V-cycles on their own, to the same
+residual) from rest over the same steps, with its throughput, the speedup of
+`value` over it and the largest relative difference of every field between
+the two paths.  `roofline` is the dominant kernel's algorithmic bytes per
+launch over its mean launch time (HIP events on the solver's stream around
+every launch in a timing region) against the 8 TB/s HBM3E peak.
+`cpu_baseline` runs the reference's own serial build (oracle/_ref, compiled
+from the reference sources) for a bounded sample continuing from the same
+state on this host.
 """
 import argparse
 import json
@@ -38,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # distinct array element read or written once, 8-byte doubles)
 BYTES_PER_CELL = {
     "jacobi_mg": 24,       # read x, b; write x'
+    "jacobi_mg32": 16,     # level 0 of the mixed V-cycle: read x (f32), b (f64 residual); write x' (f32)
     "cgs_mg": 16,          # one color: read x (all), b (half); write half
     "cgs_diffuse": 20,     # one color, turbulent: + eddy viscosity (half)
     "jacobi_diffuse": 32,  # turbulent: read x, b, ev; write x'
@@ -58,12 +64,14 @@ def parse():
     ap.add_argument("--tolerance", type=float, default=1e-8, help="pressure residual target per step")
     ap.add_argument("--reference-schedule", action="store_true",
                     help="use the reference's V-cycle schedule instead of the residual target")
-    ap.add_argument("--profile-kernel", default="jacobi_mg@0", help="kernel@level timed for the roofline")
+    ap.add_argument("--profile-kernel", default="jacobi_mg32@0", help="kernel@level timed for the roofline")
     ap.add_argument("--timing-steps", type=int, default=3, help="steps of the kernel timing region")
+    ap.add_argument("--plain-steps", type=int, default=5,
+                    help="timed steps of the plain FP64 V-cycle path run beside the accelerated one (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the reference CPU sample (0: skip)")
     ap.add_argument("--mg-schedule", choices=("graph", "fused", "plain"), default="graph",
                     help="V-cycle launch schedule (bit-identical results)")
-    ap.add_argument("--pressure", choices=("vcycle", "gcr", "mixed"), default="vcycle",
+    ap.add_argument("--pressure", choices=("vcycle", "gcr", "mixed"), default="mixed",
                     help="residual-target iteration: V-cycles alone (reference) or V-cycle-preconditioned GCR")
     ap.add_argument("--coarse-iterations", type=int, default=0,
                     help="mixed: cap of the preconditioner's coarse Jacobi solve (0: the XML's max_iter)")
@@ -93,6 +101,42 @@ def cpu_baseline(solver, config, first_step, steps, inner):
                                                                               first_step + steps - 1, secs)}
 
 
+PATHS = {"vcycle": "FP64, V-cycles alone (the reference's iteration)",
+         "gcr": "FP64, V-cycle-preconditioned GCR",
+         "mixed": "GCR in FP64 preconditioned by an FP32 V-cycle with FP64 coarse levels"}
+
+
+def make_solver(artss_amd, args, pressure):
+    solver = artss_amd.Solver(args.config, device=args.device)
+    solver.set_mg_schedule({"plain": 0, "fused": 1, "graph": 2}[args.mg_schedule], args.tail_cells)
+    if not args.reference_schedule:
+        solver.set_pressure_mode(True, args.tolerance, 500)
+        solver.set_pressure_method({"vcycle": 0, "gcr": 1, "mixed": 2}[pressure], args.coarse_iterations)
+    return solver
+
+
+class Run:
+    """Steps one solver from rest; timed(k) times k steps between syncs."""
+
+    def __init__(self, solver, dt):
+        self.solver, self.dt, self.step = solver, dt, 1
+
+    def timed(self, k):
+        dom = self.solver.domain
+        cyc, dif, res = [], [], []
+        dom.sync()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            self.solver.step(self.step * self.dt)
+            c = self.solver.counters()
+            cyc.append(c["vcycles"])
+            dif.append(c["diffusion"])
+            res.append(c["residual"])
+            self.step += 1
+        dom.sync()
+        return time.perf_counter() - t0, cyc, dif, res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -101,48 +145,28 @@ def main():
         sys.exit(2)
     import artss_amd
 
-    solver = artss_amd.Solver(args.config, device=args.device)
+    with open(args.config) as fh:
+        dt = float(fh.read().split("<dt>")[1].split("</dt>")[0])
+    solver = make_solver(artss_amd, args, args.pressure)
     dom = solver.domain
     info = dom.info(0)
     inner = info["inner_cells"]
-    solver.set_mg_schedule({"plain": 0, "fused": 1, "graph": 2}[args.mg_schedule], args.tail_cells)
-    if not args.reference_schedule:
-        solver.set_pressure_mode(True, args.tolerance, 500)
-        solver.set_pressure_method({"vcycle": 0, "gcr": 1, "mixed": 2}[args.pressure], args.coarse_iterations)
-    with open(args.config) as fh:
-        dt = float(fh.read().split("<dt>")[1].split("</dt>")[0])
-
-    step = 1
-    for _ in range(args.warmup):
-        solver.step(step * dt)
-        step += 1
-    first = step
-
-    def run(k):
-        nonlocal step
-        cyc, dif, res = [], [], []
-        dom.sync()
-        t0 = time.perf_counter()
-        for _ in range(k):
-            solver.step(step * dt)
-            c = solver.counters()
-            cyc.append(c["vcycles"])
-            dif.append(c["diffusion"])
-            res.append(c["residual"])
-            step += 1
-        dom.sync()
-        return time.perf_counter() - t0, cyc, dif, res
+    run = Run(solver, dt)
+    run.timed(args.warmup)
+    first = run.step
 
     # timed region: K steps as they run in production (V-cycles replayed from HIP graphs)
-    secs, cycles, diff_it, residuals = run(args.steps)
+    secs, cycles, diff_it, residuals = run.timed(args.steps)
     value = inner * args.steps / secs
+    after = run.step - 1
+    fields = {f: solver.field(f) for f in ("u", "v", "w", "p", "T")} if args.plain_steps > 0 else None
 
     # kernel timing region: HIP events around every launch of the profiled
     # kernel on the solver's stream; the V-cycle then runs its launches
     # eagerly (events cannot bracket launches inside a replayed graph)
     timing_steps = max(1, min(args.steps, args.timing_steps))
     dom.profile_arm(args.profile_kernel)
-    t_secs, t_cycles, _, _ = run(timing_steps)
+    t_secs, _, _, _ = run.timed(timing_steps)
     launches, kernel_ms = dom.profile_read()
     kname, klevel = args.profile_kernel.split("@")
     kinfo = dom.info(int(klevel))
@@ -157,24 +181,44 @@ def main():
 
     # the CPU sample continues from the state after the GPU regions (same
     # workload, later steps)
-    cpu = cpu_baseline(solver, args.config, step, args.cpu_steps, inner)
+    cpu = cpu_baseline(solver, args.config, run.step, args.cpu_steps, inner)
+    solver.close()
+
+    # plain FP64 path (V-cycles alone to the same residual) from rest over the
+    # same steps: its throughput, and the field error of the accelerated path
+    plain = None
+    if args.plain_steps > 0 and args.pressure != "vcycle" and not args.reference_schedule:
+        ps = make_solver(artss_amd, args, "vcycle")
+        prun = Run(ps, dt)
+        prun.timed(args.warmup)
+        k_plain = min(args.plain_steps, args.steps)
+        p_secs, p_cyc, _, p_res = prun.timed(k_plain)
+        prun.timed(after - prun.step + 1)  # to the step the accelerated fields were taken at
+        errors = {}
+        for f, a in fields.items():
+            ref = ps.field(f)
+            errors[f] = float(np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300))
+        ps.close()
+        plain = {"value": inner * k_plain / p_secs, "unit": "cell-updates/s", "steps": k_plain,
+                 "ms_per_step": p_secs * 1e3 / k_plain, "mean_vcycles": float(np.mean(p_cyc)),
+                 "max_final_residual": float(np.max(p_res)), "path": PATHS["vcycle"],
+                 "speedup_of_value": value / (inner * k_plain / p_secs),
+                 "max_rel_field_error_at_step_%d" % after: errors}
 
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.pressure != "mixed" else "f64 (pressure preconditioner f32)",
         "data": "synthetic: plume from rest, timed steps %d..%d" % (first, first + args.steps - 1),
         "config": {"workload": "plume_256x128x128", "grid": list(info["n"]), "inner_cells": inner,
-                   "path": {"vcycle": "FP64, V-cycles alone (reference iteration)",
-                            "gcr": "FP64, V-cycle-preconditioned GCR",
-                            "mixed": "GCR in FP64 preconditioned by an FP32 V-cycle with FP64 coarse levels"}[
-                       args.pressure],
-                   "mg_schedule": args.mg_schedule, "tail_cells": args.tail_cells,
+                   "path": PATHS[args.pressure], "mg_schedule": args.mg_schedule, "tail_cells": args.tail_cells,
                    "coarse_iterations": args.coarse_iterations,
                    "pressure": ("reference V-cycle schedule" if args.reference_schedule else
-                                "V-cycles until residual <= %g" % args.tolerance),
+                                "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance),
                    "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(
                        diff_it, axis=0).tolist(), "max_final_residual": float(np.max(residuals))},
+        "plain_fp64": plain,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -39,6 +39,7 @@ extern "C" {
 typedef struct artss_domain artss_domain;
 typedef struct artss_mg artss_mg;
 typedef struct artss_solver artss_solver;
+typedef struct artss_unet artss_unet;
 
 typedef struct {
     int N[3];               /* cells per axis including ghosts (DomainData::get_number_of_cells) */
@@ -157,6 +158,47 @@ int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tol
 int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max_iter);
 /* artss_mg_set_schedule for the solver's own pressure solver */
 int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cells);
+
+/* Start of the residual-target GCR pressure iteration (methods 1 and 2; no
+ * reference counterpart: the reference's iteration starts from the previous
+ * step's p, mode 0).  With p_n the previous solution, p_prev the one before
+ * and d = p_n - p_prev: mode 1 starts from p_n + d, mode 2 from p_n + d + s *
+ * net(c (rhs - lap p_n) / s, d / s) with s = rms(d) over the fluid cells and
+ * c = 1 / (2/dx^2 + 2/dy^2 + 2/dz^2), `net` created on the solver's domain.
+ * The iteration runs to the same residual target from any start.  `record`
+ * keeps each step's network input and its target (p_{n+1} - p_n - d) / s.
+ * Mode 2 and `record` need inner box sides that are multiples of 8. */
+int artss_solver_set_warm_start(artss_solver *s, int mode, artss_unet *net, int record);
+/* copies the last recorded sample to device buffers x [V][2] and t [V] (V
+ * inner box cells, x fastest); *available = 0 when none is recorded yet */
+int artss_solver_warm_sample(artss_solver *s, float *x, float *t, int *available);
+/* ||rhs - lap p||_2 at the start of the last step's GCR pressure iteration */
+int artss_solver_initial_residual(artss_solver *s, double *residual);
+
+/* ---- raw device buffers on a domain's device (any element type) ---- */
+int artss_buffer_alloc(artss_domain *d, size_t bytes, void **dev);
+int artss_buffer_free(artss_domain *d, void *dev);
+int artss_buffer_upload(artss_domain *d, void *dev, const void *host, size_t bytes);
+int artss_buffer_download(artss_domain *d, void *host, const void *dev, size_t bytes);
+
+/* ---- warm-start U-Net for the pressure solve (no reference counterpart) ----
+ * A 3-D convolutional U-Net (levels with 16, 32, 64, 64 channels, 3x3x3
+ * convolutions, BF16 weights and activations, FP32 accumulation on the
+ * matrix cores) on the domain's stream.  Tensors are channels-last FP32
+ * device arrays over an X x Y x Z voxel block (sides multiples of 8):
+ * input [V][2], output and target [V].  Parameters are FP32 in layer order
+ * in, e0, e1, e2, e3, d2, d1, d0, out, each W[27][c_in][c_out] then b[c_out]. */
+int artss_unet_create(artss_domain *d, unsigned long long seed, artss_unet **out);
+int artss_unet_destroy(artss_unet *u);
+int artss_unet_param_count(size_t *n);
+int artss_unet_set_params(artss_unet *u, const float *host);
+int artss_unet_get_params(artss_unet *u, float *host);
+int artss_unet_forward(artss_unet *u, int X, int Y, int Z, const float *x, float *y);
+/* mean squared error against target and its gradient (host `grads` may be NULL) */
+int artss_unet_gradients(artss_unet *u, int X, int Y, int Z, const float *x, const float *target, float *grads,
+                         float *loss);
+/* one Adam step (beta 0.9, 0.999) on the gradients of the last artss_unet_gradients call */
+int artss_unet_adam_step(artss_unet *u, float lr);
 
 /* ---- live kernel timing (no reference counterpart; the reference only
  * prints "Global Time", TimeIntegration.cpp:165-183) ----
