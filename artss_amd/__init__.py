@@ -8,6 +8,7 @@ computation of its own and raises if the library has not been built
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -140,8 +141,20 @@ class Domain:
         n = C.c_int()
         _check(self.L.artss_domain_levels(self.h, C.byref(n)))
         self.levels = n.value
+        self._dependents = weakref.WeakSet()
+
+    def _register(self, obj):
+        """obj (field, buffer, multigrid, network) uses this domain's device
+        memory or stream; it is released before the domain goes away."""
+        self._dependents.add(obj)
+
+    def _release_dependents(self):
+        for obj in list(getattr(self, "_dependents", ())):
+            obj._release()
 
     def close(self):
+        if self.h:
+            self._release_dependents()
         if getattr(self, "_owned", False) and self.h:
             _check(self.L.artss_domain_destroy(self.h))
         self.h = None
@@ -245,6 +258,7 @@ class DeviceField:
         _check(dom.L.artss_field_alloc(dom.h, level, C.byref(p)))
         self.ptr = p
         self._owned = True
+        dom._register(self)
 
     @classmethod
     def view(cls, dom, ptr, n, level=0):
@@ -265,9 +279,12 @@ class DeviceField:
         return out
 
     def free(self):
-        if self._owned and self.ptr:
+        if self._owned and self.ptr and self.dom.h:
             _check(self.dom.L.artss_field_free(self.dom.h, self.ptr))
-            self.ptr = None
+        self.ptr = None
+
+    def _release(self):
+        self.free()
 
     def __del__(self):
         try:
@@ -284,12 +301,16 @@ class Multigrid:
         h = C.c_void_p()
         _check(dom.L.artss_mg_create(dom.h, C.byref(h)))
         self.h = h
+        dom._register(self)
+
+    def _release(self):
+        if self.h:
+            self.dom.L.artss_mg_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:
-            if self.h:
-                self.dom.L.artss_mg_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -336,6 +357,10 @@ class Solver:
 
     def close(self):
         if self.h:
+            # fields, buffers and networks on the solver's domain go first:
+            # they use its stream and device
+            self.domain._release_dependents()
+            self.domain.h = None
             _check(self.L.artss_solver_destroy(self.h))
             self.h = None
 
@@ -393,9 +418,11 @@ class Solver:
         """Start of the GCR pressure iteration: 0 the previous p, 1 linear extrapolation in
         time, 2 extrapolation plus the correction of the warm-start UNet `net`; `record`
         keeps each step's network input and target for warm_sample()."""
-        self._warm_net = net  # the solver holds a pointer to it
         _check(self.L.artss_solver_set_warm_start(self.h, int(mode), net.h if net is not None else None,
                                                   1 if record else 0))
+        self._warm_net = net  # the solver holds a pointer to it
+        if net is not None:
+            net._users.add(self)
 
     def warm_sample(self, x_buf, t_buf):
         """Copies the last recorded (input, target) into Buffers of shapes (Z, Y, X, 2) and
@@ -418,6 +445,7 @@ class Buffer:
         h = C.c_void_p()
         _check(dom.L.artss_buffer_alloc(dom.h, a.nbytes, C.byref(h)))
         self.ptr = h
+        dom._register(self)
         self.upload(a)
 
     def upload(self, array):
@@ -430,11 +458,14 @@ class Buffer:
         _check(self.dom.L.artss_buffer_download(self.dom.h, a.ctypes.data_as(C.c_void_p), self.ptr, a.nbytes))
         return a
 
+    def _release(self):
+        if self.ptr and self.dom.h:
+            self.dom.L.artss_buffer_free(self.dom.h, self.ptr)
+        self.ptr = None
+
     def __del__(self):
         try:
-            if self.ptr:
-                self.dom.L.artss_buffer_free(self.dom.h, self.ptr)
-                self.ptr = None
+            self._release()
         except Exception:
             pass
 
@@ -448,12 +479,23 @@ class UNet:
         h = C.c_void_p()
         _check(dom.L.artss_unet_create(dom.h, seed, C.byref(h)))
         self.h = h
+        self._users = weakref.WeakSet()  # solvers warm-started by this network
+        dom._register(self)
+
+    def _release(self):
+        if self.h:
+            for s in list(self._users):
+                if s.h and getattr(s, "_warm_net", None) is self:
+                    s.set_warm_start(0)
+            self.dom.L.artss_unet_destroy(self.h)
+        self.h = None
+
+    def close(self):
+        self._release()
 
     def __del__(self):
         try:
-            if self.h:
-                self.dom.L.artss_unet_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 

@@ -36,10 +36,30 @@ inline unsigned grid1d(long long n, int threads) {
     return static_cast<unsigned>(g > 65536 ? 65536 : (g < 1 ? 1 : g));
 }
 
+// Block coordinates in the main part of a grid (blockIdx.z < main_z).  The
+// hardware deals workgroups to the 8 XCDs round robin by linear id; the
+// bijection id -> (id % 8) * per + id / 8 on the first 8 * per ids gives each
+// XCD a contiguous run of (x, y, z) blocks, so the z-neighbour planes a
+// stencil reads are mostly in that XCD's own L2 instead of being fetched
+// again by another XCD.  Cell arithmetic is unchanged; partial sums stay
+// indexed by the linear id.
+__device__ inline void main_block(int main_z, int &bx, int &by, int &bz) {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned per = gx * gy * static_cast<unsigned>(main_z) / 8u;
+    const unsigned nid = id < per * 8u ? (id % 8u) * per + id / 8u : id;
+    bx = static_cast<int>(nid % gx);
+    by = static_cast<int>((nid / gx) % gy);
+    bz = static_cast<int>(nid / (gx * gy));
+}
+
+// inner cell of a grid without item blocks
 __device__ inline bool inner_cell(const LevelDesc &L, int &i, int &j, int &k) {
-    i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    k = 1 + static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    main_block(static_cast<int>(gridDim.z), bx, by, bz);
+    i = 1 + bx * BX + static_cast<int>(threadIdx.x);
+    j = 1 + by * BY + static_cast<int>(threadIdx.y);
+    k = 1 + bz;
     return i <= L.n[0] && j <= L.n[1] && !in_obstacle(L, i, j, k);
 }
 
@@ -240,9 +260,11 @@ __global__ void k_jacobi_mg(LevelDesc L, double ax, double ay, double az, double
 __global__ void k_jacobi_mg_first(LevelDesc L, double ax, double ay, double az, double beta, double dsign,
                                   double w, double *__restrict__ dst, const double *__restrict__ src,
                                   const double *__restrict__ b) {
-    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    const int k = static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    main_block(static_cast<int>(gridDim.z), bx, by, bz);
+    const int i = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = by * BY + static_cast<int>(threadIdx.y);
+    const int k = bz;
     if (i >= L.N[0] || j >= L.N[1]) return;
     const long long idx = lin(L, i, j, k);
     const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
@@ -258,9 +280,11 @@ __global__ void k_jacobi_mg_first(LevelDesc L, double ax, double ay, double az, 
 __global__ void k_cgs_color(LevelDesc L, int color, double ax, double ay, double az, double beta, double dsign,
                             double w, double D, const double *__restrict__ ev, double dt, double rdx, double rdy,
                             double rdz, double *__restrict__ out, const double *__restrict__ b) {
-    const int t = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    const int j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    const int k = 1 + static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    main_block(static_cast<int>(gridDim.z), bx, by, bz);
+    const int t = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = 1 + by * BY + static_cast<int>(threadIdx.y);
+    const int k = 1 + bz;
     if (j > L.n[1]) return;
     const int i = (((1 + j + k) & 1) == color ? 1 : 2) + 2 * t;
     if (i > L.n[0] || in_obstacle(L, i, j, k)) return;
@@ -403,15 +427,22 @@ __device__ inline bool is_inner(const LevelDesc &L, long long c, int &i, int &j,
 __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict__ dst,
                           const double *__restrict__ src) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
-        dev::bc_item(L, V, item_index(main_z), dst, [&](long long c) { return src[c]; });
+        auto val = [&](long long c) { return src[c]; };
+        dev::bc_item<double, decltype(val), true>(L, V, item_index(main_z), dst, val);
         return;
     }
-    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    const int k = static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    main_block(main_z, bx, by, bz);
+    const int i = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = by * BY + static_cast<int>(threadIdx.y);
+    const int k = bz;
     if (i >= L.N[0] || j >= L.N[1]) return;
     const long long idx = lin(L, i, j, k);
-    if (!dev::written(V, idx)) dst[idx] = src[idx];
+    if (dev::written(V, idx)) return;
+    const double v = src[idx];
+    dst[idx] = v;
+    if (i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k))
+        dev::x_ghosts(L, V, dst, idx, i, v);
 }
 
 // One Jacobi sweep of the multigrid smoother and the fill after it
@@ -423,30 +454,37 @@ template <typename T, typename TB>
 __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int first, T ax, T ay, T az, T beta, T dsign, T w,
                             T *__restrict__ dst, const T *__restrict__ src, const TB *__restrict__ b) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
-        dev::bc_item(L, V, item_index(main_z), dst, [&](long long c) -> T {
+        auto val = [&](long long c) -> T {
             int ci, cj, ck;
             if (is_inner(L, c, ci, cj, ck)) return dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, c);
             return first ? src[c] : dst[c];
-        });
+        };
+        dev::bc_item<T, decltype(val), true>(L, V, item_index(main_z), dst, val);
         return;
     }
-    int i, j, k;
+    int i, j, k, bx, by, bz;
+    main_block(main_z, bx, by, bz);
     if (first) {
-        i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-        j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-        k = static_cast<int>(blockIdx.z);
+        i = bx * BX + static_cast<int>(threadIdx.x);
+        j = by * BY + static_cast<int>(threadIdx.y);
+        k = bz;
         if (i >= L.N[0] || j >= L.N[1]) return;
     } else {
-        i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-        j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-        k = 1 + static_cast<int>(blockIdx.z);
+        i = 1 + bx * BX + static_cast<int>(threadIdx.x);
+        j = 1 + by * BY + static_cast<int>(threadIdx.y);
+        k = 1 + bz;
         if (i > L.n[0] || j > L.n[1]) return;
     }
     const long long idx = lin(L, i, j, k);
     const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
                        !in_obstacle(L, i, j, k);
-    if (inner) dst[idx] = dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, idx);
-    else if (first && !dev::written(V, idx)) dst[idx] = src[idx];
+    if (inner) {
+        const T v = dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, idx);
+        dst[idx] = v;
+        dev::x_ghosts(L, V, dst, idx, i, v);
+    } else if (first && !dev::written(V, idx)) {
+        dst[idx] = src[idx];
+    }
 }
 
 // Residuum (VCycleMG.cpp:346-354) and the obstacle-face part of the fill
@@ -464,8 +502,12 @@ __global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, T *__restric
         });
         return;
     }
-    int i, j, k;
-    if (!inner_cell(L, i, j, k)) return;
+    int bx, by, bz;
+    main_block(main_z, bx, by, bz);
+    const int i = 1 + bx * BX + static_cast<int>(threadIdx.x);
+    const int j = 1 + by * BY + static_cast<int>(threadIdx.y);
+    const int k = 1 + bz;
+    if (i > L.n[0] || j > L.n[1] || in_obstacle(L, i, j, k)) return;
     const long long idx = lin(L, i, j, k);
     out[idx] = static_cast<T>(b[idx]) - dev::laplace_t<T>(L, x, idx);
 }
@@ -499,16 +541,22 @@ template <typename Tf, typename Tc>
 __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, Tf *__restrict__ dst,
                                     const Tf *__restrict__ fine, const Tc *__restrict__ coarse) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
-        dev::bc_item(Lf, V, item_index(main_z), dst,
-                     [&](long long c) { return corrected<Tf, Tc>(Lc, Lf, fine, coarse, c); });
+        auto val = [&](long long c) { return corrected<Tf, Tc>(Lc, Lf, fine, coarse, c); };
+        dev::bc_item<Tf, decltype(val), true>(Lf, V, item_index(main_z), dst, val);
         return;
     }
-    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    const int k = static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    main_block(main_z, bx, by, bz);
+    const int i = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = by * BY + static_cast<int>(threadIdx.y);
+    const int k = bz;
     if (i >= Lf.N[0] || j >= Lf.N[1]) return;
     const long long idx = lin(Lf, i, j, k);
-    if (!dev::written(V, idx)) dst[idx] = corrected<Tf, Tc>(Lc, Lf, fine, coarse, idx);
+    if (dev::written(V, idx)) return;
+    const Tf v = corrected<Tf, Tc>(Lc, Lf, fine, coarse, idx);
+    dst[idx] = v;
+    if (i >= 1 && j >= 1 && k >= 1 && i <= Lf.n[0] && j <= Lf.n[1] && k <= Lf.n[2] && !in_obstacle(Lf, i, j, k))
+        dev::x_ghosts(Lf, V, dst, idx, i, v);
 }
 
 // One idempotent fill in a single launch (faces, ghosts, dependent ghosts)
@@ -536,9 +584,11 @@ __global__ void k_reduce_sum(const double *__restrict__ partial, int n, double *
 __global__ void k_divergence(LevelDesc L, double hx, double hy, double hz, double *__restrict__ out,
                              const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z) {
-    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    const int k = static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    main_block(static_cast<int>(gridDim.z), bx, by, bz);
+    const int i = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = by * BY + static_cast<int>(threadIdx.y);
+    const int k = bz;
     if (i >= L.N[0] || j >= L.N[1]) return;
     const long long idx = lin(L, i, j, k);
     if (i == 0 || j == 0 || k == 0 || i == L.N[0] - 1 || j == L.N[1] - 1 || k == L.N[2] - 1) {

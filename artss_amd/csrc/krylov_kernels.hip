@@ -50,10 +50,25 @@ __device__ inline void write_partials(double (&v)[Q], double *partial) {
     }
 }
 
+// The hardware deals workgroups to the 8 XCDs round robin by linear id; the
+// bijection id -> (id % 8) * per + id / 8 on the first 8 * per ids gives each
+// XCD a contiguous run of blocks, so stencil neighbours along z are mostly in
+// that XCD's own L2.  Partial sums stay indexed by the linear id.
+__device__ inline void xcd_block(int &bx, int &by, int &bz) {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned per = gx * gy * gridDim.z / 8u;
+    const unsigned nid = id < per * 8u ? (id % 8u) * per + id / 8u : id;
+    bx = static_cast<int>(nid % gx);
+    by = static_cast<int>((nid / gx) % gy);
+    bz = static_cast<int>(nid / (gx * gy));
+}
 __device__ inline bool inner_at(const LevelDesc &L, int &i, int &j, int &k) {
-    i = 1 + static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    j = 1 + static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    k = 1 + static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    i = 1 + bx * BX + static_cast<int>(threadIdx.x);
+    j = 1 + by * BY + static_cast<int>(threadIdx.y);
+    k = 1 + bz;
     return i <= L.n[0] && j <= L.n[1] && !in_obstacle(L, i, j, k);
 }
 
@@ -137,9 +152,11 @@ template <typename Tz, typename Tw>
 __global__ void k_gcr_update(LevelDesc L, const double *__restrict__ coef, double *__restrict__ x,
                              double *__restrict__ r, const Tz *__restrict__ z, Tw *__restrict__ zp,
                              Tw *__restrict__ wp, double *__restrict__ partial) {
-    const int i = static_cast<int>(blockIdx.x) * BX + static_cast<int>(threadIdx.x);
-    const int j = static_cast<int>(blockIdx.y) * BY + static_cast<int>(threadIdx.y);
-    const int k = static_cast<int>(blockIdx.z);
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int i = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = by * BY + static_cast<int>(threadIdx.y);
+    const int k = bz;
     double v[1] = {0.0};
     if (i < L.N[0] && j < L.N[1]) {
         const long long idx = dev::lin(L, i, j, k);

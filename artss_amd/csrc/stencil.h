@@ -164,7 +164,26 @@ __device__ inline bool written(const BCView &V, long long c) { return (V.wbits[c
 // the value cell c has once the producing operator is done, for any cell c
 // the fill does not write; every item reads only such cells, so all items
 // are independent and the result equals the reference's sequential fill.
-template <typename T, typename Val>
+// The x-face ghosts whose chain starts on an inner fluid cell, 1 <= j <= n1
+// and 1 <= k <= n2 (one patch in the chain): with XGrid the thread of that
+// cell writes them (x_ghosts) and bc_item skips them, which turns the x faces'
+// strided item accesses into one store per boundary row.
+template <typename T>
+__device__ inline void x_ghosts(const LevelDesc &L, const BCView &V, T *dst, long long idx, int i, T v) {
+    if (V.n_ghosts == 0) return;
+    if (i == V.bc.target[0]) dst[idx - i] = static_cast<T>(V.bc.sign[0]) * v + static_cast<T>(V.bc.value[0]);
+    if (i == V.bc.target[1])
+        dst[idx - i + L.N[0] - 1] = static_cast<T>(V.bc.sign[1]) * v + static_cast<T>(V.bc.value[1]);
+}
+__device__ inline bool x_ghost_in_grid(const LevelDesc &L, long long i, long long j, long long k, long long s) {
+    if (!(i == 0 || i == L.N[0] - 1) || j < 1 || j > L.n[1] || k < 1 || k > L.n[2]) return false;
+    const long long Nx = L.N[0], Ny = L.N[1];
+    const int si = static_cast<int>(s % Nx);
+    return si >= 1 && si <= L.n[0] && !in_obstacle(L, si, static_cast<int>(j), static_cast<int>(k)) &&
+           s == si + Nx * (j + Ny * k);
+}
+
+template <typename T, typename Val, bool XGrid = false>
 __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t, T *dst, const Val &val) {
     if (t < V.n_faces) {
         const FaceEntry e = V.faces[t];
@@ -178,6 +197,7 @@ __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t,
     if (t < V.n_ghosts) {
         if (!ghost_coords(L, t, i, j, k)) return;
         const long long s = ghost_source(L, V.bc, i, j, k);
+        if (XGrid && x_ghost_in_grid(L, i, j, k, s)) return;  // written by the main grid
         if (written(V, s)) return;  // produced by a dependent item
         dst[i + static_cast<long long>(L.N[0]) * (j + static_cast<long long>(L.N[1]) * k)] =
             ghost_chain<T>(L, V.bc, i, j, k, val(s));

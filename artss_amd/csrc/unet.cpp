@@ -57,7 +57,10 @@ UNet::UNet(hipStream_t stream, unsigned long long seed) : m_s(stream) {
             L.W_bf = dalloc<uint16_t>(nw);
             L.Wt_bf = dalloc<uint16_t>(nw);
         }
-        const double std_ = std::sqrt(2.0 / (27.0 * s.ci));
+        // He initialisation; the output layer starts at zero, so an untrained
+        // network predicts no correction
+        const bool out_layer = m_layers.size() + 1 == sizeof(kLayers) / sizeof(kLayers[0]);
+        const double std_ = out_layer ? 0.0 : std::sqrt(2.0 / (27.0 * s.ci));
         host.resize(nw);
         for (auto &x : host) x = static_cast<float>(std_ * nd(rng));
         ARTSS_HIP(hipMemcpy(L.W, host.data(), sizeof(float) * nw, hipMemcpyHostToDevice));

@@ -77,6 +77,9 @@ def parse():
                     help="mixed: cap of the preconditioner's coarse Jacobi solve (0: the XML's max_iter)")
     ap.add_argument("--tail-cells", type=int, default=4096,
                     help="levels of at most this many cells run in one single-workgroup kernel")
+    ap.add_argument("--warm", choices=("none", "extrapolate"), default="extrapolate",
+                    help="start of each step's pressure iteration: the previous p (the reference's start) or "
+                         "p extrapolated linearly in time")
     ap.add_argument("--device", type=int, default=0)
     return ap.parse_args()
 
@@ -106,13 +109,33 @@ PATHS = {"vcycle": "FP64, V-cycles alone (the reference's iteration)",
          "mixed": "GCR in FP64 preconditioned by an FP32 V-cycle with FP64 coarse levels"}
 
 
-def make_solver(artss_amd, args, pressure):
+def make_solver(artss_amd, args, pressure, warm="none"):
     solver = artss_amd.Solver(args.config, device=args.device)
     solver.set_mg_schedule({"plain": 0, "fused": 1, "graph": 2}[args.mg_schedule], args.tail_cells)
     if not args.reference_schedule:
         solver.set_pressure_mode(True, args.tolerance, 500)
         solver.set_pressure_method({"vcycle": 0, "gcr": 1, "mixed": 2}[pressure], args.coarse_iterations)
+        if pressure != "vcycle" and warm == "extrapolate":
+            solver.set_warm_start(1)
     return solver
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 PMC summary in
+    profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a copy kernel of known size), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for name in sorted(os.listdir(d)):
+        if not name.startswith("pmc_"):
+            continue
+        with open(os.path.join(d, name)) as fh:
+            for line in fh:
+                line = line.strip().strip("`")
+                if line.startswith("traffic[%s] = " % kernel):
+                    return int(line.split("=")[1])
+    return None
 
 
 class Run:
@@ -147,7 +170,7 @@ def main():
 
     with open(args.config) as fh:
         dt = float(fh.read().split("<dt>")[1].split("</dt>")[0])
-    solver = make_solver(artss_amd, args, args.pressure)
+    solver = make_solver(artss_amd, args, args.pressure, args.warm)
     dom = solver.domain
     info = dom.info(0)
     inner = info["inner_cells"]
@@ -174,7 +197,8 @@ def main():
     mean_ms = kernel_ms / launches if launches else float("nan")
     achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9 if launches else None
     roofline = {"bound": "hbm", "kernel": args.profile_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                "traffic": measured_traffic(args.profile_kernel),
                 "bytes_per_launch": per_launch_bytes, "launches": launches, "mean_launch_us": mean_ms * 1e3,
                 "timing_steps": timing_steps, "timing_region_ms_per_step": t_secs * 1e3 / timing_steps,
                 "share_of_timing_region": kernel_ms / (t_secs * 1e3)}
@@ -213,6 +237,8 @@ def main():
         "data": "synthetic: plume from rest, timed steps %d..%d" % (first, first + args.steps - 1),
         "config": {"workload": "plume_256x128x128", "grid": list(info["n"]), "inner_cells": inner,
                    "path": PATHS[args.pressure], "mg_schedule": args.mg_schedule, "tail_cells": args.tail_cells,
+                   "warm_start": ("previous p" if args.warm == "none" or args.pressure == "vcycle" else
+                                  "p + (p - p_prev), linear extrapolation in time"),
                    "coarse_iterations": args.coarse_iterations,
                    "pressure": ("reference V-cycle schedule" if args.reference_schedule else
                                 "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance),

@@ -162,3 +162,26 @@ def test_learned_warm_start_reduces_iterations():
     r2 = np.mean([np.log(q["initial_residual"]) for q in res[2]])
     assert r2 < r0 - np.log(3.0)  # starts at least 3x closer on average
     assert it2 < it0
+
+
+def test_solver_close_releases_network_and_buffers():
+    """A network and buffers made on a solver's domain are released before the
+    solver (whose stream they use) is destroyed, whatever order Python drops
+    them in afterwards."""
+    s = _solver(_xml(2))
+    dom = s.domain
+    info = dom.info(0)
+    Nx, Ny, Nz = info["N"]
+    net = artss_amd.UNet(dom, seed=1)
+    buf = artss_amd.Buffer(dom, np.zeros((Nz - 2, Ny - 2, Nx - 2, 1), np.float32))
+    s.set_warm_start(2, net)
+    s.step(0.01)
+    s.close()
+    assert net.h is None and buf.ptr is None
+    del net, buf
+    s2 = _solver(_xml(2))
+    net2 = artss_amd.UNet(s2.domain, seed=1)
+    s2.set_warm_start(2, net2)
+    net2.close()  # detaches itself from the solver first
+    s2.step(0.01)
+    assert s2.counters()["residual"] <= TOL
