@@ -86,6 +86,10 @@ def lib():
         "artss_solver_set_pressure_method": (C.c_int, [vp, C.c_int, C.c_int]),
         "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
         "artss_solver_set_warm_start": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "artss_ensemble_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "artss_ensemble_destroy": (C.c_int, [vp]),
+        "artss_ensemble_add": (C.c_int, [vp, vp]),
+        "artss_ensemble_step": (C.c_int, [vp, C.c_long]),
         "artss_solver_warm_sample": (C.c_int, [vp, vp, vp, ip]),
         "artss_solver_initial_residual": (C.c_int, [vp, dp]),
         "artss_buffer_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
@@ -530,3 +534,34 @@ class UNet:
 
     def adam_step(self, lr):
         _check(self.dom.L.artss_unet_adam_step(self.h, lr))
+
+
+class Ensemble:
+    """Independent solvers of any sizes advanced together: step(n) runs every
+    member's step n (at its own n * dt) on `threads` host threads, each
+    member on its own stream, so their kernels overlap on the GPU."""
+
+    def __init__(self, members, threads=8):
+        self.L = lib()
+        self.members = list(members)
+        h = C.c_void_p()
+        _check(self.L.artss_ensemble_create(int(threads), C.byref(h)))
+        self.h = h
+        for m in self.members:
+            _check(self.L.artss_ensemble_add(self.h, m.h))
+
+    def step(self, n):
+        if any(not m.h for m in self.members):
+            raise ArtssError("an ensemble member was closed")
+        _check(self.L.artss_ensemble_step(self.h, int(n)))
+
+    def close(self):
+        if self.h:
+            _check(self.L.artss_ensemble_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

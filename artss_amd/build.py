@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libartss_hip.so")
 ARCH = os.environ.get("ARTSS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["kernels.hip", "mg_tail.hip", "krylov_kernels.hip", "krylov.cpp", "unet_kernels.hip", "unet.cpp",
-           "domain.cpp", "operators.cpp", "vcycle.cpp", "solver.cpp", "settings.cpp", "xml.cpp", "capi.cpp"]
+           "domain.cpp", "operators.cpp", "vcycle.cpp", "solver.cpp", "ensemble.cpp", "settings.cpp", "xml.cpp",
+           "capi.cpp"]
 
 # -ffp-contract=off keeps every FP64 expression in the reference's operation
 # order (no fused multiply-add), which the bit-exact parity tests rely on.
@@ -49,7 +50,7 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))

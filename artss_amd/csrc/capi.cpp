@@ -6,6 +6,7 @@
 #include <memory>
 #include <string>
 
+#include "ensemble.h"
 #include "solver.h"
 #include "unet.h"
 
@@ -30,6 +31,10 @@ struct artss_unet {
 struct artss_solver {
     std::unique_ptr<Solver> solver;
     artss_domain view;
+};
+
+struct artss_ensemble {
+    std::unique_ptr<Ensemble> ensemble;
 };
 
 namespace {
@@ -551,6 +556,40 @@ int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max
         m.method = method;
         m.coarse_max_iter = coarse_max_iter;
         s->solver->set_pressure_mode(m);
+        return ARTSS_OK;
+    });
+}
+
+int artss_ensemble_create(int threads, artss_ensemble **out) {
+    return guard([&] {
+        require(out && threads >= 1, "bad argument");
+        auto e = std::make_unique<artss_ensemble>();
+        e->ensemble = std::make_unique<Ensemble>(threads);
+        *out = e.release();
+        return ARTSS_OK;
+    });
+}
+
+int artss_ensemble_destroy(artss_ensemble *e) {
+    return guard([&] {
+        require(e, "null argument");
+        delete e;
+        return ARTSS_OK;
+    });
+}
+
+int artss_ensemble_add(artss_ensemble *e, artss_solver *member) {
+    return guard([&] {
+        require(e && member, "null argument");
+        e->ensemble->add(member->solver.get());
+        return ARTSS_OK;
+    });
+}
+
+int artss_ensemble_step(artss_ensemble *e, long step) {
+    return guard([&] {
+        require(e, "null argument");
+        e->ensemble->step(step);
         return ARTSS_OK;
     });
 }

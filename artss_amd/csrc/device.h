@@ -82,13 +82,16 @@ struct BCView {
 };
 
 #if defined(__HIPCC__)
+__host__ __device__ inline bool in_box(const Box &b, int i, int j, int k) {
+    return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2];
+}
+// The first two boxes are tested without a data-dependent loop so that their
+// bounds are fetched together, not one scalar load at a time.
 __host__ __device__ inline bool in_obstacle(const LevelDesc &L, int i, int j, int k) {
-    for (int o = 0; o < L.n_obst; o++) {
-        const Box &b = L.obst[o];
-        if (i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2])
-            return true;
-    }
-    return false;
+    const int n = L.n_obst;
+    bool r = (n > 0 && in_box(L.obst[0], i, j, k)) | (n > 1 && in_box(L.obst[1], i, j, k));
+    for (int o = 2; o < n; o++) r = r || in_box(L.obst[o], i, j, k);
+    return r;
 }
 #endif
 

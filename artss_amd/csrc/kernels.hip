@@ -442,7 +442,7 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict_
     const double v = src[idx];
     dst[idx] = v;
     if (i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k))
-        dev::x_ghosts(L, V, dst, idx, i, v);
+        dev::face_ghosts(L, V, dst, idx, i, j, k, v);
 }
 
 // One Jacobi sweep of the multigrid smoother and the fill after it
@@ -476,12 +476,14 @@ __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int first, T ax, 
         if (i > L.n[0] || j > L.n[1]) return;
     }
     const long long idx = lin(L, i, j, k);
-    const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
-                       !in_obstacle(L, i, j, k);
-    if (inner) {
-        const T v = dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, idx);
+    // the update is computed for every cell of the inner box before the
+    // obstacle test, so the stencil loads are in flight while it runs
+    const bool in_inner_box = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2];
+    T v = T(0);
+    if (in_inner_box) v = dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, idx);
+    if (in_inner_box && !in_obstacle(L, i, j, k)) {
         dst[idx] = v;
-        dev::x_ghosts(L, V, dst, idx, i, v);
+        dev::face_ghosts(L, V, dst, idx, i, j, k, v);
     } else if (first && !dev::written(V, idx)) {
         dst[idx] = src[idx];
     }
@@ -556,7 +558,7 @@ __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int ma
     const Tf v = corrected<Tf, Tc>(Lc, Lf, fine, coarse, idx);
     dst[idx] = v;
     if (i >= 1 && j >= 1 && k >= 1 && i <= Lf.n[0] && j <= Lf.n[1] && k <= Lf.n[2] && !in_obstacle(Lf, i, j, k))
-        dev::x_ghosts(Lf, V, dst, idx, i, v);
+        dev::face_ghosts(Lf, V, dst, idx, i, j, k, v);
 }
 
 // One idempotent fill in a single launch (faces, ghosts, dependent ghosts)

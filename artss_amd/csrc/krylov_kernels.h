@@ -39,7 +39,7 @@ namespace kk {
 // *sum = sum of d^2 over the fluid cells
 void warm_sum(hipStream_t s, const LevelDesc &L, const double *p, const double *p_prev, double *partial,
               double *sum);
-// network input [v][2]: c (b - lap p) / s and d / s, zero off the fluid
+// network input [v][2]: c (b - lap(p + d)) / s and d / s, zero off the fluid
 void warm_pack(hipStream_t s, const LevelDesc &L, const double *p, const double *b, const double *p_prev,
                const double *sum, double fluid_cells, double c, float *x);
 // p += d (+ s y[v] when y is given) on the fluid cells; nothing when d == 0

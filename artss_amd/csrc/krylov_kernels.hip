@@ -218,7 +218,10 @@ __global__ void k_warm_pack(LevelDesc L, const double *__restrict__ p, const dou
     if (!in_obstacle(L, i, j, k)) {
         const long long idx = dev::lin(L, i, j, k);
         const double q = inv_rms(sum, n);
-        r = static_cast<float>(c * (b[idx] - dev::laplace(L, p, idx)) * q);
+        // residual of the extrapolated start 2 p - p_prev (the fill is affine,
+        // so the filled arrays combine to the filled extrapolation)
+        const double lap_e = 2.0 * dev::laplace(L, p, idx) - dev::laplace(L, p_prev, idx);
+        r = static_cast<float>(c * (b[idx] - lap_e) * q);
         d = static_cast<float>((p[idx] - p_prev[idx]) * q);
     }
     const long long v = box_index(L, i, j, k);

@@ -12,6 +12,8 @@
 #include "mg_tail.h"
 #include "stencil.h"
 
+#include <mutex>
+
 namespace artss {
 namespace {
 
@@ -382,10 +384,17 @@ int mg_tail_lds_limit() {
     return bytes > static_bytes ? bytes - static_bytes : 0;
 }
 
+// The attribute is per kernel function, shared by every solver in the
+// process: it is only ever raised, so a smaller plan set up later does not
+// invalidate the launches of a larger one.
 void mg_tail_reserve_lds(int lds_bytes) {
-    if (lds_bytes > 0)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mg_tail),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    static std::mutex lock;
+    static int reserved = 0;
+    std::lock_guard<std::mutex> guard(lock);
+    if (lds_bytes > reserved &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mg_tail), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds_bytes) == hipSuccess)
+        reserved = lds_bytes;
 }
 
 void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter) {

@@ -59,8 +59,9 @@ struct PressureMode {
 //   0  p_n, the start the reference's iteration uses
 //   1  p_n + d with d = p_n - p_prev (linear extrapolation in time)
 //   2  p_n + d + s * net(c r / s, d / s), the warm-start U-Net's correction
-//      to the extrapolation, r = rhs - lap p_n, s = rms(d) over the fluid
-//      cells, c = 1 / (2/dx^2 + 2/dy^2 + 2/dz^2)
+//      to the extrapolation, r = rhs - lap(p_n + d) the residual of the
+//      extrapolated start, s = rms(d) over the fluid cells,
+//      c = 1 / (2/dx^2 + 2/dy^2 + 2/dz^2)
 // The iteration then runs to the same residual target from any start.
 // `record` keeps the last step's network input and target, (p_{n+1} - p_n -
 // d) / s, for training.

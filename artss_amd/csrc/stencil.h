@@ -111,21 +111,23 @@ __device__ __host__ inline long long ghost_count(const LevelDesc &L) {
 
 // Ghost cell number t (0 <= t < ghost_count) of a level: its coordinates
 // (the x faces first, then y without x edges, then z without x and y edges)
-__device__ inline bool ghost_coords(const LevelDesc &L, long long t, long long &i, long long &j, long long &k) {
-    const long long Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
-    const long long fx = Ny * Nz, fy = (Nx - 2) * Nz, fz = (Nx - 2) * (Ny - 2);
+// (32-bit arithmetic: a level's ghost count stays below 2^31)
+__device__ inline bool ghost_coords(const LevelDesc &L, long long t64, long long &i, long long &j, long long &k) {
+    const int Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
+    const int fx = Ny * Nz, fy = (Nx - 2) * Nz, fz = (Nx - 2) * (Ny - 2);
+    int t = static_cast<int>(t64);
     if (t < 2 * fx) {
-        const long long side = t / fx, r = t % fx;
+        const int side = t >= fx, r = side ? t - fx : t;
         i = side ? Nx - 1 : 0;
         j = r % Ny;
         k = r / Ny;
     } else if ((t -= 2 * fx) < 2 * fy) {
-        const long long side = t / fy, r = t % fy;
+        const int side = t >= fy, r = side ? t - fy : t;
         j = side ? Ny - 1 : 0;
         i = 1 + r % (Nx - 2);
         k = r / (Nx - 2);
     } else if ((t -= 2 * fy) < 2 * fz) {
-        const long long side = t / fz, r = t % fz;
+        const int side = t >= fz, r = side ? t - fz : t;
         k = side ? Nz - 1 : 0;
         i = 1 + r % (Nx - 2);
         j = 1 + r / (Nx - 2);
@@ -164,23 +166,34 @@ __device__ inline bool written(const BCView &V, long long c) { return (V.wbits[c
 // the value cell c has once the producing operator is done, for any cell c
 // the fill does not write; every item reads only such cells, so all items
 // are independent and the result equals the reference's sequential fill.
-// The x-face ghosts whose chain starts on an inner fluid cell, 1 <= j <= n1
-// and 1 <= k <= n2 (one patch in the chain): with XGrid the thread of that
-// cell writes them (x_ghosts) and bc_item skips them, which turns the x faces'
-// strided item accesses into one store per boundary row.
+// Face ghosts (exactly one coordinate on the ghost shell, so one patch in
+// the chain) whose chain starts on an inner fluid cell: with XGrid the
+// thread of that cell writes them (face_ghosts) and bc_item skips them, so a
+// sweep's fill leaves only edges, corners and obstacle faces as items.
 template <typename T>
-__device__ inline void x_ghosts(const LevelDesc &L, const BCView &V, T *dst, long long idx, int i, T v) {
+__device__ inline void face_ghosts(const LevelDesc &L, const BCView &V, T *dst, long long idx, int i, int j, int k,
+                                   T v) {
     if (V.n_ghosts == 0) return;
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
     if (i == V.bc.target[0]) dst[idx - i] = static_cast<T>(V.bc.sign[0]) * v + static_cast<T>(V.bc.value[0]);
     if (i == V.bc.target[1])
         dst[idx - i + L.N[0] - 1] = static_cast<T>(V.bc.sign[1]) * v + static_cast<T>(V.bc.value[1]);
+    if (j == V.bc.target[2]) dst[idx - j * nj] = static_cast<T>(V.bc.sign[2]) * v + static_cast<T>(V.bc.value[2]);
+    if (j == V.bc.target[3])
+        dst[idx + (L.N[1] - 1 - j) * nj] = static_cast<T>(V.bc.sign[3]) * v + static_cast<T>(V.bc.value[3]);
+    if (k == V.bc.target[4]) dst[idx - k * nk] = static_cast<T>(V.bc.sign[4]) * v + static_cast<T>(V.bc.value[4]);
+    if (k == V.bc.target[5])
+        dst[idx + (L.N[2] - 1 - k) * nk] = static_cast<T>(V.bc.sign[5]) * v + static_cast<T>(V.bc.value[5]);
 }
-__device__ inline bool x_ghost_in_grid(const LevelDesc &L, long long i, long long j, long long k, long long s) {
-    if (!(i == 0 || i == L.N[0] - 1) || j < 1 || j > L.n[1] || k < 1 || k > L.n[2]) return false;
-    const long long Nx = L.N[0], Ny = L.N[1];
-    const int si = static_cast<int>(s % Nx);
-    return si >= 1 && si <= L.n[0] && !in_obstacle(L, si, static_cast<int>(j), static_cast<int>(k)) &&
-           s == si + Nx * (j + Ny * k);
+__device__ inline bool face_ghost_in_grid(const LevelDesc &L, const DomainBC &bc, long long i, long long j,
+                                          long long k) {
+    const bool ox = i == 0 || i == L.N[0] - 1, oy = j == 0 || j == L.N[1] - 1, oz = k == 0 || k == L.N[2] - 1;
+    if (static_cast<int>(ox) + static_cast<int>(oy) + static_cast<int>(oz) != 1) return false;
+    const int si = ox ? bc.target[i == 0 ? 0 : 1] : static_cast<int>(i);
+    const int sj = oy ? bc.target[j == 0 ? 2 : 3] : static_cast<int>(j);
+    const int sk = oz ? bc.target[k == 0 ? 4 : 5] : static_cast<int>(k);
+    return si >= 1 && si <= L.n[0] && sj >= 1 && sj <= L.n[1] && sk >= 1 && sk <= L.n[2] &&
+           !in_obstacle(L, si, sj, sk);
 }
 
 template <typename T, typename Val, bool XGrid = false>
@@ -197,7 +210,7 @@ __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t,
     if (t < V.n_ghosts) {
         if (!ghost_coords(L, t, i, j, k)) return;
         const long long s = ghost_source(L, V.bc, i, j, k);
-        if (XGrid && x_ghost_in_grid(L, i, j, k, s)) return;  // written by the main grid
+        if (XGrid && face_ghost_in_grid(L, V.bc, i, j, k)) return;  // written by the main grid
         if (written(V, s)) return;  // produced by a dependent item
         dst[i + static_cast<long long>(L.N[0]) * (j + static_cast<long long>(L.N[1]) * k)] =
             ghost_chain<T>(L, V.bc, i, j, k, val(s));

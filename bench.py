@@ -80,6 +80,10 @@ def parse():
     ap.add_argument("--warm", choices=("none", "extrapolate"), default="extrapolate",
                     help="start of each step's pressure iteration: the previous p (the reference's start) or "
                          "p extrapolated linearly in time")
+    ap.add_argument("--ensemble-members", type=int, default=16,
+                    help="members of the ensemble measurement (uneven plume boxes; 0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=5, help="timed steps of the ensemble measurement")
+    ap.add_argument("--ensemble-threads", type=int, default=8, help="host threads issuing the members' steps")
     ap.add_argument("--device", type=int, default=0)
     return ap.parse_args()
 
@@ -136,6 +140,44 @@ def measured_traffic(kernel):
                 if line.startswith("traffic[%s] = " % kernel):
                     return int(line.split("=")[1])
     return None
+
+
+def ensemble_run(artss_amd, args):
+    """Members of uneven size (64..160 x 64..96 x 32..64 cells, 0-3 obstacles,
+    20-80 kW/m^3), stepped concurrently and, for comparison, one after
+    another, from the same state."""
+    from artss_amd import scenarios
+    xmls = scenarios.ensemble_members(args.ensemble_members, seed=2026)
+    out = {"members": args.ensemble_members, "threads": args.ensemble_threads, "steps": args.ensemble_steps}
+    for label, threads in (("concurrent", args.ensemble_threads), ("sequential", 1)):
+        members = []
+        for x in xmls:
+            m = artss_amd.Solver(x, device=args.device)
+            m.set_mg_schedule(2, args.tail_cells)
+            m.set_pressure_mode(True, args.tolerance, 500)
+            m.set_pressure_method(2)
+            m.set_warm_start(1)
+            members.append(m)
+        cells = sum(m.domain.info(0)["inner_cells"] for m in members)
+        ens = artss_amd.Ensemble(members, threads=threads)
+        for n in range(1, 4):
+            ens.step(n)
+        for m in members:
+            m.domain.sync()
+        t0 = time.perf_counter()
+        for n in range(4, 4 + args.ensemble_steps):
+            ens.step(n)
+        for m in members:
+            m.domain.sync()
+        secs = time.perf_counter() - t0
+        out["cells"] = cells
+        out[label] = {"value": cells * args.ensemble_steps / secs, "unit": "cell-updates/s",
+                      "ms_per_step": secs * 1e3 / args.ensemble_steps}
+        ens.close()
+        for m in members:
+            m.close()
+    out["concurrency_gain"] = out["concurrent"]["value"] / out["sequential"]["value"]
+    return out
 
 
 class Run:
@@ -229,6 +271,8 @@ def main():
                  "speedup_of_value": value / (inner * k_plain / p_secs),
                  "max_rel_field_error_at_step_%d" % after: errors}
 
+    ensemble = ensemble_run(artss_amd, args) if args.ensemble_members > 0 else None
+
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True,
@@ -244,7 +288,7 @@ def main():
                                 "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance),
                    "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(
                        diff_it, axis=0).tolist(), "max_final_residual": float(np.max(residuals))},
-        "plain_fp64": plain,
+        "plain_fp64": plain, "ensemble": ensemble,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

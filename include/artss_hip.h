@@ -40,6 +40,7 @@ typedef struct artss_domain artss_domain;
 typedef struct artss_mg artss_mg;
 typedef struct artss_solver artss_solver;
 typedef struct artss_unet artss_unet;
+typedef struct artss_ensemble artss_ensemble;
 
 typedef struct {
     int N[3];               /* cells per axis including ghosts (DomainData::get_number_of_cells) */
@@ -163,7 +164,7 @@ int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cell
  * reference counterpart: the reference's iteration starts from the previous
  * step's p, mode 0).  With p_n the previous solution, p_prev the one before
  * and d = p_n - p_prev: mode 1 starts from p_n + d, mode 2 from p_n + d + s *
- * net(c (rhs - lap p_n) / s, d / s) with s = rms(d) over the fluid cells and
+ * net(c (rhs - lap(p_n + d)) / s, d / s) with s = rms(d) over the fluid cells and
  * c = 1 / (2/dx^2 + 2/dy^2 + 2/dz^2), `net` created on the solver's domain.
  * The iteration runs to the same residual target from any start.  `record`
  * keeps each step's network input and its target (p_{n+1} - p_n - d) / s.
@@ -174,6 +175,18 @@ int artss_solver_set_warm_start(artss_solver *s, int mode, artss_unet *net, int 
 int artss_solver_warm_sample(artss_solver *s, float *x, float *t, int *available);
 /* ||rhs - lap p||_2 at the start of the last step's GCR pressure iteration */
 int artss_solver_initial_residual(artss_solver *s, double *residual);
+
+/* ---- ensembles (no reference counterpart: the reference advances one
+ * domain per process).  Independent solvers of any sizes, each on its own
+ * stream, advanced together: artss_ensemble_step runs Solver::step(step *
+ * dt_m) for every member m on `threads` host threads, so the members'
+ * kernels overlap on the GPU.  Members are not owned; they must outlive the
+ * ensemble and not be stepped elsewhere during artss_ensemble_step.  An
+ * error in any member is reported after all members are done. */
+int artss_ensemble_create(int threads, artss_ensemble **out);
+int artss_ensemble_destroy(artss_ensemble *e);
+int artss_ensemble_add(artss_ensemble *e, artss_solver *member);
+int artss_ensemble_step(artss_ensemble *e, long step);
 
 /* ---- raw device buffers on a domain's device (any element type) ---- */
 int artss_buffer_alloc(artss_domain *d, size_t bytes, void **dev);
