@@ -79,6 +79,11 @@ struct BCView {
     const uint32_t *wbits;  // bit c set: cell c is written by the fill
     DomainBC bc;
     long long n_ghosts;     // 0 when the field type has no domain boundary data
+    // ghost cells a sweep kernel's main grid does not write itself (edges,
+    // corners, face ghosts whose chain starts off the inner fluid), without
+    // the dependent ones
+    const uint32_t *rest;
+    long long n_rest;
 };
 
 #if defined(__HIPCC__)

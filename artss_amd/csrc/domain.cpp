@@ -137,6 +137,7 @@ Domain::~Domain() {
     (void)hipFree(m_params_hom_dev);
     (void)hipFree(m_bits_dev);
     (void)hipFree(m_deps_dev);
+    (void)hipFree(m_rest_dev);
     (void)hipStreamDestroy(m_stream);
 }
 
@@ -306,11 +307,14 @@ void Domain::build_boundary_tables() {
     std::vector<FaceParam> params;
     std::vector<uint32_t> bits;
     std::vector<DepGhost> deps;
+    std::vector<uint32_t> rest;
     m_idem.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 0));
     m_face_values_zero.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 1));
     m_view_bits.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
     m_view_deps.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
     m_view_ndeps.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
+    m_view_rest.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
+    m_view_nrest.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
 
     for (int l = 0; l <= m_levels; l++) {
         const LevelDesc &L = m_level[l];
@@ -420,6 +424,7 @@ void Domain::build_boundary_tables() {
             //    faces and ghosts are independent.
             std::vector<char> written(static_cast<size_t>(L.size), 0);
             std::vector<long long> ghost_src;  // per ghost (k_bc_domain order) the cell its chain reads
+            std::vector<long long> ghost_cell; // and the ghost cell itself
             {
                 for (const auto &w : ws)
                     if (w.live) written[w.cell] = 2;  // 2: face-written
@@ -428,6 +433,7 @@ void Domain::build_boundary_tables() {
                     const long long fx = Ny * Nz, fy = (Nx - 2) * Nz, fz = (Nx - 2) * (Ny - 2);
                     const long long n_ghost = 2 * fx + 2 * fy + 2 * fz;
                     ghost_src.resize(static_cast<size_t>(n_ghost));
+                    ghost_cell.resize(static_cast<size_t>(n_ghost));
                     for (long long t0 = 0; t0 < n_ghost; t0++) {
                         long long t = t0, i, j, k;
                         if (t < 2 * fx) {
@@ -449,6 +455,7 @@ void Domain::build_boundary_tables() {
                         const long long jj = (j == 0 || j == Ny - 1) ? d.target[j == 0 ? 2 : 3] : j;
                         const long long kk = (k == 0 || k == Nz - 1) ? d.target[k == 0 ? 4 : 5] : k;
                         ghost_src[static_cast<size_t>(t0)] = ii + Nx * (jj + Ny * kk);
+                        ghost_cell[static_cast<size_t>(t0)] = i + Nx * (j + Ny * k);
                     }
                 }
                 bool idem = true;
@@ -512,6 +519,26 @@ void Domain::build_boundary_tables() {
                     }
                 }
                 m_view_ndeps[l][f] = static_cast<long long>(deps.size()) - m_view_deps[l][f];
+                // ghosts the sweep kernels' main grid does not write
+                // (stencil.h face_ghosts): edges, corners and face ghosts
+                // whose chain starts on a cell that is not inner fluid; the
+                // ghosts reading a face cell are the dependent ones above
+                m_view_rest[l][f] = static_cast<long long>(rest.size());
+                const long long Nz = L.N[2];
+                for (size_t t0 = 0; t0 < ghost_src.size(); t0++) {
+                    const long long src = ghost_src[t0], g = ghost_cell[t0];
+                    if (written[static_cast<size_t>(src)]) continue;
+                    const long long i = g % Nx, j = (g / Nx) % Ny, k = g / (Nx * Ny);
+                    const int on = (i == 0 || i == Nx - 1) + (j == 0 || j == Ny - 1) + (k == 0 || k == Nz - 1);
+                    const int si = static_cast<int>(src % Nx), sj = static_cast<int>((src / Nx) % Ny),
+                              sk = static_cast<int>(src / (Nx * Ny));
+                    const bool fluid = si >= 1 && si <= L.n[0] && sj >= 1 && sj <= L.n[1] && sk >= 1 &&
+                                       sk <= L.n[2] && !in_obstacle(L, si, sj, sk);
+                    if (on == 1 && fluid) continue;
+                    if (g > 0xffffffffLL) throw config_error("grid too large for 32-bit ghost indices");
+                    rest.push_back(static_cast<uint32_t>(g));
+                }
+                m_view_nrest[l][f] = static_cast<long long>(rest.size()) - m_view_rest[l][f];
             }
         }
     }
@@ -538,6 +565,10 @@ void Domain::build_boundary_tables() {
         ARTSS_HIP(hipMalloc(&m_deps_dev, sizeof(DepGhost) * deps.size()));
         ARTSS_HIP(hipMemcpy(m_deps_dev, deps.data(), sizeof(DepGhost) * deps.size(), hipMemcpyHostToDevice));
     }
+    if (!rest.empty()) {
+        ARTSS_HIP(hipMalloc(&m_rest_dev, sizeof(uint32_t) * rest.size()));
+        ARTSS_HIP(hipMemcpy(m_rest_dev, rest.data(), sizeof(uint32_t) * rest.size(), hipMemcpyHostToDevice));
+    }
 }
 
 BCView Domain::bc_view(int level, int ft, bool homogeneous) const {
@@ -550,6 +581,8 @@ BCView Domain::bc_view(int level, int ft, bool homogeneous) const {
     v.params = params ? params + m_param_offset[level][ft] : nullptr;
     v.deps = m_deps_dev ? m_deps_dev + m_view_deps[level][ft] : nullptr;
     v.n_deps = m_view_ndeps[level][ft];
+    v.rest = m_rest_dev ? m_rest_dev + m_view_rest[level][ft] : nullptr;
+    v.n_rest = m_view_nrest[level][ft];
     v.wbits = m_bits_dev + m_view_bits[level][ft];
     v.bc = m_dom_bc[level][ft];
     if (homogeneous)

@@ -156,7 +156,8 @@ class Domain {
     std::vector<std::vector<char>> m_idem;               // [level][field type]
     std::vector<std::vector<char>> m_face_values_zero;   // [level][field type]
     FaceParam *m_params_hom_dev = nullptr;
-    std::vector<std::vector<long long>> m_view_bits, m_view_deps, m_view_ndeps;
+    std::vector<std::vector<long long>> m_view_bits, m_view_deps, m_view_ndeps, m_view_rest, m_view_nrest;
+    uint32_t *m_rest_dev = nullptr;
     uint32_t *m_bits_dev = nullptr;
     DepGhost *m_deps_dev = nullptr;
     long long m_face_entries = 0;

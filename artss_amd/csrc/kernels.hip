@@ -758,13 +758,13 @@ void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, dou
 
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src) {
     const dim3 g = full_grid(L);
-    k_copy_bc<<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), dst, src);
+    k_copy_bc<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), dst, src);
 }
 template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
                double ax, double ay, double az, double beta, double dsign, double w) {
     const dim3 g = first ? full_grid(L) : inner_grid(L);
-    k_jacobi_bc<T, TB><<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(
+    k_jacobi_bc<T, TB><<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(
         L, V, static_cast<int>(g.z), first ? 1 : 0, static_cast<T>(ax), static_cast<T>(ay), static_cast<T>(az),
         static_cast<T>(beta), static_cast<T>(dsign), static_cast<T>(w), dst, src, b);
 }
@@ -792,7 +792,7 @@ template <typename Tf, typename Tc>
 void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, Tf *dst,
                        const Tf *fine, const Tc *coarse) {
     const dim3 g = full_grid(Lf);
-    k_prolongate_add_bc<Tf, Tc><<<with_items(g, dev::bc_items(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z),
+    k_prolongate_add_bc<Tf, Tc><<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z),
                                                                                      dst, fine, coarse);
 }
 template void prolongate_add_bc<double, double>(hipStream_t, const LevelDesc &, const LevelDesc &, const BCView &,

@@ -168,8 +168,9 @@ __device__ inline bool written(const BCView &V, long long c) { return (V.wbits[c
 // are independent and the result equals the reference's sequential fill.
 // Face ghosts (exactly one coordinate on the ghost shell, so one patch in
 // the chain) whose chain starts on an inner fluid cell: with XGrid the
-// thread of that cell writes them (face_ghosts) and bc_item skips them, so a
-// sweep's fill leaves only edges, corners and obstacle faces as items.
+// thread of that cell writes them (face_ghosts) and the XGrid form of
+// bc_item takes the remaining ghosts from the view's rest list, so a sweep's
+// fill leaves only edges, corners, obstacle faces and their shadows as items.
 template <typename T>
 __device__ inline void face_ghosts(const LevelDesc &L, const BCView &V, T *dst, long long idx, int i, int j, int k,
                                    T v) {
@@ -185,17 +186,6 @@ __device__ inline void face_ghosts(const LevelDesc &L, const BCView &V, T *dst, 
     if (k == V.bc.target[5])
         dst[idx + (L.N[2] - 1 - k) * nk] = static_cast<T>(V.bc.sign[5]) * v + static_cast<T>(V.bc.value[5]);
 }
-__device__ inline bool face_ghost_in_grid(const LevelDesc &L, const DomainBC &bc, long long i, long long j,
-                                          long long k) {
-    const bool ox = i == 0 || i == L.N[0] - 1, oy = j == 0 || j == L.N[1] - 1, oz = k == 0 || k == L.N[2] - 1;
-    if (static_cast<int>(ox) + static_cast<int>(oy) + static_cast<int>(oz) != 1) return false;
-    const int si = ox ? bc.target[i == 0 ? 0 : 1] : static_cast<int>(i);
-    const int sj = oy ? bc.target[j == 0 ? 2 : 3] : static_cast<int>(j);
-    const int sk = oz ? bc.target[k == 0 ? 4 : 5] : static_cast<int>(k);
-    return si >= 1 && si <= L.n[0] && sj >= 1 && sj <= L.n[1] && sk >= 1 && sk <= L.n[2] &&
-           !in_obstacle(L, si, sj, sk);
-}
-
 template <typename T, typename Val, bool XGrid = false>
 __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t, T *dst, const Val &val) {
     if (t < V.n_faces) {
@@ -207,16 +197,28 @@ __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t,
     }
     t -= V.n_faces;
     long long i, j, k;
-    if (t < V.n_ghosts) {
-        if (!ghost_coords(L, t, i, j, k)) return;
-        const long long s = ghost_source(L, V.bc, i, j, k);
-        if (XGrid && face_ghost_in_grid(L, V.bc, i, j, k)) return;  // written by the main grid
-        if (written(V, s)) return;  // produced by a dependent item
-        dst[i + static_cast<long long>(L.N[0]) * (j + static_cast<long long>(L.N[1]) * k)] =
-            ghost_chain<T>(L, V.bc, i, j, k, val(s));
-        return;
+    if (XGrid) {
+        // the main grid writes the face ghosts of inner fluid cells
+        if (t < V.n_rest) {
+            const unsigned g = V.rest[t], Nx = static_cast<unsigned>(L.N[0]), Ny = static_cast<unsigned>(L.N[1]);
+            i = g % Nx;
+            j = (g / Nx) % Ny;
+            k = g / (Nx * Ny);
+            dst[g] = ghost_chain<T>(L, V.bc, i, j, k, val(ghost_source(L, V.bc, i, j, k)));
+            return;
+        }
+        t -= V.n_rest;
+    } else {
+        if (t < V.n_ghosts) {
+            if (!ghost_coords(L, t, i, j, k)) return;
+            const long long s = ghost_source(L, V.bc, i, j, k);
+            if (written(V, s)) return;  // produced by a dependent item
+            dst[i + static_cast<long long>(L.N[0]) * (j + static_cast<long long>(L.N[1]) * k)] =
+                ghost_chain<T>(L, V.bc, i, j, k, val(s));
+            return;
+        }
+        t -= V.n_ghosts;
     }
-    t -= V.n_ghosts;
     if (t < V.n_deps) {
         const DepGhost d = V.deps[t];
         const FaceEntry e = V.faces[d.face];
@@ -230,6 +232,8 @@ __device__ inline void bc_item(const LevelDesc &L, const BCView &V, long long t,
     }
 }
 __device__ __host__ inline long long bc_items(const BCView &V) { return V.n_faces + V.n_ghosts + V.n_deps; }
+// items of the XGrid form (the main grid writes the other face ghosts)
+__device__ __host__ inline long long bc_items_grid(const BCView &V) { return V.n_faces + V.n_rest + V.n_deps; }
 
 // DomainBoundary.cpp:25-49 for all six patches at once, ghost cell number t
 // (0 <= t < ghost_count).  The final value of a ghost cell is the composition
