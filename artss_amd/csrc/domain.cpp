@@ -168,6 +168,8 @@ void Domain::build_levels() {
         }
         if (L.N[0] != L.n[0] + 2 || L.N[1] != L.n[1] + 2 || L.N[2] != L.n[2] + 2)
             throw config_error("inconsistent level sizes");
+        // kernels decode cell indices and fill items in 32-bit arithmetic
+        if (L.size >= (1LL << 31)) throw config_error("grid too large: a level needs fewer than 2^31 cells");
     }
 }
 

@@ -39,6 +39,9 @@ void sl_advect(hipStream_t s, const LevelDesc &L, double dt, double *out, const 
 void jacobi_diffuse_step(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b,
                          double ax, double ay, double az, double rb, double D, const double *ev, double dt,
                          double w, double *partial);
+// ExplicitDiffuse.cpp:49-121: one explicit Euler step on inner cells (ev == nullptr: plain form)
+void explicit_diffuse(hipStream_t s, const LevelDesc &L, double *out, const double *in, double D, const double *ev,
+                      double dt);
 // multigrid form, multiplies by beta (:292-300)
 void jacobi_step_mg(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b, double ax,
                     double ay, double az, double beta, double dsign, double w);

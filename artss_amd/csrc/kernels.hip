@@ -272,6 +272,38 @@ __global__ void k_jacobi_mg_first(LevelDesc L, double ax, double ay, double az, 
     dst[idx] = inner ? dev::jacobi_mg(L, ax, ay, az, beta, dsign, w, src, b, idx) : src[idx];
 }
 
+// ------------------------------------------------------------------ explicit diffusion
+// ExplicitDiffuse.cpp:49-78 (ev == nullptr) and :81-121: one explicit Euler
+// step on inner cells; the turbulent form has no 1/h^2 and subtracts the
+// upper flux from the lower one, both as the reference writes them.
+__global__ void k_explicit_diffuse(LevelDesc L, double rdx, double rdy, double rdz, double D, double dt,
+                                   double *__restrict__ out, const double *__restrict__ in,
+                                   const double *__restrict__ ev) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long ni = 1, nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    if (ev == nullptr) {
+        out[idx] = in[idx] + dt * ((in[idx + ni] - 2 * in[idx] + in[idx - ni]) * rdx +
+                                   (in[idx + nj] - 2 * in[idx] + in[idx - nj]) * rdy +
+                                   (in[idx + nk] - 2 * in[idx] + in[idx - nk]) * rdz);
+        return;
+    }
+    const double nu_x1 = 0.5 * (ev[idx + ni] + ev[idx]) + D;
+    const double nu_x2 = 0.5 * (ev[idx - ni] + ev[idx]) + D;
+    const double nu_y1 = 0.5 * (ev[idx + nj] + ev[idx]) + D;
+    const double nu_y2 = 0.5 * (ev[idx - nj] + ev[idx]) + D;
+    const double nu_z1 = 0.5 * (ev[idx + nk] + ev[idx]) + D;
+    const double nu_z2 = 0.5 * (ev[idx - nk] + ev[idx]) + D;
+    const double di0 = (in[idx] - in[idx - ni]);
+    const double di1 = (in[idx + ni] - in[idx]);
+    const double dj0 = (in[idx] - in[idx - nj]);
+    const double dj1 = (in[idx + nj] - in[idx]);
+    const double dk0 = (in[idx] - in[idx - nk]);
+    const double dk1 = (in[idx + nk] - in[idx]);
+    out[idx] = in[idx] + dt * ((nu_x1 * di0 - nu_x2 * di1) + (nu_y1 * dj0 - nu_y2 * dj1) + (nu_z1 * dk0 - nu_z2 * dk1));
+}
+
 // ------------------------------------------------------------------ colored Gauss-Seidel
 // One color of ColoredGaussSeidelDiffuse::colored_gauss_seidel_step
 // (:212-222, :270-294) with the stencil of :316-341.  Cells of one color only
@@ -415,11 +447,12 @@ __device__ inline long long item_index(int main_z) {
     return ((static_cast<long long>(blockIdx.z - main_z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) *
                (BX * BY) + threadIdx.y * BX + threadIdx.x;
 }
+// (32-bit arithmetic: a level has fewer than 2^32 cells)
 __device__ inline bool is_inner(const LevelDesc &L, long long c, int &i, int &j, int &k) {
-    const long long Nx = L.N[0], Ny = L.N[1];
-    k = static_cast<int>(c / (Nx * Ny));
-    j = static_cast<int>((c / Nx) % Ny);
-    i = static_cast<int>(c % Nx);
+    const unsigned u = static_cast<unsigned>(c), Nx = static_cast<unsigned>(L.N[0]), Ny = static_cast<unsigned>(L.N[1]);
+    k = static_cast<int>(u / (Nx * Ny));
+    j = static_cast<int>((u / Nx) % Ny);
+    i = static_cast<int>(u % Nx);
     return i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k);
 }
 
@@ -450,41 +483,43 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict_
 // initial copy is folded in, every cell of dst is written (inner cells with
 // the update of src, filled cells by the fill, all others with src);
 // first = 0: cells that are neither inner nor filled keep dst's own values.
-template <typename T, typename TB>
-__global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int first, T ax, T ay, T az, T beta, T dsign, T w,
+template <typename T, typename TB, bool FIRST>
+__global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, T ax, T ay, T az, T beta, T dsign, T w,
                             T *__restrict__ dst, const T *__restrict__ src, const TB *__restrict__ b) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
         auto val = [&](long long c) -> T {
             int ci, cj, ck;
             if (is_inner(L, c, ci, cj, ck)) return dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, c);
-            return first ? src[c] : dst[c];
+            return FIRST ? src[c] : dst[c];
         };
         dev::bc_item<T, decltype(val), true>(L, V, item_index(main_z), dst, val);
         return;
     }
-    int i, j, k, bx, by, bz;
+    int bx, by, bz;
     main_block(main_z, bx, by, bz);
-    if (first) {
-        i = bx * BX + static_cast<int>(threadIdx.x);
-        j = by * BY + static_cast<int>(threadIdx.y);
-        k = bz;
-        if (i >= L.N[0] || j >= L.N[1]) return;
-    } else {
-        i = 1 + bx * BX + static_cast<int>(threadIdx.x);
-        j = 1 + by * BY + static_cast<int>(threadIdx.y);
-        k = 1 + bz;
-        if (i > L.n[0] || j > L.n[1]) return;
-    }
-    const long long idx = lin(L, i, j, k);
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2], N0 = L.N[0], N1 = L.N[1];
+    const int o = FIRST ? 0 : 1;
+    const int i = o + bx * BX + static_cast<int>(threadIdx.x);
+    const int j = o + by * BY + static_cast<int>(threadIdx.y);
+    const int k = o + bz;
+    if (FIRST ? (i >= N0 || j >= N1) : (i > n0 || j > n1)) return;
+    // 32-bit offsets (a level has fewer than 2^31 cells)
+    const unsigned nj = static_cast<unsigned>(N0), nk = nj * static_cast<unsigned>(N1);
+    const unsigned idx = static_cast<unsigned>(i) + nj * static_cast<unsigned>(j) + nk * static_cast<unsigned>(k);
+    const bool in_inner_box = FIRST ? (i >= 1 && j >= 1 && k >= 1 && i <= n0 && j <= n1 && k <= n2) : true;
     // the update is computed for every cell of the inner box before the
     // obstacle test, so the stencil loads are in flight while it runs
-    const bool in_inner_box = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2];
+    // (the expression of JacobiDiffuse.cpp:292-300, as dev::jacobi_t)
     T v = T(0);
-    if (in_inner_box) v = dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, idx);
+    if (in_inner_box) {
+        const T out_h = beta * (dsign * static_cast<T>(b[idx]) + ax * (src[idx + 1] + src[idx - 1]) +
+                                ay * (src[idx + nj] + src[idx - nj]) + az * (src[idx + nk] + src[idx - nk]));
+        v = (1 - w) * src[idx] + w * out_h;
+    }
     if (in_inner_box && !in_obstacle(L, i, j, k)) {
         dst[idx] = v;
-        dev::face_ghosts(L, V, dst, idx, i, j, k, v);
-    } else if (first && !dev::written(V, idx)) {
+        dev::face_ghosts(L, V, dst, static_cast<long long>(idx), i, j, k, v);
+    } else if (FIRST && !dev::written(V, idx)) {
         dst[idx] = src[idx];
     }
 }
@@ -520,15 +555,8 @@ __global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, T *__restric
 // prolongation does not reach get fine + 0.0, as the reference's whole-array
 // addition of a zero correction gives.
 template <typename Tf, typename Tc>
-__device__ inline Tf corrected(const LevelDesc &Lc, const LevelDesc &Lf, const Tf *__restrict__ fine,
-                               const Tc *__restrict__ coarse, long long c) {
-    int i, j, k;
-    {
-        const long long Nx = Lf.N[0], Ny = Lf.N[1];
-        k = static_cast<int>(c / (Nx * Ny));
-        j = static_cast<int>((c / Nx) % Ny);
-        i = static_cast<int>(c % Nx);
-    }
+__device__ inline Tf corrected_at(const LevelDesc &Lc, const LevelDesc &Lf, const Tf *__restrict__ fine,
+                                  const Tc *__restrict__ coarse, int i, int j, int k, long long c) {
     Tf p = 0.0;
     if (i >= 1 && j >= 1 && k >= 1) {
         const int ci = (i + 1) / 2, cj = (j + 1) / 2, ck = (k + 1) / 2;
@@ -538,6 +566,14 @@ __device__ inline Tf corrected(const LevelDesc &Lc, const LevelDesc &Lf, const T
         }
     }
     return fine[c] + p;
+}
+template <typename Tf, typename Tc>
+__device__ inline Tf corrected(const LevelDesc &Lc, const LevelDesc &Lf, const Tf *__restrict__ fine,
+                               const Tc *__restrict__ coarse, long long c) {
+    const unsigned u = static_cast<unsigned>(c), Nx = static_cast<unsigned>(Lf.N[0]),
+                   Ny = static_cast<unsigned>(Lf.N[1]);
+    return corrected_at<Tf, Tc>(Lc, Lf, fine, coarse, static_cast<int>(u % Nx), static_cast<int>((u / Nx) % Ny),
+                                static_cast<int>(u / (Nx * Ny)), c);
 }
 template <typename Tf, typename Tc>
 __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, Tf *__restrict__ dst,
@@ -553,12 +589,28 @@ __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int ma
     const int j = by * BY + static_cast<int>(threadIdx.y);
     const int k = bz;
     if (i >= Lf.N[0] || j >= Lf.N[1]) return;
-    const long long idx = lin(Lf, i, j, k);
+    // 32-bit offsets (a level has fewer than 2^31 cells); the coarse and
+    // fine loads are issued before the obstacle and fill tests
+    const unsigned idx = static_cast<unsigned>(i) +
+                         static_cast<unsigned>(Lf.N[0]) * (static_cast<unsigned>(j) +
+                                                           static_cast<unsigned>(Lf.N[1]) * static_cast<unsigned>(k));
+    const int ci = (i + 1) / 2, cj = (j + 1) / 2, ck = (k + 1) / 2;
+    const bool reach = i >= 1 && j >= 1 && k >= 1 && ci <= Lc.n[0] && cj <= Lc.n[1] && ck <= Lc.n[2];
+    const Tf f = fine[idx];
+    Tf p = 0.0;
+    if (reach) {
+        const int q = ((i & 1) ? 4 : 0) | ((j & 1) ? 2 : 0) | ((k & 1) ? 1 : 0);
+        const unsigned cidx = static_cast<unsigned>(ci) +
+                              static_cast<unsigned>(Lc.N[0]) * (static_cast<unsigned>(cj) +
+                                                                static_cast<unsigned>(Lc.N[1]) * static_cast<unsigned>(ck));
+        const Tf pc = static_cast<Tf>(dev::prolong_t<Tc>(Lc, coarse, cidx, q));
+        if (!in_obstacle(Lc, ci, cj, ck)) p = pc;
+    }
     if (dev::written(V, idx)) return;
-    const Tf v = corrected<Tf, Tc>(Lc, Lf, fine, coarse, idx);
+    const Tf v = f + p;  // corrected_at's value
     dst[idx] = v;
     if (i >= 1 && j >= 1 && k >= 1 && i <= Lf.n[0] && j <= Lf.n[1] && k <= Lf.n[2] && !in_obstacle(Lf, i, j, k))
-        dev::face_ghosts(Lf, V, dst, idx, i, j, k, v);
+        dev::face_ghosts(Lf, V, dst, static_cast<long long>(idx), i, j, k, v);
 }
 
 // One idempotent fill in a single launch (faces, ghosts, dependent ghosts)
@@ -764,9 +816,15 @@ template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
                double ax, double ay, double az, double beta, double dsign, double w) {
     const dim3 g = first ? full_grid(L) : inner_grid(L);
-    k_jacobi_bc<T, TB><<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(
-        L, V, static_cast<int>(g.z), first ? 1 : 0, static_cast<T>(ax), static_cast<T>(ay), static_cast<T>(az),
-        static_cast<T>(beta), static_cast<T>(dsign), static_cast<T>(w), dst, src, b);
+    const dim3 gi = with_items(g, dev::bc_items_grid(V));
+    const T ax_ = static_cast<T>(ax), ay_ = static_cast<T>(ay), az_ = static_cast<T>(az), beta_ = static_cast<T>(beta),
+            ds_ = static_cast<T>(dsign), w_ = static_cast<T>(w);
+    if (first)
+        k_jacobi_bc<T, TB, true><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_, w_,
+                                                        dst, src, b);
+    else
+        k_jacobi_bc<T, TB, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_, w_,
+                                                         dst, src, b);
 }
 template void jacobi_bc<double, double>(hipStream_t, const LevelDesc &, const BCView &, bool, double *,
                                         const double *, const double *, double, double, double, double, double,
@@ -819,6 +877,11 @@ void reduce_sum(hipStream_t s, const double *partial, int n, double *out) {
     k_reduce_sum<<<1, 1024, 0, s>>>(partial, n, out);
 }
 
+void explicit_diffuse(hipStream_t s, const LevelDesc &L, double *out, const double *in, double D, const double *ev,
+                      double dt) {
+    const double rdx = D / (L.h[0] * L.h[0]), rdy = D / (L.h[1] * L.h[1]), rdz = D / (L.h[2] * L.h[2]);
+    k_explicit_diffuse<<<inner_grid(L), block(), 0, s>>>(L, rdx, rdy, rdz, D, dt, out, in, ev);
+}
 void divergence(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *y, const double *z) {
     const double hx = 0.5 * (1. / L.h[0]), hy = 0.5 * (1. / L.h[1]), hz = 0.5 * (1. / L.h[2]);
     k_divergence<<<full_grid(L), block(), 0, s>>>(L, hx, hy, hz, out, x, y, z);

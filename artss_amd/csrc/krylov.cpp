@@ -49,7 +49,7 @@ void GcrPressure::iteration(Field &p) {
         m_pre.vcycle_mixed(m_r.data, m_coarse_max_iter);
         const float *z = m_pre.mixed_result();
         d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, z, m_wp32, m_r.data, m_partial); });
-        kk::gcr_coef(s, m_partial, kk::blocks_inner(L), m_coef);
+        kk::gcr_coef(s, m_partial, kk::blocks_dots(L), m_coef);
         d.timed("gcr_update", 0, [&] {
             kk::gcr_update(s, L, m_coef, p.data, m_r.data, z, m_zp32, m_wp32, m_partial);
         });
@@ -58,7 +58,7 @@ void GcrPressure::iteration(Field &p) {
         m_pre.vcycle_eager(m_z, m_r);
         d.apply_boundary_linear(m_z.data, P, 0);
         d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_z.data, m_wp.data, m_r.data, m_partial); });
-        kk::gcr_coef(s, m_partial, kk::blocks_inner(L), m_coef);
+        kk::gcr_coef(s, m_partial, kk::blocks_dots(L), m_coef);
         d.timed("gcr_update", 0, [&] {
             kk::gcr_update(s, L, m_coef, p.data, m_r.data, m_z.data, m_zp.data, m_wp.data, m_partial);
         });

@@ -11,6 +11,7 @@ namespace kk {
 
 int blocks_inner(const LevelDesc &L);  // partial slots per sum of the inner-grid kernels
 int blocks_full(const LevelDesc &L);   // ... of the whole-array kernels
+int blocks_dots(const LevelDesc &L);   // ... of lap_dots
 
 // r = b - lap(x) on inner cells (x must carry its boundary fill); 1 sum: (r, r)
 void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x, const double *b, double *partial);

@@ -96,20 +96,33 @@ __global__ void k_residual_vec(LevelDesc L, double *__restrict__ r, const double
 }
 
 // w = lap(z) on inner cells (not stored: the update recomputes it);
-// partials (w,w), (w,wp), (r,w), (r,wp)
+// partials (w,w), (w,wp), (r,w), (r,wp).  A thread sums kDotsZ planes
+// before the four workgroup reductions, which cost more than the loads.
+constexpr int kDotsZ = 4;
+inline dim3 dots_grid(const LevelDesc &L) {
+    return dim3((L.n[0] + BX - 1) / BX, (L.n[1] + BY - 1) / BY, (L.n[2] + kDotsZ - 1) / kDotsZ);
+}
 template <typename Tz, typename Tw>
 __global__ void k_lap_dots(LevelDesc L, const Tz *__restrict__ z, const Tw *__restrict__ wp,
                            const double *__restrict__ r, double *__restrict__ partial) {
-    int i, j, k;
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int i = 1 + bx * BX + static_cast<int>(threadIdx.x);
+    const int j = 1 + by * BY + static_cast<int>(threadIdx.y);
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    if (inner_at(L, i, j, k)) {
-        const long long idx = dev::lin(L, i, j, k);
-        const double x = lap(L, z, idx);
-        const double p = wp[idx], q = r[idx];
-        v[0] = x * x;
-        v[1] = x * p;
-        v[2] = q * x;
-        v[3] = q * p;
+    if (i <= L.n[0] && j <= L.n[1]) {
+        for (int q = 0; q < kDotsZ; q++) {
+            const int k = 1 + bz * kDotsZ + q;
+            if (k > L.n[2]) break;
+            if (in_obstacle(L, i, j, k)) continue;
+            const long long idx = dev::lin(L, i, j, k);
+            const double x = lap(L, z, idx);
+            const double p = wp[idx], rr = r[idx];
+            v[0] += x * x;
+            v[1] += x * p;
+            v[2] += rr * x;
+            v[3] += rr * p;
+        }
     }
     write_partials<4>(v, partial);
 }
@@ -278,6 +291,10 @@ int blocks_inner(const LevelDesc &L) {
     const dim3 g = inner_grid(L);
     return static_cast<int>(g.x * g.y * g.z);
 }
+int blocks_dots(const LevelDesc &L) {
+    const dim3 g = dots_grid(L);
+    return static_cast<int>(g.x * g.y * g.z);
+}
 int blocks_full(const LevelDesc &L) {
     const dim3 g = full_grid(L);
     return static_cast<int>(g.x * g.y * g.z);
@@ -288,7 +305,7 @@ void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x,
 }
 template <typename Tz, typename Tw>
 void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tw *wp, const double *r, double *partial) {
-    k_lap_dots<Tz, Tw><<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, z, wp, r, partial);
+    k_lap_dots<Tz, Tw><<<dots_grid(L), dim3(BX, BY), 0, s>>>(L, z, wp, r, partial);
 }
 template void lap_dots<double, double>(hipStream_t, const LevelDesc &, const double *, const double *,
                                        const double *, double *);

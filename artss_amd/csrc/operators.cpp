@@ -80,6 +80,26 @@ void JacobiDiffuse::diffuse(Field &out, const Field &in, const Field &b, double 
     maybe_sync(m_domain, sync);
 }
 
+// ExplicitDiffuse.cpp:27-47 (no iteration: one step, then the fill)
+void ExplicitDiffuse::run(Field &out, const Field &in, double D, const Field *EV) {
+    Domain &dm = m_domain;
+    dm.timed("explicit_diffuse", 0, [&] {
+        k::explicit_diffuse(dm.stream(), dm.level(0), out.data, in.data, D, EV ? EV->data : nullptr,
+                            dm.settings().dt);
+    });
+    dm.apply_boundary(out.data, out.get_type(), 0);
+    m_iterations = 1;
+    m_residual = 0.0;
+}
+void ExplicitDiffuse::diffuse(Field &out, const Field &in, const Field &, double D, bool sync) {
+    run(out, in, D, nullptr);
+    maybe_sync(m_domain, sync);
+}
+void ExplicitDiffuse::diffuse(Field &out, const Field &in, const Field &, double D, const Field &EV, bool sync) {
+    run(out, in, D, &EV);
+    maybe_sync(m_domain, sync);
+}
+
 // ColoredGaussSeidelDiffuse.cpp:38-173: red then black in place on `out`,
 // `in` holds the previous iterate (whole-array copy) for the update norm.
 void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b, double D, const Field *EV) {
@@ -176,6 +196,7 @@ std::unique_ptr<IAdvection> make_advection(Domain &d, const std::string &type) {
 std::unique_ptr<IDiffusion> make_diffusion(Domain &d, const DiffusionSettings &s) {
     if (s.type == "Jacobi") return std::make_unique<JacobiDiffuse>(d, s);
     if (s.type == "ColoredGaussSeidel") return std::make_unique<ColoredGaussSeidelDiffuse>(d, s);
+    if (s.type == "Explicit") return std::make_unique<ExplicitDiffuse>(d);
     throw config_error("diffusion method '" + s.type + "' is not implemented");
 }
 

@@ -96,6 +96,18 @@ class JacobiDiffuse : public IDiffusion {
     DiffusionSettings m_settings;
 };
 
+// diffusion/ExplicitDiffuse.cpp:27-47: one explicit Euler step and the fill
+class ExplicitDiffuse : public IDiffusion {
+  public:
+    explicit ExplicitDiffuse(Domain &d) : m_domain(d) {}
+    void diffuse(Field &out, const Field &in, const Field &b, double D, bool sync) override;
+    void diffuse(Field &out, const Field &in, const Field &b, double D, const Field &EV, bool sync) override;
+
+  private:
+    void run(Field &out, const Field &in, double D, const Field *EV);
+    Domain &m_domain;
+};
+
 // diffusion/ColoredGaussSeidelDiffuse.cpp:38-173
 class ColoredGaussSeidelDiffuse : public IDiffusion {
   public:

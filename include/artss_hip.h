@@ -81,9 +81,10 @@ int artss_apply_boundary(artss_domain *d, double *field, int field_type, int lev
 int artss_sl_advect(artss_domain *d, double *out, const double *in, const double *u, const double *v,
                     const double *w, int out_type);
 /* IDiffusion::diffuse (both overloads, ev == NULL for the laminar one),
- * method "Jacobi" (diffusion/JacobiDiffuse.cpp:31,123) or
- * "ColoredGaussSeidel" (diffusion/ColoredGaussSeidelDiffuse.cpp:38,111).
- * `in` is modified as in the reference. */
+ * method "Jacobi" (diffusion/JacobiDiffuse.cpp:31,123),
+ * "ColoredGaussSeidel" (diffusion/ColoredGaussSeidelDiffuse.cpp:38,111) or
+ * "Explicit" (diffusion/ExplicitDiffuse.cpp:27,40; one step, max_iter, tol_res
+ * and w unused).  `in` is modified as in the reference. */
 int artss_diffuse(artss_domain *d, const char *method, double *out, double *in, const double *b, double D,
                   const double *ev, int field_type, int max_iter, double tol_res, double w, int *iterations,
                   double *residual);

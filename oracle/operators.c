@@ -202,6 +202,47 @@ int or_jacobi_diffuse(const or_domain *D, double *out, double *in, const double 
     return it;
 }
 
+/* ---------------- explicit diffusion, diffusion/ExplicitDiffuse.cpp ---------------- */
+/* one explicit Euler step on the inner list (:49-78 plain, :81-121 turbulent:
+   the turbulent form has no 1/h^2 factors and subtracts the upper flux from
+   the lower one, both as in the reference), then the boundary fill (:27, :44) */
+void or_explicit_diffuse(const or_domain *D, double *out, const double *in, double Dc, const double *ev, double dt,
+                         int ft) {
+    const size_t Nx = D->N[0][0], Ny = D->N[0][1];
+    const size_t ni = 1, nj = Nx, nk = Nx * Ny;
+    const ivec *inner = &D->inner[0];
+    if (!ev) {
+        const double rdx = Dc / (D->h[0][0] * D->h[0][0]);
+        const double rdy = Dc / (D->h[0][1] * D->h[0][1]);
+        const double rdz = Dc / (D->h[0][2] * D->h[0][2]);
+        for (size_t l = 0; l < inner->n; ++l) {
+            const size_t i = inner->d[l];
+            out[i] = in[i] + dt * ((in[i + ni] - 2 * in[i] + in[i - ni]) * rdx +
+                                   (in[i + nj] - 2 * in[i] + in[i - nj]) * rdy +
+                                   (in[i + nk] - 2 * in[i] + in[i - nk]) * rdz);
+        }
+    } else {
+        for (size_t l = 0; l < inner->n; ++l) {
+            const size_t i = inner->d[l];
+            double nu_x1 = 0.5 * (ev[i + ni] + ev[i]) + Dc;
+            double nu_x2 = 0.5 * (ev[i - ni] + ev[i]) + Dc;
+            double nu_y1 = 0.5 * (ev[i + nj] + ev[i]) + Dc;
+            double nu_y2 = 0.5 * (ev[i - nj] + ev[i]) + Dc;
+            double nu_z1 = 0.5 * (ev[i + nk] + ev[i]) + Dc;
+            double nu_z2 = 0.5 * (ev[i - nk] + ev[i]) + Dc;
+            double di0 = (in[i] - in[i - ni]);
+            double di1 = (in[i + ni] - in[i]);
+            double dj0 = (in[i] - in[i - nj]);
+            double dj1 = (in[i + nj] - in[i]);
+            double dk0 = (in[i] - in[i - nk]);
+            double dk1 = (in[i + nk] - in[i]);
+            out[i] = in[i] + dt * ((nu_x1 * di0 - nu_x2 * di1) + (nu_y1 * dj0 - nu_y2 * dj1) +
+                                   (nu_z1 * dk0 - nu_z2 * dk1));
+        }
+    }
+    or_apply_bc(D, out, ft, 0);
+}
+
 /* ---------------- colored Gauss-Seidel, diffusion/ColoredGaussSeidelDiffuse.cpp ---------------- */
 static void cgs_stencil(size_t index, double *out, const double *b, double ax, double ay, double az, double dsign,
                         double beta, double w, size_t Nx, size_t Ny) {

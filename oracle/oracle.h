@@ -72,6 +72,8 @@ int or_jacobi_diffuse(const or_domain *D, double *out, double *in, const double 
                       double dt, int max_iter, double tol_res, double wgt, int ft, double *res_out);
 int or_cgs_diffuse(const or_domain *D, double *out, double *in, const double *b, double Dc, const double *ev,
                    double dt, int max_iter, double tol_res, double wgt, int ft, double *res_out);
+void or_explicit_diffuse(const or_domain *D, double *out, const double *in, double Dc, const double *ev, double dt,
+                         int ft);
 void or_divergence(const or_domain *D, double *out, const double *x, const double *y, const double *z);
 void or_projection(const or_domain *D, double *ou, double *ov, double *ow, const double *iu, const double *iv,
                    const double *iw, const double *ip);
@@ -103,7 +105,7 @@ int or_mg_pressure(or_mg *mg, double *out, const double *b, double t, double dt)
 typedef struct {
     int solver;         /* 0 NSSolver, 1 NSTempTurbSolver */
     double dt, nu, kappa, beta, g;
-    int diff_type;      /* 0 Jacobi, 1 CGS */
+    int diff_type;      /* 0 Jacobi, 1 CGS, 2 Explicit */
     int diff_max_iter;
     double diff_tol, diff_w;
     double cs;          /* ConstSmagorinsky */

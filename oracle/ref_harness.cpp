@@ -32,6 +32,7 @@
 
 #include "advection/SLAdvect.h"
 #include "diffusion/ColoredGaussSeidelDiffuse.h"
+#include "diffusion/ExplicitDiffuse.h"
 #include "diffusion/JacobiDiffuse.h"
 #include "domain/DomainController.h"
 #include "domain/DomainData.h"
@@ -345,7 +346,8 @@ int mode_op(const std::string &xml, const std::string &op, const std::string &ou
         }
         return 0;
     }
-    if (op == "jacobi" || op == "jacobi_turb" || op == "cgs" || op == "cgs_turb") {
+    if (op == "jacobi" || op == "jacobi_turb" || op == "cgs" || op == "cgs_turb" || op == "explicit" ||
+        op == "explicit_turb") {
         const bool turb = op.find("turb") != std::string::npos;
         Field out(FieldType::U, 0.0), in(FieldType::U, 0.0), b(FieldType::U, 0.0), ev(FieldType::NU, 0.0);
         fill_random(in, 1.0, 0.0);
@@ -360,6 +362,8 @@ int mode_op(const std::string &xml, const std::string &op, const std::string &ou
         IDiffusion *dif = nullptr;
         if (op.rfind("jacobi", 0) == 0) {
             dif = new JacobiDiffuse(std::get<Settings::solver::diffusion_solvers::jacobi>(sp.diffusion.solver.value()));
+        } else if (op.rfind("explicit", 0) == 0) {
+            dif = new ExplicitDiffuse();
         } else {
             dif = new ColoredGaussSeidelDiffuse(
                     std::get<Settings::solver::diffusion_solvers::colored_gauss_seidel>(sp.diffusion.solver.value()));

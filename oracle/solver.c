@@ -38,6 +38,10 @@ static int diffuse(or_sim *s, double *out, double *in, const double *b, double D
     const or_sim_params *p = &s->p;
     if (p->diff_type == 0)
         return or_jacobi_diffuse(s->D, out, in, b, Dc, ev, p->dt, p->diff_max_iter, p->diff_tol, p->diff_w, ft, NULL);
+    if (p->diff_type == 2) {
+        or_explicit_diffuse(s->D, out, in, Dc, ev, p->dt, ft);
+        return 1;
+    }
     return or_cgs_diffuse(s->D, out, in, b, Dc, ev, p->dt, p->diff_max_iter, p->diff_tol, p->diff_w, ft, NULL);
 }
 

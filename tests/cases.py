@@ -15,6 +15,8 @@ _CGS = ('<diffusion type="ColoredGaussSeidel" field="{f}"><max_iter> {mi} </max_
 
 
 def diffusion_xml(kind, field, max_iter=100, tol=1e-7, w=1.0):
+    if kind == "Explicit":
+        return '<diffusion type="Explicit" field="{}"></diffusion>'.format(field)
     t = _JACOBI if kind == "Jacobi" else _CGS
     return t.format(f=field, mi=max_iter, tol=repr(tol), w=repr(w))
 

@@ -40,6 +40,10 @@ CASES = {
                                             diffusion="ColoredGaussSeidel", smoother="ColoredGaussSeidel",
                                             obstacles=OBST_ONE, hrr=3.0, source_x=1.0),
                     ops=[("cgs", 1e-2), ("cgs_turb", 1e-2), ("vcycle1", 1.0), ("vcycle2", 1.0)], lists=False),
+    "ops_explicit": dict(xml=lambda: cases.plume(n=(16, 8, 8), X=(0.0, 2.0), levels=2, steps=2, dt=0.01,
+                                                 diffusion="Jacobi", smoother="Jacobi", obstacles=OBST_ONE,
+                                                 hrr=3.0, source_x=1.0),
+                         ops=[("explicit", 1e-2), ("explicit_turb", 1e-2)], lists=False),
     "geo_adjacent": dict(xml=lambda: _bc_mixed(OBST_ADJ), ops=[("bc", 1.0)], lists=True),
     # whole-solver runs: state after steps 1, 2 and the last one
     "run_ns_tg": dict(xml=lambda: cases.ns_taylor_green(n=16, levels=2, steps=5, dt=2e-3, reference=True),
@@ -47,6 +51,9 @@ CASES = {
     "run_plume_jacobi": dict(xml=lambda: cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02,
                                                      diffusion="Jacobi", obstacles=OBST_ONE, hrr=5.0),
                              run=("1,2", "")),
+    "run_plume_explicit": dict(xml=lambda: cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02,
+                                                       diffusion="Explicit", obstacles=OBST_ONE, hrr=5.0),
+                               run=("1,2", "")),
     "run_plume_cgs": dict(xml=lambda: cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02,
                                                   diffusion="ColoredGaussSeidel", smoother="ColoredGaussSeidel",
                                                   obstacles=OBST_ONE, hrr=5.0),

@@ -75,6 +75,7 @@ def lib():
             fn.restype = C.c_int
             fn.argtypes = [C.c_void_p, dp, dp, dp, C.c_double, dp, C.c_double, C.c_int, C.c_double, C.c_double,
                            C.c_int, dp]
+        L.or_explicit_diffuse.argtypes = [C.c_void_p, dp, dp, C.c_double, dp, C.c_double, C.c_int]
         L.or_divergence.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_projection.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp, dp]
         L.or_smagorinsky.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_double, C.c_int]
@@ -196,6 +197,9 @@ class Domain:
         self.L.or_sl_advect(self.h, _p(out), _p(inp), _p(u), _p(v), _p(w), dt, ft)
 
     def diffuse(self, kind, out, inp, b, D, ev, dt, max_iter, tol, w, ft):
+        if kind == "Explicit":
+            self.L.or_explicit_diffuse(self.h, _p(out), _p(inp), D, _p(ev), dt, ft)
+            return 1, 0.0
         res = C.c_double()
         fn = self.L.or_jacobi_diffuse if kind == "Jacobi" else self.L.or_cgs_diffuse
         it = fn(self.h, _p(out), _p(inp), _p(b), D, _p(ev), dt, max_iter, tol, w, ft, C.byref(res))
@@ -257,8 +261,8 @@ def sim_params(cfg):
         raise ValueError("oracle drives NSSolver and NSTempTurbSolver only: " + desc)
     sp.dt, sp.nu, sp.kappa, sp.beta, sp.g = cfg["dt"], cfg["nu"], cfg["kappa"], cfg["beta"], cfg["g"]
     dif = cfg["diffusion"]
-    sp.diff_type = 0 if dif["type"] == "Jacobi" else 1
-    sp.diff_max_iter, sp.diff_tol, sp.diff_w = dif["max_iter"], dif["tol_res"], dif["w"]
+    sp.diff_type = {"Jacobi": 0, "ColoredGaussSeidel": 1, "Explicit": 2}[dif["type"]]
+    sp.diff_max_iter, sp.diff_tol, sp.diff_w = dif.get("max_iter", 0), dif.get("tol_res", 0.0), dif.get("w", 0.0)
     if "turbulence" in cfg:
         sp.cs = cfg["turbulence"]["cs"]
     src = cfg.get("source", {"force_fct": "Zero", "dir": []})

@@ -54,6 +54,12 @@ def ops_cgs():
 
 
 @pytest.fixture(scope="module")
+def ops_explicit():
+    z, xml, cfg = load("ops_explicit")
+    return z, cfg, artss_amd.Domain(xml)
+
+
+@pytest.fixture(scope="module")
 def geo_adjacent():
     z, xml, cfg = load("geo_adjacent")
     return z, cfg, artss_amd.Domain(xml)
@@ -91,6 +97,18 @@ def test_diffusion_golden(case, op, request):
     it, _ = dom.diffuse(d["type"], out, inp, dom.field(z[op + "/dif_b"]), cfg["nu"], ev, 1, d["max_iter"],
                         d["tol_res"], d["w"])
     assert it >= 2
+    same(out.download(), z[op + "/dif_out"], op + " out")
+    same(inp.download(), z[op + "/dif_in"], op + " in")
+
+
+@pytest.mark.parametrize("op", ["explicit", "explicit_turb"])
+def test_explicit_diffusion_golden(ops_explicit, op):
+    z, cfg, dom = ops_explicit
+    out = dom.field(z[op + "/dif_out0"])
+    inp = dom.field(z[op + "/dif_in0"])
+    ev = dom.field(z[op + "/dif_ev"]) if op.endswith("turb") else None
+    it, _ = dom.diffuse("Explicit", out, inp, dom.field(z[op + "/dif_b"]), cfg["nu"], ev, 1, 0, 0.0, 0.0)
+    assert it == 1
     same(out.download(), z[op + "/dif_out"], op + " out")
     same(inp.download(), z[op + "/dif_in"], op + " in")
 
@@ -134,7 +152,7 @@ def test_smagorinsky_sources_golden(ops_jacobi):
     same(st.download(), z["sources/gauss_st"], "gauss")
 
 
-@pytest.mark.parametrize("case", ["run_ns_tg", "run_plume_jacobi", "run_plume_cgs"])
+@pytest.mark.parametrize("case", ["run_ns_tg", "run_plume_jacobi", "run_plume_cgs", "run_plume_explicit"])
 def test_solver_golden(case):
     z, xml, cfg = load(case)
     xml = xml.replace('usr_fct="Zero"', 'usr_fct="TaylorGreen"') if cfg["solver"] == "NSSolver" else xml

@@ -10,11 +10,12 @@ Every oracle function is compared bit for bit (np.array_equal on float64):
   SL advection (a1)             test_advect
   Jacobi / turbulent (a2, a3)   test_diffusion[jacobi, jacobi_turb]
   CGS / turbulent (a5)          test_diffusion[cgs, cgs_turb]
+  explicit diffusion (8(f) 3)   test_explicit_diffusion[explicit, explicit_turb]
   V-cycle MG (a4, a6-a11)       test_vcycle (Jacobi and CGS smoothers, Nt == 1 and Nt > 1)
   divergence, projection (a12)  test_divergence, test_projection
   ConstSmagorinsky (a13)        test_smagorinsky
   sources (a17)                 test_sources (add_source scalar/vector, buoyancy, Gauss)
-  solver drivers (a18, a19)     test_run (NSSolver Taylor-Green, NSTempTurb plume Jacobi/CGS)
+  solver drivers (a18, a19)     test_run (NSSolver Taylor-Green, NSTempTurb plume Jacobi/CGS/explicit)
 
 The reference's committed tests/*/*_ref.dat files are not used: they do not
 reproduce with the reference's current sources (their initial conditions
@@ -48,6 +49,12 @@ def ops_jacobi():
 @pytest.fixture(scope="module")
 def ops_cgs():
     z, cfg = load("ops_cgs")
+    return z, cfg, O.Domain(cfg)
+
+
+@pytest.fixture(scope="module")
+def ops_explicit():
+    z, cfg = load("ops_explicit")
     return z, cfg, O.Domain(cfg)
 
 
@@ -132,6 +139,18 @@ def test_diffusion(case, op, request):
     same(inp, z[op + "/dif_in"])
 
 
+@pytest.mark.parametrize("op", ["explicit", "explicit_turb"])
+def test_explicit_diffusion(ops_explicit, op):
+    z, cfg, dom = ops_explicit
+    out = z[op + "/dif_out0"].copy()
+    inp = z[op + "/dif_in0"].copy()
+    ev = z[op + "/dif_ev"] if op.endswith("turb") else None
+    dom.diffuse("Explicit", out, inp, z[op + "/dif_b"], cfg["nu"], ev, cfg["dt"], 0, 0.0, 0.0, 1)
+    same(out, z[op + "/dif_out"])
+    same(inp, z[op + "/dif_in"])  # the explicit step leaves its input alone
+    assert not np.array_equal(out, z[op + "/dif_out0"])
+
+
 @pytest.mark.parametrize("case", ["ops_jacobi", "ops_cgs"])
 @pytest.mark.parametrize("op", ["vcycle1", "vcycle2"])
 def test_vcycle(case, op, request):
@@ -187,7 +206,7 @@ def test_sources(ops_jacobi):
     same(st, z["sources/gauss_st"])
 
 
-@pytest.mark.parametrize("case", ["run_ns_tg", "run_plume_jacobi", "run_plume_cgs"])
+@pytest.mark.parametrize("case", ["run_ns_tg", "run_plume_jacobi", "run_plume_cgs", "run_plume_explicit"])
 def test_run(case):
     z, cfg = load(case)
     dom = O.Domain(cfg)
