@@ -39,6 +39,9 @@ void sl_advect(hipStream_t s, const LevelDesc &L, double dt, double *out, const 
 void jacobi_diffuse_step(hipStream_t s, const LevelDesc &L, double *out, const double *in, const double *b,
                          double ax, double ay, double az, double rb, double D, const double *ev, double dt,
                          double w, double *partial);
+// ISource.cpp:63-114: out += dt * (viscous dissipation) on inner cells
+void dissipate(hipStream_t s, const LevelDesc &L, double *out, const double *u, const double *v, const double *w,
+               double nu, double dt);
 // ExplicitDiffuse.cpp:49-121: one explicit Euler step on inner cells (ev == nullptr: plain form)
 void explicit_diffuse(hipStream_t s, const LevelDesc &L, double *out, const double *in, double D, const double *ev,
                       double dt);

@@ -304,6 +304,30 @@ __global__ void k_explicit_diffuse(LevelDesc L, double rdx, double rdy, double r
     out[idx] = in[idx] + dt * ((nu_x1 * di0 - nu_x2 * di1) + (nu_y1 * dj0 - nu_y2 * dj1) + (nu_z1 * dk0 - nu_z2 * dk1));
 }
 
+// ------------------------------------------------------------------ dissipation
+// ISource.cpp:85-114: the expression as the reference writes it (every
+// difference quotient formed the same way each time it appears)
+__global__ void k_dissipate(LevelDesc L, double rdx, double rdy, double rdz, double nu, double dt,
+                            double *__restrict__ out, const double *__restrict__ in_u,
+                            const double *__restrict__ in_v, const double *__restrict__ in_w) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long ni = 1, nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const double ux = 0.5 * rdx * (in_u[idx + ni] - in_u[idx - ni]);
+    const double vy = 0.5 * rdy * (in_v[idx + nj] - in_v[idx - nj]);
+    const double wz = 0.5 * rdz * (in_w[idx + nk] - in_w[idx - nk]);
+    const double vx = 0.5 * rdx * (in_v[idx + ni] - in_v[idx - ni]);
+    const double uy = 0.5 * rdy * (in_u[idx + nj] - in_u[idx - nj]);
+    const double wy = 0.5 * rdy * (in_w[idx + nj] - in_w[idx - nj]);
+    const double vz = 0.5 * rdz * (in_v[idx + nk] - in_v[idx - nk]);
+    const double uz = 0.5 * rdz * (in_u[idx + nk] - in_u[idx - nk]);
+    const double wx = 0.5 * rdx * (in_w[idx + ni] - in_w[idx - ni]);
+    const double out_h = nu * (2 * ux * ux + 2 * vy * vy + 2 * wz * wz + (vx + uy) * (vx + uy) +
+                               (wy + vz) * (wy + vz) + (uz + wx) * (uz + wx));
+    out[idx] += dt * out_h;
+}
+
 // ------------------------------------------------------------------ colored Gauss-Seidel
 // One color of ColoredGaussSeidelDiffuse::colored_gauss_seidel_step
 // (:212-222, :270-294) with the stencil of :316-341.  Cells of one color only
@@ -877,6 +901,10 @@ void reduce_sum(hipStream_t s, const double *partial, int n, double *out) {
     k_reduce_sum<<<1, 1024, 0, s>>>(partial, n, out);
 }
 
+void dissipate(hipStream_t s, const LevelDesc &L, double *out, const double *u, const double *v, const double *w,
+               double nu, double dt) {
+    k_dissipate<<<inner_grid(L), block(), 0, s>>>(L, 1. / L.h[0], 1. / L.h[1], 1. / L.h[2], nu, dt, out, u, v, w);
+}
 void explicit_diffuse(hipStream_t s, const LevelDesc &L, double *out, const double *in, double D, const double *ev,
                       double dt) {
     const double rdx = D / (L.h[0] * L.h[0]), rdy = D / (L.h[1] * L.h[1]), rdz = D / (L.h[2] * L.h[2]);

@@ -34,6 +34,14 @@ void ISource::buoyancy_force(Field &out, const Field &in, const Field &in_a, dou
     maybe_sync(d, sync);
 }
 
+void ISource::dissipate(Field &out, const Field &in_u, const Field &in_v, const Field &in_w, bool sync) {
+    Domain &d = out.domain();
+    k::dissipate(d.stream(), d.level(0), out.data, in_u.data, in_v.data, in_w.data, d.settings().nu,
+                 d.settings().dt);
+    d.apply_boundary(out.data, out.get_type(), 0);
+    maybe_sync(d, sync);
+}
+
 void SLAdvect::advect(Field &out, const Field &in, const Field &u, const Field &v, const Field &w, bool sync) {
     m_domain.timed("sl_advect", 0, [&] {
         k::sl_advect(m_domain.stream(), m_domain.level(0), m_domain.settings().dt, out.data, in.data, u.data,

@@ -68,6 +68,8 @@ class ISource {
                             const Field &S_z, bool sync) = 0;
     virtual void add_source(Field &out, const Field &S, bool sync) = 0;
     static void buoyancy_force(Field &out, const Field &in, const Field &in_a, double beta, double g, bool sync);
+    // ISource.cpp:63-121: out += dt * nu * (viscous dissipation of u, v, w), then the fill
+    static void dissipate(Field &out, const Field &in_u, const Field &in_v, const Field &in_w, bool sync);
 };
 
 // ---------------------------------------------------------------- implementations

@@ -124,30 +124,44 @@ void upload(Domain &d, double *dst, const double *src, size_t n) {
 Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     Domain &d = *m_domain;
     const Settings &s = d.settings();
-    if (s.solver == "NSSolver") m_kind = 0;
-    else if (s.solver == "NSTempTurbSolver") m_kind = 1;
-    else if (s.solver == "PressureSolver") m_kind = 2;
-    else throw config_error("solver '" + s.solver + "' is not part of this build (NSSolver, NSTempTurbSolver, PressureSolver)");
+    static const char *kinds[] = {"NSSolver",        "NSTempTurbSolver",         "PressureSolver",
+                                  "NSTempSolver",    "NSTurbSolver",             "DiffusionSolver",
+                                  "AdvectionDiffusionSolver", "DiffusionTurbSolver"};
+    m_kind = -1;
+    for (int k = 0; k < 8; k++)
+        if (s.solver == kinds[k]) m_kind = k;
+    if (m_kind < 0)
+        throw config_error("solver '" + s.solver + "' is not part of this build (NSSolver, NSTurbSolver, NSTempSolver, "
+                           "NSTempTurbSolver, PressureSolver, DiffusionSolver, DiffusionTurbSolver, "
+                           "AdvectionDiffusionSolver)");
     if (s.has_source && s.force_fct == "Buoyancy" && s.use_init_values)
         throw config_error("buoyancy with use_init_values is not supported");
     if (s.has_source && s.force_fct == "Uniform") throw config_error("uniform momentum source is not supported");
-    if (s.has_temperature && (s.dissipation || s.temp_random))
-        throw config_error("dissipation / random temperature source are not supported");
+    if (s.has_temperature && s.temp_random) throw config_error("random temperature source is not supported");
+    if (s.has_temperature && s.dissipation && m_kind != 3)
+        throw config_error("dissipation is only part of NSTempSolver (NSTempSolver.cpp:181-185)");
+    if (s.has_turbulence && s.turbulence_type != "ConstSmagorinsky")
+        throw config_error("turbulence model '" + s.turbulence_type + "' is not part of this build");
     if (s.ic_random) throw config_error("random initial conditions are not supported");
 
     m_fields = std::make_unique<FieldController>(d);
-    // NSSolver / NSTempTurbSolver constructors (SolverSelection::set_*)
-    if (m_kind != 2) {
-        m_adv = make_advection(d, s.advection_type);
-        m_dif = make_diffusion(d, s.diffusion);
+    // the solver constructors (SolverSelection::set_*): NSSolver.cpp:17-36,
+    // NSTurbSolver.cpp, NSTempSolver.cpp:16-49, NSTempTurbSolver.cpp,
+    // DiffusionSolver.cpp, AdvectionDiffusionSolver.cpp, DiffusionTurbSolver.cpp
+    const bool velocity_adv = m_kind == 0 || m_kind == 1 || m_kind == 3 || m_kind == 4 || m_kind == 6;
+    const bool velocity_dif = m_kind != 2;
+    const bool ns = m_kind == 0 || m_kind == 1 || m_kind == 3 || m_kind == 4;
+    if (velocity_adv) m_adv = make_advection(d, s.advection_type);
+    if (velocity_dif) m_dif = make_diffusion(d, s.diffusion);
+    if (ns) {
         m_src_vel = std::make_unique<ExplicitEulerSource>(d, s.force_dir);
         m_add_source = s.force_fct != "Zero";
     }
-    m_pres = std::make_unique<VCycleMG>(d, s.pressure);
-    if (m_kind == 1) {
+    if (ns || m_kind == 2) m_pres = std::make_unique<VCycleMG>(d, s.pressure);
+    if (m_kind == 1 || m_kind == 4 || m_kind == 7) m_turb = make_turbulence(d, s);
+    if (m_kind == 1 || m_kind == 3) {
         if (s.temp_advection_type != "SemiLagrangian") throw config_error("temperature advection must be SemiLagrangian");
-        m_turb = make_turbulence(d, s);
-        m_dif_temp = make_diffusion(d, s.diffusion);  // NSTempTurbSolver.cpp:50 uses the velocity settings
+        m_dif_temp = make_diffusion(d, s.diffusion);  // both solvers use the velocity settings (NSTempSolver.cpp:33)
         m_src_temp = std::make_unique<ExplicitEulerSource>(d, s.temp_dir);
         if (s.temp_fct == "Gauss") m_gauss = std::make_unique<GaussFunction>(d, s.gauss);
         m_add_temp_source = s.temp_fct != "Zero";
@@ -283,7 +297,7 @@ void Solver::update_sources(double t) {
     Domain &d = *m_domain;
     const Settings &s = d.settings();
     FieldController &f = *m_fields;
-    if (m_kind != 2 && s.force_fct == "Buoyancy") {
+    if (m_src_vel && s.force_fct == "Buoyancy") {
         Field *out[3] = {&f.f_x, &f.f_y, &f.f_z};
         for (int a : s.force_dir) ISource::buoyancy_force(*out[a], f.T, f.T_ambient, s.beta, s.g, false);
     }
@@ -392,11 +406,104 @@ void Solver::do_step_nstempturb(double t) {
     }
 }
 
+// NSTempSolver.cpp:69-198: NSTempTurbSolver without turbulence, laminar
+// diffusion, and the optional viscous dissipation source
+void Solver::do_step_nstemp(double t) {
+    Domain &d = *m_domain;
+    const Settings &s = d.settings();
+    FieldController &f = *m_fields;
+    m_adv->advect(f.u, f.u0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.v, f.v0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.w, f.w0, f.u0, f.v0, f.w0, false);
+    f.couple_vector();
+    if (s.nu != 0.) {
+        m_dif->diffuse(f.u, f.u0, f.u_tmp, s.nu, false);
+        m_counters.diffusion[0] = m_dif->last_iterations();
+        m_dif->diffuse(f.v, f.v0, f.v_tmp, s.nu, false);
+        m_counters.diffusion[1] = m_dif->last_iterations();
+        m_dif->diffuse(f.w, f.w0, f.w_tmp, s.nu, false);
+        m_counters.diffusion[2] = m_dif->last_iterations();
+        f.couple_vector();
+    }
+    if (m_add_source) {
+        m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
+        f.couple_vector();
+    }
+    m_pres->divergence(f.rhs, f.u_tmp, f.v_tmp, f.w_tmp, false);
+    pressure_step(t);
+    m_pres->projection(f.u, f.v, f.w, f.u_tmp, f.v_tmp, f.w_tmp, f.p, false);
+    m_adv->advect(f.T, f.T0, f.u, f.v, f.w, false);
+    f.couple_scalar();
+    if (s.kappa != 0.) {
+        m_dif_temp->diffuse(f.T, f.T0, f.T_tmp, s.kappa, false);
+        m_counters.diffusion[3] = m_dif_temp->last_iterations();
+        f.couple_scalar();
+    }
+    if (s.dissipation) {
+        ISource::dissipate(f.T, f.u, f.v, f.w, false);
+        f.couple_scalar();
+    }
+    if (m_add_temp_source) {
+        m_src_temp->add_source(f.T, f.S_T, false);
+        f.couple_scalar();
+    }
+}
+
+// NSTurbSolver.cpp do_step: NSSolver with Smagorinsky viscosity in the diffusion
+void Solver::do_step_nsturb(double t) {
+    const Settings &s = m_domain->settings();
+    FieldController &f = *m_fields;
+    m_adv->advect(f.u, f.u0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.v, f.v0, f.u0, f.v0, f.w0, false);
+    m_adv->advect(f.w, f.w0, f.u0, f.v0, f.w0, false);
+    f.couple_vector();
+    m_turb->calc_turbulent_viscosity(f.nu_t, f.u, f.v, f.w, false);
+    m_dif->diffuse(f.u, f.u0, f.u_tmp, s.nu, f.nu_t, false);
+    m_counters.diffusion[0] = m_dif->last_iterations();
+    m_dif->diffuse(f.v, f.v0, f.v_tmp, s.nu, f.nu_t, false);
+    m_counters.diffusion[1] = m_dif->last_iterations();
+    m_dif->diffuse(f.w, f.w0, f.w_tmp, s.nu, f.nu_t, false);
+    m_counters.diffusion[2] = m_dif->last_iterations();
+    f.couple_vector();
+    if (m_add_source) {
+        m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
+        f.couple_vector();
+    }
+    m_pres->divergence(f.rhs, f.u_tmp, f.v_tmp, f.w_tmp, false);
+    pressure_step(t);
+    m_pres->projection(f.u, f.v, f.w, f.u_tmp, f.v_tmp, f.w_tmp, f.p, false);
+}
+
+// DiffusionSolver.cpp, AdvectionDiffusionSolver.cpp, DiffusionTurbSolver.cpp do_step
+void Solver::do_step_transport() {
+    const Settings &s = m_domain->settings();
+    FieldController &f = *m_fields;
+    if (m_kind == 6) {
+        m_adv->advect(f.u, f.u0, f.u0, f.v0, f.w0, false);
+        m_adv->advect(f.v, f.v0, f.u0, f.v0, f.w0, false);
+        m_adv->advect(f.w, f.w0, f.u0, f.v0, f.w0, false);
+        f.couple_vector();
+        if (s.nu == 0.) return;
+    }
+    if (m_kind == 7) m_turb->calc_turbulent_viscosity(f.nu_t, f.u, f.v, f.w, false);
+    Field *out[3] = {&f.u, &f.v, &f.w}, *in[3] = {&f.u0, &f.v0, &f.w0}, *b[3] = {&f.u_tmp, &f.v_tmp, &f.w_tmp};
+    for (int c = 0; c < 3; c++) {
+        if (m_kind == 7) m_dif->diffuse(*out[c], *in[c], *b[c], s.nu, f.nu_t, false);
+        else m_dif->diffuse(*out[c], *in[c], *b[c], s.nu, false);
+        m_counters.diffusion[c] = m_dif->last_iterations();
+    }
+}
+
 void Solver::step(double t) {
     m_counters = StepCounters{};
-    if (m_kind == 0) do_step_ns(t);
-    else if (m_kind == 1) do_step_nstempturb(t);
-    else pressure_step(t);
+    switch (m_kind) {
+        case 0: do_step_ns(t); break;
+        case 1: do_step_nstempturb(t); break;
+        case 2: pressure_step(t); break;
+        case 3: do_step_nstemp(t); break;
+        case 4: do_step_nsturb(t); break;
+        default: do_step_transport(); break;
+    }
     update_sources(t);
     m_fields->update_data();
 }

@@ -1,5 +1,7 @@
 // Solver drivers: FieldController (field/FieldController.cpp), NSSolver
 // (solver/NSSolver.cpp:50-124), NSTempTurbSolver (solver/NSTempTurbSolver.cpp:94-248),
+// NSTempSolver (solver/NSTempSolver.cpp:69-198), NSTurbSolver, DiffusionSolver,
+// AdvectionDiffusionSolver and DiffusionTurbSolver (their do_step),
 // SolverController setup and source updates (solver/SolverController.cpp:26-311)
 // and one TimeIntegration iteration (TimeIntegration.cpp:86-149).
 #pragma once
@@ -106,6 +108,9 @@ class Solver {
     void warm_end();
     void do_step_ns(double t);
     void do_step_nstempturb(double t);
+    void do_step_nstemp(double t);
+    void do_step_nsturb(double t);
+    void do_step_transport();
     void pressure_step(double t);
     void update_sources(double t);
     void set_up_fields();
@@ -122,7 +127,10 @@ class Solver {
     int m_gcr_coarse = 0;
     std::unique_ptr<ExplicitEulerSource> m_src_vel, m_src_temp;
     std::unique_ptr<GaussFunction> m_gauss;
-    int m_kind = 0;  // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver
+    // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver, 3 NSTempSolver,
+    // 4 NSTurbSolver, 5 DiffusionSolver, 6 AdvectionDiffusionSolver,
+    // 7 DiffusionTurbSolver
+    int m_kind = 0;
     bool m_add_source = false, m_add_temp_source = false;
     StepCounters m_counters;
     PressureMode m_pmode;

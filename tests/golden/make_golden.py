@@ -54,11 +54,37 @@ CASES = {
     "run_plume_explicit": dict(xml=lambda: cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02,
                                                        diffusion="Explicit", obstacles=OBST_ONE, hrr=5.0),
                                run=("1,2", "")),
+    "run_nstemp_dissipation": dict(xml=lambda: _nstemp_dissipation(), run=("1,2", "")),
+    "run_nsturb_tg": dict(xml=lambda: _tg_variant("NSTurbSolver", turbulence=True), run=("1,2", "taylor_green")),
+    "run_diffusion_tg": dict(xml=lambda: _tg_variant("DiffusionSolver"), run=("1,2", "taylor_green")),
+    "run_diffusion_turb_tg": dict(xml=lambda: _tg_variant("DiffusionTurbSolver", turbulence=True),
+                                  run=("1,2", "taylor_green")),
+    "run_advection_diffusion_tg": dict(xml=lambda: _tg_variant("AdvectionDiffusionSolver",
+                                                               diffusion="ColoredGaussSeidel"),
+                                       run=("1,2", "taylor_green")),
     "run_plume_cgs": dict(xml=lambda: cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02,
                                                   diffusion="ColoredGaussSeidel", smoother="ColoredGaussSeidel",
                                                   obstacles=OBST_ONE, hrr=5.0),
                           run=("1,2", "")),
 }
+
+
+def _tg_variant(solver, diffusion="Jacobi", turbulence=False):
+    """the Taylor-Green NSSolver input with another solver description (the
+    harness sets the Taylor-Green velocity, usr_fct stays "Zero")"""
+    x = cases.ns_taylor_green(n=16, levels=2, steps=5, dt=2e-3, reference=True, diffusion=diffusion)
+    x = x.replace('<solver description="NSSolver">', '<solver description="%s">' % solver)
+    if turbulence:
+        x = x.replace('<source type="ExplicitEuler"',
+                      '<turbulence type="ConstSmagorinsky"><Cs> 0.2 </Cs></turbulence><source type="ExplicitEuler"', 1)
+    return x
+
+
+def _nstemp_dissipation():
+    x = cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02, diffusion="Jacobi", obstacles=OBST_ONE, hrr=5.0)
+    x = x.replace('<solver description="NSTempTurbSolver">', '<solver description="NSTempSolver">')
+    x = x.replace('dissipation="No"', 'dissipation="Yes"')
+    return x
 
 
 def _bc_mixed(obstacles):
