@@ -4,7 +4,9 @@
 #include <utility>
 
 #include <cmath>
+#include <cstdlib>
 
+#include "functions.h"
 #include "krylov_kernels.h"
 
 namespace artss {
@@ -64,57 +66,6 @@ Field *FieldController::by_name(const std::string &n) {
     return nullptr;
 }
 
-// Functions.cpp initial conditions, evaluated with libm on the host on the
-// inner list; "TaylorGreen" has no reference counterpart (u = sin2πx cos2πy
-// cos2πz, v = -cos2πx sin2πy cos2πz, w = 0).  Returns a packed array of
-// three fields for TaylorGreen, one field otherwise.
-std::vector<double> initial_condition(const Domain &d, const std::string &what) {
-    const Settings &s = d.settings();
-    const LevelDesc &L = d.level(0);
-    const size_t n = static_cast<size_t>(L.size);
-    const std::vector<uint64_t> inner = d.inner_list(0);
-    const double X1 = s.X[0], Y1 = s.Y[0], Z1 = s.Z[0];
-    const double dx = L.h[0], dy = L.h[1], dz = L.h[2];
-    auto coords = [&](uint64_t idx, size_t &i, size_t &j, size_t &k) {
-        k = idx / (static_cast<size_t>(L.N[0]) * L.N[1]);
-        j = (idx - k * L.N[0] * L.N[1]) / L.N[0];
-        i = idx - k * L.N[0] * L.N[1] - j * L.N[0];
-    };
-    if (what == "TaylorGreen") {
-        std::vector<double> out(3 * n, 0.0);
-        for (uint64_t idx : inner) {
-            size_t i, j, k;
-            coords(idx, i, j, k);
-            const double x = X1 + (static_cast<double>(i) - 0.5) * dx;
-            const double y = Y1 + (static_cast<double>(j) - 0.5) * dy;
-            const double z = Z1 + (static_cast<double>(k) - 0.5) * dz;
-            out[idx] = std::sin(2. * M_PI * x) * std::cos(2. * M_PI * y) * std::cos(2. * M_PI * z);
-            out[n + idx] = -std::cos(2. * M_PI * x) * std::sin(2. * M_PI * y) * std::cos(2. * M_PI * z);
-            out[2 * n + idx] = 0.;
-        }
-        return out;
-    }
-    std::vector<double> out(n, 0.0);
-    if (what == "Uniform") {  // Functions.cpp:820-832
-        auto it = s.ic_values.find("val");
-        if (it == s.ic_values.end()) throw config_error("Uniform initial condition needs <val>");
-        const double val = std::strtod(it->second.c_str(), nullptr);
-        for (uint64_t idx : inner) out[idx] = val;
-    } else if (what == "SinSinSin") {  // Functions.cpp:781-811
-        auto it = s.ic_values.find("l");
-        if (it == s.ic_values.end()) throw config_error("SinSinSin initial condition needs <l>");
-        const double l = std::strtod(it->second.c_str(), nullptr);
-        for (uint64_t idx : inner) {
-            size_t i, j, k;
-            coords(idx, i, j, k);
-            out[idx] = std::sin(l * M_PI * (X1 + (static_cast<double>(i) - 0.5) * dx)) *
-                       std::sin(l * M_PI * (Y1 + (static_cast<double>(j) - 0.5) * dy)) *
-                       std::sin(l * M_PI * (Z1 + (static_cast<double>(k) - 0.5) * dz));
-        }
-    }
-    return out;
-}
-
 namespace {
 void upload(Domain &d, double *dst, const double *src, size_t n) {
     ARTSS_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, d.stream()));
@@ -126,14 +77,17 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     const Settings &s = d.settings();
     static const char *kinds[] = {"NSSolver",        "NSTempTurbSolver",         "PressureSolver",
                                   "NSTempSolver",    "NSTurbSolver",             "DiffusionSolver",
-                                  "AdvectionDiffusionSolver", "DiffusionTurbSolver"};
+                                  "AdvectionDiffusionSolver", "DiffusionTurbSolver", "AdvectionSolver"};
     m_kind = -1;
-    for (int k = 0; k < 8; k++)
+    for (int k = 0; k < 9; k++)
         if (s.solver == kinds[k]) m_kind = k;
     if (m_kind < 0)
         throw config_error("solver '" + s.solver + "' is not part of this build (NSSolver, NSTurbSolver, NSTempSolver, "
                            "NSTempTurbSolver, PressureSolver, DiffusionSolver, DiffusionTurbSolver, "
-                           "AdvectionDiffusionSolver)");
+                           "AdvectionDiffusionSolver, AdvectionSolver)");
+    if (m_kind == 8 && s.ic != "GaussBubble")
+        throw config_error("AdvectionSolver takes its velocity from the GaussBubble initial condition "
+                           "(SolverController.cpp:54-56)");
     if (s.has_source && s.force_fct == "Buoyancy" && s.use_init_values)
         throw config_error("buoyancy with use_init_values is not supported");
     if (s.has_source && s.force_fct == "Uniform") throw config_error("uniform momentum source is not supported");
@@ -148,8 +102,8 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     // the solver constructors (SolverSelection::set_*): NSSolver.cpp:17-36,
     // NSTurbSolver.cpp, NSTempSolver.cpp:16-49, NSTempTurbSolver.cpp,
     // DiffusionSolver.cpp, AdvectionDiffusionSolver.cpp, DiffusionTurbSolver.cpp
-    const bool velocity_adv = m_kind == 0 || m_kind == 1 || m_kind == 3 || m_kind == 4 || m_kind == 6;
-    const bool velocity_dif = m_kind != 2;
+    const bool velocity_adv = m_kind == 0 || m_kind == 1 || m_kind == 3 || m_kind == 4 || m_kind == 6 || m_kind == 8;
+    const bool velocity_dif = m_kind != 2 && m_kind != 8;
     const bool ns = m_kind == 0 || m_kind == 1 || m_kind == 3 || m_kind == 4;
     if (velocity_adv) m_adv = make_advection(d, s.advection_type);
     if (velocity_dif) m_dif = make_diffusion(d, s.diffusion);
@@ -159,6 +113,15 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     }
     if (ns || m_kind == 2) m_pres = std::make_unique<VCycleMG>(d, s.pressure);
     if (m_kind == 1 || m_kind == 4 || m_kind == 7) m_turb = make_turbulence(d, s);
+    if (m_kind == 8) {
+        // AdvectionSolver.cpp:16-34: constant transport velocity from the GaussBubble parameters
+        const char *keys[3] = {"u_lin", "v_lin", "w_lin"};
+        for (int a = 0; a < 3; a++) {
+            auto it = s.ic_values.find(keys[a]);
+            if (it == s.ic_values.end()) throw config_error(std::string("GaussBubble needs <") + keys[a] + ">");
+            m_lin[a] = std::make_unique<Field>(d, U + a, std::strtod(it->second.c_str(), nullptr));
+        }
+    }
     if (m_kind == 1 || m_kind == 3) {
         if (s.temp_advection_type != "SemiLagrangian") throw config_error("temperature advection must be SemiLagrangian");
         m_dif_temp = make_diffusion(d, s.diffusion);  // both solvers use the velocity settings (NSTempSolver.cpp:33)
@@ -261,25 +224,17 @@ void Solver::set_up_fields() {
     const Settings &s = d.settings();
     FieldController &f = *m_fields;
     const size_t n = static_cast<size_t>(d.level(0).size);
-    if (s.ic == "TaylorGreen") {
-        std::vector<double> ic = initial_condition(d, "TaylorGreen");
-        upload(d, f.u.data, ic.data(), n);
-        upload(d, f.v.data, ic.data() + n, n);
-        upload(d, f.w.data, ic.data() + 2 * n, n);
-        ARTSS_HIP(hipStreamSynchronize(d.stream()));
-    } else if (s.ic == "Uniform") {
-        std::vector<double> ic = initial_condition(d, "Uniform");
-        upload(d, f.T.data, ic.data(), n);
-        ARTSS_HIP(hipStreamSynchronize(d.stream()));
-    } else if (s.ic == "SinSinSin") {
-        std::vector<double> ic = initial_condition(d, "SinSinSin");
-        f.p.set_value(0.);
-        upload(d, f.rhs.data, ic.data(), n);
-        ARTSS_HIP(hipStreamSynchronize(d.stream()));
-    } else if (s.ic != "Zero") {
-        throw config_error("initial condition '" + s.ic + "' is not supported");
+    // SolverController::set_up_fields (SolverController.cpp:94-196)
+    const InitialFields ic = initial_fields(d);
+    for (const auto &kv : ic.arrays) {
+        Field *fld = f.by_name(kv.first);
+        if (fld == nullptr) throw config_error("no field '" + kv.first + "' for the initial condition");
+        upload(d, fld->data, kv.second.data(), n);
     }
-    if (m_kind == 1 && s.force_fct == "Buoyancy") f.T_ambient.set_value(s.ambient_temperature);  // force_source
+    ARTSS_HIP(hipStreamSynchronize(d.stream()));
+    for (const auto &kv : ic.constants) f.by_name(kv.first)->set_value(kv.second);
+    // SolverController::force_source (SolverController.cpp:197-202, 229-236) for the temperature solvers
+    if ((m_kind == 1 || m_kind == 3) && s.force_fct == "Buoyancy") f.T_ambient.set_value(s.ambient_temperature);
     // FieldController::set_up_boundary, FieldController.cpp:101-112
     d.apply_boundary(f.u.data, U, 0);
     d.apply_boundary(f.v.data, V, 0);
@@ -474,10 +429,17 @@ void Solver::do_step_nsturb(double t) {
     m_pres->projection(f.u, f.v, f.w, f.u_tmp, f.v_tmp, f.w_tmp, f.p, false);
 }
 
-// DiffusionSolver.cpp, AdvectionDiffusionSolver.cpp, DiffusionTurbSolver.cpp do_step
+// DiffusionSolver.cpp, AdvectionDiffusionSolver.cpp, DiffusionTurbSolver.cpp
+// and AdvectionSolver.cpp do_step
 void Solver::do_step_transport() {
     const Settings &s = m_domain->settings();
     FieldController &f = *m_fields;
+    if (m_kind == 8) {
+        m_adv->advect(f.u, f.u0, *m_lin[0], *m_lin[1], *m_lin[2], false);
+        m_adv->advect(f.v, f.v0, *m_lin[0], *m_lin[1], *m_lin[2], false);
+        m_adv->advect(f.w, f.w0, *m_lin[0], *m_lin[1], *m_lin[2], false);
+        return;
+    }
     if (m_kind == 6) {
         m_adv->advect(f.u, f.u0, f.u0, f.v0, f.w0, false);
         m_adv->advect(f.v, f.v0, f.u0, f.v0, f.w0, false);

@@ -1,7 +1,7 @@
 // Solver drivers: FieldController (field/FieldController.cpp), NSSolver
 // (solver/NSSolver.cpp:50-124), NSTempTurbSolver (solver/NSTempTurbSolver.cpp:94-248),
 // NSTempSolver (solver/NSTempSolver.cpp:69-198), NSTurbSolver, DiffusionSolver,
-// AdvectionDiffusionSolver and DiffusionTurbSolver (their do_step),
+// AdvectionDiffusionSolver, DiffusionTurbSolver and AdvectionSolver (their do_step),
 // SolverController setup and source updates (solver/SolverController.cpp:26-311)
 // and one TimeIntegration iteration (TimeIntegration.cpp:86-149).
 #pragma once
@@ -129,8 +129,9 @@ class Solver {
     std::unique_ptr<GaussFunction> m_gauss;
     // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver, 3 NSTempSolver,
     // 4 NSTurbSolver, 5 DiffusionSolver, 6 AdvectionDiffusionSolver,
-    // 7 DiffusionTurbSolver
+    // 7 DiffusionTurbSolver, 8 AdvectionSolver
     int m_kind = 0;
+    std::unique_ptr<Field> m_lin[3];  // AdvectionSolver's constant velocity
     bool m_add_source = false, m_add_temp_source = false;
     StepCounters m_counters;
     PressureMode m_pmode;
@@ -142,7 +143,5 @@ class Solver {
     float *m_warm_x = nullptr, *m_warm_y = nullptr, *m_warm_t = nullptr;
 };
 
-// initial conditions on the inner list (Functions.cpp) for the supported usr_fct values
-std::vector<double> initial_condition(const Domain &d, const std::string &what);
 
 }  // namespace artss

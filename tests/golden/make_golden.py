@@ -62,6 +62,31 @@ CASES = {
     "run_advection_diffusion_tg": dict(xml=lambda: _tg_variant("AdvectionDiffusionSolver",
                                                                diffusion="ColoredGaussSeidel"),
                                        run=("1,2", "taylor_green")),
+    # runs from the reference's own initial conditions (SolverController.cpp:94-196)
+    "run_advection_gauss": dict(xml=lambda: _ic_case(
+        "AdvectionSolver", ("GaussBubble", "<u_lin> 0.5 </u_lin><v_lin> 0.25 </v_lin><w_lin> -0.5 </w_lin>"
+                                           "<x_shift> 0.4 </x_shift><y_shift> 0.5 </y_shift><z_shift> 0.55 </z_shift>"
+                                           "<l> 0.125 </l>"), _PERIODIC, dt=0.01), run=("1,2", "")),
+    "run_diffusion_expsinusprod": dict(xml=lambda: _ic_case(
+        "DiffusionSolver", ("ExpSinusProd", "<l> 2.0 </l>"), _DIRICHLET0, nu=0.05), run=("1,2", "")),
+    "run_diffusion_hat": dict(xml=lambda: _ic_case(
+        "DiffusionSolver", ("Hat", "<x1> 0.25 </x1><x2> 0.625 </x2><y1> 0.3 </y1><y2> 0.7 </y2><z1> 0.125 </z1>"
+                                   "<z2> 0.75 </z2><val_in> 2.0 </val_in><val_out> 1.0 </val_out>"),
+        _DIRICHLET0, diffusion="ColoredGaussSeidel"), run=("1,2", "")),
+    "run_burgers_expsinussum": dict(xml=lambda: _ic_case(
+        "AdvectionDiffusionSolver", ("ExpSinusSum", ""), _PERIODIC, X=(0.0, 6.283185307179586),
+        Y=(0.0, 6.283185307179586), Z=(0.0, 6.283185307179586)), run=("1,2", "")),
+    "run_ns_mcdermott": dict(xml=lambda: _ic_case(
+        "NSSolver", ("McDermott", "<A> 2.0 </A>"), _PERIODIC, X=(-3.141592653589793, 3.141592653589793),
+        Y=(-3.141592653589793, 3.141592653589793), Z=(0.0, 1.0), n=(16, 16, 8)), run=("1,2", "")),
+    "run_ns_vortex": dict(xml=lambda: _ic_case(
+        "NSSolver", ("Vortex", "<u_lin> 0.1 </u_lin><v_lin> 0.0 </v_lin><w_lin> 0.05 </w_lin><pa> 0.0 </pa>"
+                               "<rhoa> 1.0 </rhoa>"), _PERIODIC, X=(-0.1556, 0.1556), Y=(-0.1556, 0.1556),
+        Z=(0.0, 0.0389), n=(16, 16, 8), nu=1e-4, dt=0.005), run=("1,2", "")),
+    "run_ns_drift": dict(xml=lambda: _ic_case(
+        "NSSolver", ("Drift", "<u_lin> 0.2 </u_lin><v_lin> -0.1 </v_lin><w_lin> 0.05 </w_lin><pa> 0.3 </pa>"),
+        _PERIODIC, obstacles=OBST_ONE), run=("1,2", "")),
+    "run_nstemp_layers": dict(xml=lambda: _nstemp_layers(), run=("1,2", "")),
     "run_plume_cgs": dict(xml=lambda: cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02,
                                                   diffusion="ColoredGaussSeidel", smoother="ColoredGaussSeidel",
                                                   obstacles=OBST_ONE, hrr=5.0),
@@ -85,6 +110,36 @@ def _nstemp_dissipation():
     x = x.replace('<solver description="NSTempTurbSolver">', '<solver description="NSTempSolver">')
     x = x.replace('dissipation="No"', 'dissipation="Yes"')
     return x
+
+
+def _ic_case(solver, ic, bnd, X=(0.0, 1.0), Y=(0.0, 1.0), Z=(0.0, 1.0), n=(16, 16, 16), diffusion="Jacobi",
+             nu=0.01, dt=2e-3, steps=5, levels=2, obstacles=(), extra=""):
+    """a run from one of the reference's own initial conditions (Functions.cpp)"""
+    phys = '<t_end> {!r} </t_end><dt> {!r} </dt><nu> {!r} </nu>'.format(steps * dt, dt, nu)
+    body = '<advection type="SemiLagrangian" field="u,v,w"></advection>'
+    if solver != "AdvectionSolver":
+        body += cases.diffusion_xml(diffusion, "u,v,w", 100, 1e-7, 1.0)
+    if solver in ("NSSolver", "NSTurbSolver"):
+        body += ('<source type="ExplicitEuler" force_fct="Zero" dir="x"></source>' +
+                 cases.pressure_xml(levels))
+    body += extra
+    return cases._wrap(phys, (solver, body), cases.domain_xml(X, Y, Z, n), bnd, cases.obstacles_xml(list(obstacles)),
+                       '<initial_conditions usr_fct="{}" random="No">{}</initial_conditions>'.format(*ic))
+
+
+_PERIODIC = '<boundary field="u,v,w,p" patch="front,back,left,right,bottom,top" type="periodic" />'
+_DIRICHLET0 = '<boundary field="u,v,w,p" patch="front,back,left,right,bottom,top" type="dirichlet" value="0.0" />'
+
+
+def _nstemp_layers():
+    x = cases.plume(n=(16, 16, 16), levels=2, steps=5, dt=0.02, diffusion="Jacobi", obstacles=OBST_ONE, hrr=5.0)
+    x = x.replace('<solver description="NSTempTurbSolver">', '<solver description="NSTempSolver">')
+    start = x.index("<initial_conditions")
+    end = x.index("</initial_conditions>") + len("</initial_conditions>")
+    ic = ('<initial_conditions usr_fct="LayersT" random="No"><n_layers> 3 </n_layers><dir>y</dir>'
+          '<border_1> 0.3 </border_1><border_2> 0.59375 </border_2><value_1> 293.15 </value_1>'
+          '<value_2> 296.0 </value_2><value_3> 300.5 </value_3></initial_conditions>')
+    return x[:start] + ic + x[end:]
 
 
 def _bc_mixed(obstacles):

@@ -154,7 +154,10 @@ def test_smagorinsky_sources_golden(ops_jacobi):
 
 @pytest.mark.parametrize("case", ["run_ns_tg", "run_plume_jacobi", "run_plume_cgs", "run_plume_explicit",
                                   "run_nstemp_dissipation", "run_nsturb_tg", "run_diffusion_tg",
-                                  "run_diffusion_turb_tg", "run_advection_diffusion_tg"])
+                                  "run_diffusion_turb_tg", "run_advection_diffusion_tg",
+                                  "run_advection_gauss", "run_diffusion_expsinusprod", "run_diffusion_hat",
+                                  "run_burgers_expsinussum", "run_ns_mcdermott", "run_ns_vortex", "run_ns_drift",
+                                  "run_nstemp_layers"])
 def test_solver_golden(case):
     z, xml, cfg = load(case)
     # the Taylor-Green runs: the reference harness set the velocity itself
