@@ -86,6 +86,11 @@ def lib():
         "artss_solver_set_pressure_method": (C.c_int, [vp, C.c_int, C.c_int]),
         "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
         "artss_solver_set_warm_start": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "artss_solver_snapshot": (C.c_int, [vp, C.c_double, ip]),
+        "artss_solver_restore": (C.c_int, [vp, C.c_int, dp]),
+        "artss_solver_drop_snapshot": (C.c_int, [vp, C.c_int]),
+        "artss_solver_save": (C.c_int, [vp, C.c_char_p, C.c_double]),
+        "artss_solver_load": (C.c_int, [vp, C.c_char_p, dp]),
         "artss_ensemble_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
         "artss_ensemble_destroy": (C.c_int, [vp]),
         "artss_ensemble_add": (C.c_int, [vp, vp]),
@@ -434,6 +439,29 @@ class Solver:
         ok = C.c_int()
         _check(self.L.artss_solver_warm_sample(self.h, x_buf.ptr, t_buf.ptr, C.byref(ok)))
         return bool(ok.value)
+
+    def snapshot(self, t):
+        """Keeps the state between two steps in HBM; returns its slot."""
+        slot = C.c_int()
+        _check(self.L.artss_solver_snapshot(self.h, t, C.byref(slot)))
+        return slot.value
+
+    def restore(self, slot):
+        """Rolls back to a snapshot; returns its time."""
+        t = C.c_double()
+        _check(self.L.artss_solver_restore(self.h, int(slot), C.byref(t)))
+        return t.value
+
+    def drop_snapshot(self, slot):
+        _check(self.L.artss_solver_drop_snapshot(self.h, int(slot)))
+
+    def save(self, path, t):
+        _check(self.L.artss_solver_save(self.h, os.fspath(path).encode(), t))
+
+    def load(self, path):
+        t = C.c_double()
+        _check(self.L.artss_solver_load(self.h, os.fspath(path).encode(), C.byref(t)))
+        return t.value
 
     def set_mg_schedule(self, mode=2, tail_cells=4096):
         _check(self.L.artss_solver_set_mg_schedule(self.h, int(mode), int(tail_cells)))

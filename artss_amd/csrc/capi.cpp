@@ -560,6 +560,48 @@ int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max
     });
 }
 
+int artss_solver_snapshot(artss_solver *s, double t, int *slot) {
+    return guard([&] {
+        require(s && slot, "null argument");
+        *slot = s->solver->snapshot(t);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_restore(artss_solver *s, int slot, double *t) {
+    return guard([&] {
+        require(s, "null argument");
+        const double tt = s->solver->restore(slot);
+        if (t) *t = tt;
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_drop_snapshot(artss_solver *s, int slot) {
+    return guard([&] {
+        require(s, "null argument");
+        s->solver->drop_snapshot(slot);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_save(artss_solver *s, const char *path, double t) {
+    return guard([&] {
+        require(s && path, "null argument");
+        s->solver->save(path, t);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_load(artss_solver *s, const char *path, double *t) {
+    return guard([&] {
+        require(s && path, "null argument");
+        const double tt = s->solver->load(path);
+        if (t) *t = tt;
+        return ARTSS_OK;
+    });
+}
+
 int artss_ensemble_create(int threads, artss_ensemble **out) {
     return guard([&] {
         require(out && threads >= 1, "bad argument");

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include "functions.h"
 #include "krylov_kernels.h"
@@ -39,6 +40,14 @@ void FieldController::update_data() {
     w_tmp.copy_data(w);
     T0.copy_data(T);
     T_tmp.copy_data(T);
+}
+
+std::vector<std::pair<const char *, Field *>> FieldController::all() {
+    return {{"u", &u},         {"v", &v},         {"w", &w},         {"u0", &u0},   {"v0", &v0},
+            {"w0", &w0},       {"u_tmp", &u_tmp}, {"v_tmp", &v_tmp}, {"w_tmp", &w_tmp},
+            {"nu_t", &nu_t},   {"kappa_t", &kappa_t}, {"p", &p},     {"rhs", &rhs}, {"T", &T},
+            {"T0", &T0},       {"T_tmp", &T_tmp}, {"T_ambient", &T_ambient}, {"f_x", &f_x},
+            {"f_y", &f_y},     {"f_z", &f_z},     {"S_T", &S_T}};
 }
 
 Field *FieldController::by_name(const std::string &n) {
@@ -134,6 +143,9 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
 
 Solver::~Solver() {
     if (m_domain) (void)hipStreamSynchronize(m_domain->stream());
+    for (auto &snap : m_snapshots)
+        if (snap)
+            for (double *q : snap->data) (void)hipFree(q);
     for (void *q : {static_cast<void *>(m_p_prev), static_cast<void *>(m_p_start),
                     static_cast<void *>(m_warm_partial), static_cast<void *>(m_warm_sum),
                     static_cast<void *>(m_warm_x), static_cast<void *>(m_warm_y), static_cast<void *>(m_warm_t)})
@@ -468,6 +480,154 @@ void Solver::step(double t) {
     }
     update_sources(t);
     m_fields->update_data();
+}
+
+int Solver::snapshot(double t) {
+    Domain &d = *m_domain;
+    const size_t n = static_cast<size_t>(d.level(0).size);
+    auto snap = std::make_unique<Snapshot>();
+    snap->t = t;
+    auto fields = m_fields->all();
+    try {
+        for (auto &kv : fields) {
+            double *q = nullptr;
+            ARTSS_HIP(hipMalloc(&q, sizeof(double) * n));
+            snap->data.push_back(q);
+            ARTSS_HIP(hipMemcpyAsync(q, kv.second->data, sizeof(double) * n, hipMemcpyDeviceToDevice, d.stream()));
+        }
+        if (m_p_prev) {
+            double *q = nullptr;
+            ARTSS_HIP(hipMalloc(&q, sizeof(double) * n));
+            snap->data.push_back(q);
+            ARTSS_HIP(hipMemcpyAsync(q, m_p_prev, sizeof(double) * n, hipMemcpyDeviceToDevice, d.stream()));
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(d.stream());
+        for (double *q : snap->data) (void)hipFree(q);
+        throw;
+    }
+    snap->warm_history = m_warm_history;
+    ARTSS_HIP(hipStreamSynchronize(d.stream()));
+    for (size_t i = 0; i < m_snapshots.size(); i++)
+        if (!m_snapshots[i]) {
+            m_snapshots[i] = std::move(snap);
+            return static_cast<int>(i);
+        }
+    m_snapshots.push_back(std::move(snap));
+    return static_cast<int>(m_snapshots.size()) - 1;
+}
+
+double Solver::restore(int slot) {
+    if (slot < 0 || slot >= static_cast<int>(m_snapshots.size()) || !m_snapshots[static_cast<size_t>(slot)])
+        throw config_error("no snapshot in slot " + std::to_string(slot));
+    Domain &d = *m_domain;
+    const size_t n = static_cast<size_t>(d.level(0).size);
+    const Snapshot &snap = *m_snapshots[static_cast<size_t>(slot)];
+    auto fields = m_fields->all();
+    for (size_t i = 0; i < fields.size(); i++)
+        ARTSS_HIP(hipMemcpyAsync(fields[i].second->data, snap.data[i], sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                 d.stream()));
+    if (m_p_prev && snap.data.size() > fields.size()) {
+        ARTSS_HIP(hipMemcpyAsync(m_p_prev, snap.data[fields.size()], sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                 d.stream()));
+        m_warm_history = snap.warm_history;
+    } else {
+        m_warm_history = 0;  // no previous solution in the snapshot: the next solve starts plainly
+    }
+    m_warm_sample = false;
+    ARTSS_HIP(hipStreamSynchronize(d.stream()));
+    return snap.t;
+}
+
+void Solver::drop_snapshot(int slot) {
+    if (slot < 0 || slot >= static_cast<int>(m_snapshots.size()) || !m_snapshots[static_cast<size_t>(slot)]) return;
+    (void)hipStreamSynchronize(m_domain->stream());
+    for (double *q : m_snapshots[static_cast<size_t>(slot)]->data) (void)hipFree(q);
+    m_snapshots[static_cast<size_t>(slot)].reset();
+}
+
+namespace {
+constexpr char kCheckpointMagic[8] = {'A', 'R', 'T', 'S', 'S', 'H', 'I', 'P'};
+constexpr uint32_t kCheckpointVersion = 1;
+}  // namespace
+
+void Solver::save(const std::string &path, double t) {
+    Domain &d = *m_domain;
+    const LevelDesc &L = d.level(0);
+    const size_t n = static_cast<size_t>(L.size);
+    std::ofstream out(path, std::ios::binary | std::ios::trunc);
+    if (!out) throw config_error("cannot write checkpoint '" + path + "'");
+    auto fields = m_fields->all();
+    const bool warm = m_p_prev != nullptr;
+    std::vector<std::string> names;
+    for (auto &kv : fields) names.push_back(kv.first);
+    if (warm) names.push_back("p_prev");
+    const uint32_t header[6] = {kCheckpointVersion, static_cast<uint32_t>(L.N[0]), static_cast<uint32_t>(L.N[1]),
+                                static_cast<uint32_t>(L.N[2]), static_cast<uint32_t>(names.size()),
+                                static_cast<uint32_t>(m_warm_history)};
+    const double times[2] = {t, d.settings().dt};
+    out.write(kCheckpointMagic, sizeof(kCheckpointMagic));
+    out.write(reinterpret_cast<const char *>(header), sizeof(header));
+    out.write(reinterpret_cast<const char *>(times), sizeof(times));
+    for (const std::string &nm : names) {
+        const uint32_t len = static_cast<uint32_t>(nm.size());
+        out.write(reinterpret_cast<const char *>(&len), sizeof(len));
+        out.write(nm.data(), len);
+    }
+    std::vector<double> host(n);
+    for (size_t i = 0; i < names.size(); i++) {
+        const double *src = i < fields.size() ? fields[i].second->data : m_p_prev;
+        ARTSS_HIP(hipMemcpy(host.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost));
+        out.write(reinterpret_cast<const char *>(host.data()), static_cast<std::streamsize>(sizeof(double) * n));
+    }
+    if (!out) throw config_error("writing checkpoint '" + path + "' failed");
+}
+
+double Solver::load(const std::string &path) {
+    Domain &d = *m_domain;
+    const LevelDesc &L = d.level(0);
+    const size_t n = static_cast<size_t>(L.size);
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw config_error("cannot read checkpoint '" + path + "'");
+    char magic[8];
+    uint32_t header[6];
+    double times[2];
+    in.read(magic, sizeof(magic));
+    in.read(reinterpret_cast<char *>(header), sizeof(header));
+    in.read(reinterpret_cast<char *>(times), sizeof(times));
+    if (!in || std::memcmp(magic, kCheckpointMagic, sizeof(magic)) != 0 || header[0] != kCheckpointVersion)
+        throw config_error("'" + path + "' is not a checkpoint of this format");
+    if (header[1] != static_cast<uint32_t>(L.N[0]) || header[2] != static_cast<uint32_t>(L.N[1]) ||
+        header[3] != static_cast<uint32_t>(L.N[2]))
+        throw config_error("checkpoint '" + path + "' is for another grid");
+    std::vector<std::string> names(header[4]);
+    for (std::string &nm : names) {
+        uint32_t len = 0;
+        in.read(reinterpret_cast<char *>(&len), sizeof(len));
+        if (!in || len > 64) throw config_error("corrupt checkpoint '" + path + "'");
+        nm.resize(len);
+        in.read(&nm[0], len);
+    }
+    std::vector<double> host(n);
+    bool have_prev = false;
+    for (const std::string &nm : names) {
+        in.read(reinterpret_cast<char *>(host.data()), static_cast<std::streamsize>(sizeof(double) * n));
+        if (!in) throw config_error("checkpoint '" + path + "' is truncated");
+        double *dst = nullptr;
+        if (nm == "p_prev") {
+            if (!m_p_prev) continue;  // no warm start set up here: nothing to continue
+            dst = m_p_prev;
+            have_prev = true;
+        } else {
+            Field *f = m_fields->by_name(nm);
+            if (f == nullptr) throw config_error("checkpoint '" + path + "' holds an unknown field '" + nm + "'");
+            dst = f->data;
+        }
+        ARTSS_HIP(hipMemcpy(dst, host.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    m_warm_history = have_prev ? static_cast<int>(header[5]) : 0;
+    m_warm_sample = false;
+    return times[0];
 }
 
 int Solver::write_dat(const std::string &dir) {

@@ -31,6 +31,8 @@ struct FieldController {
     // FieldController.cpp:119-135 (concentration fields are not part of this path)
     void update_data();
     Field *by_name(const std::string &name);
+    // every field, in a fixed order, with its name (checkpoints)
+    std::vector<std::pair<const char *, Field *>> all();
 };
 
 // Counters of the last time step: diffusion iterations per component and
@@ -103,7 +105,27 @@ class Solver {
     // (V = inner box cells, x fastest); false when none is recorded yet
     bool warm_sample(float *x, float *t);
 
+    // ---- checkpoint / rollback (the reference's FieldIO and DataAssimilation
+    // rollback, dataAssimilation/FieldIO.cpp:44-100, DataAssimilation.cpp:56-161)
+    // The state between two steps is every FieldController field plus the
+    // warm start's previous solution; restoring it continues a run bit for
+    // bit as if it had not been interrupted.
+    // in HBM: returns a slot id; restore(slot) rolls the solver back to it
+    int snapshot(double t);
+    double restore(int slot);
+    void drop_snapshot(int slot);
+    // on disk: one file per call, a header (magic, version, cell counts,
+    // time, time step, field names) followed by the raw FP64 arrays
+    void save(const std::string &path, double t);
+    double load(const std::string &path);
+
   private:
+    struct Snapshot {
+        double t = 0.0;
+        std::vector<double *> data;  // device copies, in FieldController::all() order, then p_prev
+        int warm_history = 0;
+    };
+    std::vector<std::unique_ptr<Snapshot>> m_snapshots;
     void warm_begin();
     void warm_end();
     void do_step_ns(double t);

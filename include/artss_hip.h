@@ -177,6 +177,21 @@ int artss_solver_warm_sample(artss_solver *s, float *x, float *t, int *available
 /* ||rhs - lap p||_2 at the start of the last step's GCR pressure iteration */
 int artss_solver_initial_residual(artss_solver *s, double *residual);
 
+/* ---- checkpoint and rollback (dataAssimilation/FieldIO.cpp:44-100 writes
+ * u, v, w, p, T, C per time to HDF5, DataAssimilation.cpp:56-161 rolls a run
+ * back to such a state).  The state between two steps is every field of the
+ * solver plus the warm start's previous pressure; restoring it continues the
+ * run bit for bit.  Snapshots stay in HBM (slot ids, freed by
+ * artss_solver_drop_snapshot or with the solver); save/load write one file
+ * of this library's own format (magic, version, cell counts, time, dt, field
+ * names, raw FP64 arrays; HDF5 is not part of this build) and load checks
+ * that the grid matches. */
+int artss_solver_snapshot(artss_solver *s, double t, int *slot);
+int artss_solver_restore(artss_solver *s, int slot, double *t);
+int artss_solver_drop_snapshot(artss_solver *s, int slot);
+int artss_solver_save(artss_solver *s, const char *path, double t);
+int artss_solver_load(artss_solver *s, const char *path, double *t);
+
 /* ---- ensembles (no reference counterpart: the reference advances one
  * domain per process).  Independent solvers of any sizes, each on its own
  * stream, advanced together: artss_ensemble_step runs Solver::step(step *
