@@ -666,6 +666,17 @@ int artss_solver_initial_residual(artss_solver *s, double *residual) {
     });
 }
 
+int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post) {
+    return guard([&] {
+        require(s && pre >= 0 && post >= 0, "bad argument");
+        PressureMode m = s->solver->pressure_mode();
+        m.mixed_pre = pre;
+        m.mixed_post = post;
+        s->solver->set_pressure_mode(m);
+        return ARTSS_OK;
+    });
+}
+
 int artss_solver_set_mg_schedule(artss_solver *s, int mode, long long tail_cells) {
     return guard([&] {
         require(s && tail_cells >= 0 && mode >= 0 && mode <= 2, "bad argument");

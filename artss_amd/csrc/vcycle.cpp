@@ -44,7 +44,7 @@ bool VCycleMG::mixed_supported() const {
 }
 
 void VCycleMG::free_mixed() {
-    for (auto *v : {&m_err1f, &m_tmpf, &m_res0f, &m_res1f}) {
+    for (auto *v : {&m_err1f, &m_tmpf, &m_scrf, &m_res0f, &m_res1f}) {
         for (float *p : *v)
             if (p) (void)hipFree(p);
         v->clear();
@@ -64,30 +64,52 @@ void VCycleMG::alloc_mixed() {
     for (int l = 0; l < f; l++) {
         m_err1f.push_back(alloc(l));
         m_tmpf.push_back(alloc(l));
+        m_scrf.push_back(alloc(l));
         m_res0f.push_back(alloc(l));
         m_res1f.push_back(l > 0 ? alloc(l) : nullptr);
     }
 }
 
 // smooth_fast in FP32: tmpf -> err1f -> ... -> err1f, linear part of the fill
-void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32) {
+// `sweeps` weighted-Jacobi sweeps from tmp into err1 (0: the XML's
+// effective count).  An even count starts with one sweep into a third
+// buffer, so the result always lands in err1; the first sweep into a buffer
+// copies the cells it does not update, so err1 holds tmp's values there.
+void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int sweeps) {
     Domain &d = m_domain;
     double ax, ay, az, beta;
     coeffs(level, &ax, &ay, &az, &beta);
     const LevelDesc &Ld = d.level(level);
     const BCView V = d.bc_view(level, P, true);
-    const int n = m_s.n_relax;
-    const int effective = (n % 2 == 0 && n > 0) ? n - 1 : n;
-    float *src = m_tmpf[level], *dst = m_err1f[level];
-    for (int i = 0; i < effective; i++) {
+    const int nx = m_s.n_relax;
+    const int n = sweeps > 0 ? sweeps : ((nx % 2 == 0 && nx > 0) ? nx - 1 : nx);
+    auto sweep = [&](bool first, float *dst, const float *src) {
         d.timed("jacobi_mg32", level, [&] {
-            if (b64) k::jacobi_bc<float, double>(d.stream(), Ld, V, i == 0, dst, src, b64, ax, ay, az, beta, kDsign,
+            if (b64) k::jacobi_bc<float, double>(d.stream(), Ld, V, first, dst, src, b64, ax, ay, az, beta, kDsign,
                                                  m_s.smoother.w);
-            else k::jacobi_bc<float, float>(d.stream(), Ld, V, i == 0, dst, src, b32, ax, ay, az, beta, kDsign,
+            else k::jacobi_bc<float, float>(d.stream(), Ld, V, first, dst, src, b32, ax, ay, az, beta, kDsign,
                                             m_s.smoother.w);
         });
+    };
+    float *src = m_tmpf[level];
+    int m = n;
+    if (n % 2 == 0 && n > 0) {
+        sweep(true, m_scrf[level], src);
+        src = m_scrf[level];
+        m = n - 1;
+    }
+    float *dst = m_err1f[level];
+    for (int i = 0; i < m; i++) {
+        sweep(i == 0, dst, src);
         std::swap(src, dst);
     }
+}
+
+void VCycleMG::set_mixed_sweeps(int pre, int post) {
+    if (pre < 0 || post < 0) throw config_error("sweep counts must not be negative");
+    drop_graphs();
+    m_mixed_pre = pre;
+    m_mixed_post = post;
 }
 
 void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
@@ -101,7 +123,7 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
         const BCView V = d.bc_view(level, P, true);
         const double *b64 = level == 0 ? r : nullptr;
         const float *b32 = level == 0 ? nullptr : m_res1f[level];
-        smooth_fast_f(level, b64, b32);
+        smooth_fast_f(level, b64, b32, m_mixed_pre);
         d.timed("residuum32", level, [&] {
             if (b64) k::residuum_faces<float, double>(s, d.level(level), V, m_res0f[level], m_err1f[level], b64);
             else k::residuum_faces<float, float>(s, d.level(level), V, m_res0f[level], m_err1f[level], b32);
@@ -127,7 +149,7 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
                 k::prolongate_add_bc<float, float>(s, d.level(level), d.level(fine), V, m_tmpf[fine], m_err1f[fine],
                                                    m_err1f[level]);
         });
-        smooth_fast_f(fine, fine == 0 ? r : nullptr, fine == 0 ? nullptr : m_res1f[fine]);
+        smooth_fast_f(fine, fine == 0 ? r : nullptr, fine == 0 ? nullptr : m_res1f[fine], m_mixed_post);
     }
 }
 

@@ -52,6 +52,9 @@ class VCycleMG : public IPressure {
     // The result stays in mixed_result() (FP32, level 0) until the next call.
     bool mixed_supported() const;
     void prepare_mixed() { alloc_mixed(); }  // allocates; call before capturing vcycle_mixed
+    // pre- and post-smoothing sweeps of the mixed V-cycle on the levels
+    // above its single-workgroup kernel (0: the XML's effective count)
+    void set_mixed_sweeps(int pre, int post);
     // coarse_max_iter > 0 caps the coarse solve's Jacobi iterations
     void vcycle_mixed(const double *r, int coarse_max_iter = 0);
     float *mixed_result() const { return m_err1f.empty() ? nullptr : m_err1f[0]; }
@@ -101,10 +104,11 @@ class VCycleMG : public IPressure {
     TailPlan *m_plan_dev = nullptr;
     int m_tail_lds_bytes = 0;
     // FP32 level arrays of the mixed V-cycle (levels above the tail)
-    std::vector<float *> m_err1f, m_tmpf, m_res0f, m_res1f;
+    std::vector<float *> m_err1f, m_tmpf, m_scrf, m_res0f, m_res1f;
+    int m_mixed_pre = 0, m_mixed_post = 0;  // sweeps of the mixed V-cycle (0: the XML's)
     void alloc_mixed();
     void free_mixed();
-    void smooth_fast_f(int level, const double *b64, const float *b32);
+    void smooth_fast_f(int level, const double *b64, const float *b32, int sweeps);
     bool m_use_graphs = true;
     CycleGraph m_graph[2];  // [0]: V-cycle, [1]: V-cycle + residual
 };

@@ -161,6 +161,12 @@ int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max
 /* artss_mg_set_schedule for the solver's own pressure solver */
 int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cells);
 
+/* Method 2 only (no reference counterpart): weighted-Jacobi sweeps before
+ * and after the coarse correction on the FP32 levels of the preconditioner
+ * (0: the XML's n_relax with the reference's effective count).  The
+ * iteration still meets the same FP64 residual target. */
+int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post);
+
 /* Start of the residual-target GCR pressure iteration (methods 1 and 2; no
  * reference counterpart: the reference's iteration starts from the previous
  * step's p, mode 0).  With p_n the previous solution, p_prev the one before

@@ -202,6 +202,29 @@ int or_jacobi_diffuse(const or_domain *D, double *out, double *in, const double 
     return it;
 }
 
+/* ---------------- viscous dissipation, interfaces/ISource.cpp:63-121 ---------------- */
+void or_dissipate(const or_domain *D, double *out, const double *in_u, const double *in_v, const double *in_w,
+                  double nu, double dt, int ft) {
+    const size_t Nx = D->N[0][0], Ny = D->N[0][1];
+    const size_t ni = 1, nj = Nx, nk = Nx * Ny;
+    const double rdx = 1. / D->h[0][0], rdy = 1. / D->h[0][1], rdz = 1. / D->h[0][2];
+    const ivec *inner = &D->inner[0];
+    for (size_t l = 0; l < inner->n; ++l) {
+        const size_t i = inner->d[l];
+        double out_h = nu * (2 * (0.5 * rdx * (in_u[i + ni] - in_u[i - ni])) * (0.5 * rdx * (in_u[i + ni] - in_u[i - ni])) +
+                             2 * (0.5 * rdy * (in_v[i + nj] - in_v[i - nj])) * (0.5 * rdy * (in_v[i + nj] - in_v[i - nj])) +
+                             2 * (0.5 * rdz * (in_w[i + nk] - in_w[i - nk])) * (0.5 * rdz * (in_w[i + nk] - in_w[i - nk])) +
+                             ((0.5 * rdx * (in_v[i + ni] - in_v[i - ni])) + (0.5 * rdy * (in_u[i + nj] - in_u[i - nj]))) *
+                                 ((0.5 * rdx * (in_v[i + ni] - in_v[i - ni])) + (0.5 * rdy * (in_u[i + nj] - in_u[i - nj]))) +
+                             ((0.5 * rdy * (in_w[i + nj] - in_w[i - nj])) + (0.5 * rdz * (in_v[i + nk] - in_v[i - nk]))) *
+                                 ((0.5 * rdy * (in_w[i + nj] - in_w[i - nj])) + (0.5 * rdz * (in_v[i + nk] - in_v[i - nk]))) +
+                             ((0.5 * rdz * (in_u[i + nk] - in_u[i - nk])) + (0.5 * rdx * (in_w[i + ni] - in_w[i - ni]))) *
+                                 ((0.5 * rdz * (in_u[i + nk] - in_u[i - nk])) + (0.5 * rdx * (in_w[i + ni] - in_w[i - ni]))));
+        out[i] += dt * out_h;
+    }
+    or_apply_bc(D, out, ft, 0);
+}
+
 /* ---------------- explicit diffusion, diffusion/ExplicitDiffuse.cpp ---------------- */
 /* one explicit Euler step on the inner list (:49-78 plain, :81-121 turbulent:
    the turbulent form has no 1/h^2 factors and subtracts the upper flux from

@@ -74,6 +74,8 @@ int or_cgs_diffuse(const or_domain *D, double *out, double *in, const double *b,
                    double dt, int max_iter, double tol_res, double wgt, int ft, double *res_out);
 void or_explicit_diffuse(const or_domain *D, double *out, const double *in, double Dc, const double *ev, double dt,
                          int ft);
+void or_dissipate(const or_domain *D, double *out, const double *in_u, const double *in_v, const double *in_w,
+                  double nu, double dt, int ft);
 void or_divergence(const or_domain *D, double *out, const double *x, const double *y, const double *z);
 void or_projection(const or_domain *D, double *ou, double *ov, double *ow, const double *iu, const double *iv,
                    const double *iw, const double *ip);
@@ -103,7 +105,8 @@ int or_mg_pressure(or_mg *mg, double *out, const double *b, double t, double dt)
 
 /* ---- whole solver (solver/NSSolver.cpp, NSTempTurbSolver.cpp, TimeIntegration.cpp) ---- */
 typedef struct {
-    int solver;         /* 0 NSSolver, 1 NSTempTurbSolver */
+    int solver;         /* 0 NSSolver, 1 NSTempTurbSolver, 3 NSTempSolver, 4 NSTurbSolver, 5 DiffusionSolver,
+                           6 AdvectionDiffusionSolver, 7 DiffusionTurbSolver, 8 AdvectionSolver */
     double dt, nu, kappa, beta, g;
     int diff_type;      /* 0 Jacobi, 1 CGS, 2 Explicit */
     int diff_max_iter;
@@ -118,11 +121,13 @@ typedef struct {
     double gauss[8];    /* HRR, cp, x0, y0, z0, sigma_x, sigma_y, sigma_z */
     double tau;
     or_mg_params mg;
+    int dissipation;    /* NSTempSolver: temperature source dissipation */
+    double u_lin[3];    /* AdvectionSolver: constant transport velocity */
 } or_sim_params;
 typedef struct or_sim or_sim;
-/* u0v0w0T0: full arrays whose inner cells carry the initial condition (may be NULL) */
+/* u_ic ... p_ic: full arrays whose inner cells carry the initial condition (may be NULL) */
 or_sim *or_sim_create(const or_domain *D, const or_sim_params *p, const double *u_ic, const double *v_ic,
-                      const double *w_ic, const double *T_ic);
+                      const double *w_ic, const double *T_ic, const double *p_ic);
 void or_sim_free(or_sim *s);
 /* one TimeIntegration iteration: do_step(t), update_sources(t), update_data */
 void or_sim_step(or_sim *s, double t);

@@ -1,5 +1,8 @@
 /* Oracle solver drivers.  TEST INFRASTRUCTURE ONLY.
  * Restates solver/NSSolver.cpp:50-124, solver/NSTempTurbSolver.cpp:94-248,
+ * solver/NSTempSolver.cpp:69-198, the do_step of solver/NSTurbSolver.cpp,
+ * DiffusionSolver.cpp, AdvectionDiffusionSolver.cpp, DiffusionTurbSolver.cpp
+ * and AdvectionSolver.cpp,
  * solver/SolverController.cpp:26-43,229-307, field/FieldController.cpp:101-185
  * and one iteration of TimeIntegration.cpp:86-149. */
 #include "oracle_internal.h"
@@ -15,6 +18,7 @@ struct or_sim {
     size_t n;
     double *u, *v, *w, *u0, *v0, *w0, *ut, *vt, *wt;
     double *p_, *rhs, *T, *T0, *Tt, *Ta, *nu_t, *kappa_t, *fx, *fy, *fz, *S_T, *spatial;
+    double *lin[3]; /* AdvectionSolver's constant velocity fields */
     int counters[5];
 };
 
@@ -53,7 +57,7 @@ static void update_sources(or_sim *s, double t) {
         for (int a = 0; a < 3; a++)
             if (has_dir(p->force_dir, a)) or_buoyancy(s->D, f[a], s->T, s->Ta, p->beta, p->g);
     }
-    if (p->solver == 1 && p->has_temp_source) { /* GaussFunction::update_source, GaussFunction.cpp:27-40 */
+    if ((p->solver == 1 || p->solver == 3) && p->has_temp_source) { /* GaussFunction::update_source, GaussFunction.cpp:27-40 */
         double tv = tanh(t / p->tau);
         cp(s->S_T, s->spatial, s->n);
         for (size_t i = 0; i < s->n; i++) s->S_T[i] *= tv;
@@ -68,7 +72,7 @@ static void update_data(or_sim *s) {
 }
 
 or_sim *or_sim_create(const or_domain *D, const or_sim_params *p, const double *u_ic, const double *v_ic,
-                      const double *w_ic, const double *T_ic) {
+                      const double *w_ic, const double *T_ic, const double *p_ic) {
     or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
     s->D = D;
     s->p = *p;
@@ -78,7 +82,8 @@ or_sim *or_sim_create(const or_domain *D, const or_sim_params *p, const double *
                       &s->T, &s->T0, &s->Tt, &s->Ta, &s->nu_t, &s->kappa_t, &s->fx, &s->fy, &s->fz, &s->S_T,
                       &s->spatial};
     for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) *all[i] = alloc0(n);
-    s->mg = or_mg_create(D, &p->mg);
+    /* the drivers with a pressure solve */
+    s->mg = (p->solver <= 1 || p->solver == 3 || p->solver == 4) ? or_mg_create(D, &p->mg) : NULL;
     /* set_up_fields: initial conditions on the inner list, SolverController.cpp:95-205 */
     const ivec *inner = &D->inner[0];
     for (size_t l = 0; l < inner->n; l++) {
@@ -87,17 +92,24 @@ or_sim *or_sim_create(const or_domain *D, const or_sim_params *p, const double *
         if (v_ic) s->v[i] = v_ic[i];
         if (w_ic) s->w[i] = w_ic[i];
         if (T_ic) s->T[i] = T_ic[i];
+        if (p_ic) s->p_[i] = p_ic[i];
     }
-    if (p->solver == 1 && p->has_force) /* force_source: T_ambient.set_value, :229-236 */
+    const int temp = p->solver == 1 || p->solver == 3;
+    if (temp && p->has_force) /* force_source: T_ambient.set_value, :229-236 */
         for (size_t i = 0; i < n; i++) s->Ta[i] = p->T_ambient;
-    if (p->solver == 1 && p->has_temp_source) or_gauss_spatial(D, s->spatial, p->gauss);
+    if (temp && p->has_temp_source) or_gauss_spatial(D, s->spatial, p->gauss);
+    if (p->solver == 8) /* AdvectionSolver.cpp:16-34, Field(FieldType, u_lin) */
+        for (int a = 0; a < 3; a++) {
+            s->lin[a] = alloc0(n);
+            for (size_t i = 0; i < n; i++) s->lin[a][i] = p->u_lin[a];
+        }
     /* FieldController::set_up_boundary, FieldController.cpp:101-112 */
     or_apply_bc(D, s->u, OR_U, 0);
     or_apply_bc(D, s->v, OR_V, 0);
     or_apply_bc(D, s->w, OR_W, 0);
     or_apply_bc(D, s->p_, OR_P, 0);
     or_apply_bc(D, s->T, OR_T, 0);
-    if (p->solver == 1) or_apply_bc(D, s->Ta, OR_T, 0);
+    if (temp) or_apply_bc(D, s->Ta, OR_T, 0);
     else { /* T_ambient of a solver without temperature is never read */ }
     update_sources(s, 0);
     update_data(s);
@@ -109,6 +121,7 @@ void or_sim_free(or_sim *s) {
     double *all[] = {s->u, s->v, s->w, s->u0, s->v0, s->w0, s->ut, s->vt, s->wt, s->p_, s->rhs,
                      s->T, s->T0, s->Tt, s->Ta, s->nu_t, s->kappa_t, s->fx, s->fy, s->fz, s->S_T, s->spatial};
     for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) free(all[i]);
+    for (int a = 0; a < 3; a++) free(s->lin[a]);
     or_mg_free(s->mg);
     free(s);
 }
@@ -176,10 +189,105 @@ static void do_step_nstempturb(or_sim *s, double t) {
     }
 }
 
+static void vector_force(or_sim *s) {
+    const or_sim_params *p = &s->p;
+    if (p->has_force) {
+        if (has_dir(p->force_dir, 0)) or_add_source(s->D, s->u, s->fx, p->dt, OR_U);
+        if (has_dir(p->force_dir, 1)) or_add_source(s->D, s->v, s->fy, p->dt, OR_V);
+        if (has_dir(p->force_dir, 2)) or_add_source(s->D, s->w, s->fz, p->dt, OR_W);
+        couple_vector(s);
+    }
+}
+
+/* NSTempSolver.cpp:69-198 */
+static void do_step_nstemp(or_sim *s, double t) {
+    const or_domain *D = s->D;
+    const or_sim_params *p = &s->p;
+    or_sl_advect(D, s->u, s->u0, s->u0, s->v0, s->w0, p->dt, OR_U);
+    or_sl_advect(D, s->v, s->v0, s->u0, s->v0, s->w0, p->dt, OR_V);
+    or_sl_advect(D, s->w, s->w0, s->u0, s->v0, s->w0, p->dt, OR_W);
+    couple_vector(s);
+    if (p->nu != 0.) {
+        s->counters[0] = diffuse(s, s->u, s->u0, s->ut, p->nu, NULL, OR_U);
+        s->counters[1] = diffuse(s, s->v, s->v0, s->vt, p->nu, NULL, OR_V);
+        s->counters[2] = diffuse(s, s->w, s->w0, s->wt, p->nu, NULL, OR_W);
+        couple_vector(s);
+    }
+    vector_force(s);
+    or_divergence(D, s->rhs, s->ut, s->vt, s->wt);
+    s->counters[4] = or_mg_pressure(s->mg, s->p_, s->rhs, t, p->dt);
+    or_projection(D, s->u, s->v, s->w, s->ut, s->vt, s->wt, s->p_);
+    or_sl_advect(D, s->T, s->T0, s->u, s->v, s->w, p->dt, OR_T);
+    couple_scalar(s);
+    if (p->kappa != 0.) {
+        s->counters[3] = diffuse(s, s->T, s->T0, s->Tt, p->kappa, NULL, OR_T);
+        couple_scalar(s);
+    }
+    if (p->dissipation) {
+        or_dissipate(D, s->T, s->u, s->v, s->w, p->nu, p->dt, OR_T);
+        couple_scalar(s);
+    }
+    if (p->has_temp_source) {
+        or_add_source(D, s->T, s->S_T, p->dt, OR_T);
+        couple_scalar(s);
+    }
+}
+
+/* NSTurbSolver.cpp do_step */
+static void do_step_nsturb(or_sim *s, double t) {
+    const or_domain *D = s->D;
+    const or_sim_params *p = &s->p;
+    or_sl_advect(D, s->u, s->u0, s->u0, s->v0, s->w0, p->dt, OR_U);
+    or_sl_advect(D, s->v, s->v0, s->u0, s->v0, s->w0, p->dt, OR_V);
+    or_sl_advect(D, s->w, s->w0, s->u0, s->v0, s->w0, p->dt, OR_W);
+    couple_vector(s);
+    or_smagorinsky(D, s->nu_t, s->u, s->v, s->w, p->cs, OR_NU);
+    s->counters[0] = diffuse(s, s->u, s->u0, s->ut, p->nu, s->nu_t, OR_U);
+    s->counters[1] = diffuse(s, s->v, s->v0, s->vt, p->nu, s->nu_t, OR_V);
+    s->counters[2] = diffuse(s, s->w, s->w0, s->wt, p->nu, s->nu_t, OR_W);
+    couple_vector(s);
+    vector_force(s);
+    or_divergence(D, s->rhs, s->ut, s->vt, s->wt);
+    s->counters[4] = or_mg_pressure(s->mg, s->p_, s->rhs, t, p->dt);
+    or_projection(D, s->u, s->v, s->w, s->ut, s->vt, s->wt, s->p_);
+}
+
+/* DiffusionSolver, AdvectionDiffusionSolver, DiffusionTurbSolver, AdvectionSolver do_step */
+static void do_step_transport(or_sim *s) {
+    const or_domain *D = s->D;
+    const or_sim_params *p = &s->p;
+    if (p->solver == 8) {
+        or_sl_advect(D, s->u, s->u0, s->lin[0], s->lin[1], s->lin[2], p->dt, OR_U);
+        or_sl_advect(D, s->v, s->v0, s->lin[0], s->lin[1], s->lin[2], p->dt, OR_V);
+        or_sl_advect(D, s->w, s->w0, s->lin[0], s->lin[1], s->lin[2], p->dt, OR_W);
+        return;
+    }
+    if (p->solver == 6) {
+        or_sl_advect(D, s->u, s->u0, s->u0, s->v0, s->w0, p->dt, OR_U);
+        or_sl_advect(D, s->v, s->v0, s->u0, s->v0, s->w0, p->dt, OR_V);
+        or_sl_advect(D, s->w, s->w0, s->u0, s->v0, s->w0, p->dt, OR_W);
+        couple_vector(s);
+        if (p->nu == 0.) return;
+    }
+    const double *ev = NULL;
+    if (p->solver == 7) {
+        or_smagorinsky(D, s->nu_t, s->u, s->v, s->w, p->cs, OR_NU);
+        ev = s->nu_t;
+    }
+    s->counters[0] = diffuse(s, s->u, s->u0, s->ut, p->nu, ev, OR_U);
+    s->counters[1] = diffuse(s, s->v, s->v0, s->vt, p->nu, ev, OR_V);
+    s->counters[2] = diffuse(s, s->w, s->w0, s->wt, p->nu, ev, OR_W);
+}
+
 void or_sim_step(or_sim *s, double t) {
     memset(s->counters, 0, sizeof(s->counters));
-    if (s->p.solver == 0) do_step_ns(s, t);
-    else do_step_nstempturb(s, t);
+    switch (s->p.solver) {
+        case 0: do_step_ns(s, t); break;
+        case 1: do_step_nstempturb(s, t); break;
+        case 3: do_step_nstemp(s, t); break;
+        case 4: do_step_nsturb(s, t); break;
+        default: do_step_transport(s); break;
+    }
     update_sources(s, t);
     update_data(s);
 }

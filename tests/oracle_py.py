@@ -45,7 +45,7 @@ class SimParams(C.Structure):
                 ("diff_tol", C.c_double), ("diff_w", C.c_double), ("cs", C.c_double), ("prandtl", C.c_double),
                 ("has_force", C.c_int), ("force_dir", C.c_int * 3), ("T_ambient", C.c_double),
                 ("has_temp_source", C.c_int), ("temp_dir", C.c_int * 3), ("gauss", C.c_double * 8),
-                ("tau", C.c_double), ("mg", MGParams)]
+                ("tau", C.c_double), ("mg", MGParams), ("dissipation", C.c_int), ("u_lin", C.c_double * 3)]
 
 
 _lib = None
@@ -90,7 +90,7 @@ def lib():
         L.or_mg_pressure.restype = C.c_int
         L.or_mg_pressure.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double]
         L.or_sim_create.restype = C.c_void_p
-        L.or_sim_create.argtypes = [C.c_void_p, C.POINTER(SimParams), dp, dp, dp, dp]
+        L.or_sim_create.argtypes = [C.c_void_p, C.POINTER(SimParams), dp, dp, dp, dp, dp]
         L.or_sim_free.argtypes = [C.c_void_p]
         L.or_sim_step.argtypes = [C.c_void_p, C.c_double]
         L.or_sim_field.restype = dp
@@ -253,14 +253,17 @@ class MG:
 def sim_params(cfg):
     sp = SimParams()
     desc = cfg["solver"]
-    if desc == "NSSolver":
-        sp.solver = 0
-    elif desc == "NSTempTurbSolver":
-        sp.solver = 1
-    else:
-        raise ValueError("oracle drives NSSolver and NSTempTurbSolver only: " + desc)
+    kinds = {"NSSolver": 0, "NSTempTurbSolver": 1, "NSTempSolver": 3, "NSTurbSolver": 4, "DiffusionSolver": 5,
+             "AdvectionDiffusionSolver": 6, "DiffusionTurbSolver": 7, "AdvectionSolver": 8}
+    if desc not in kinds:
+        raise ValueError("the oracle has no driver for " + desc)
+    sp.solver = kinds[desc]
+    if desc == "AdvectionSolver":
+        v = cfg["ic"]["values"]
+        for a, k in enumerate(("u_lin", "v_lin", "w_lin")):
+            sp.u_lin[a] = float(v[k])
     sp.dt, sp.nu, sp.kappa, sp.beta, sp.g = cfg["dt"], cfg["nu"], cfg["kappa"], cfg["beta"], cfg["g"]
-    dif = cfg["diffusion"]
+    dif = cfg.get("diffusion", {"type": "Jacobi"})
     sp.diff_type = {"Jacobi": 0, "ColoredGaussSeidel": 1, "Explicit": 2}[dif["type"]]
     sp.diff_max_iter, sp.diff_tol, sp.diff_w = dif.get("max_iter", 0), dif.get("tol_res", 0.0), dif.get("w", 0.0)
     if "turbulence" in cfg:
@@ -282,17 +285,19 @@ def sim_params(cfg):
             for i, x in enumerate(ts["gauss"]):
                 sp.gauss[i] = x
             sp.tau = ts["tau"]
-    sp.mg = mg_params(cfg)
+        sp.dissipation = 1 if ts.get("dissipation") else 0
+    if "pressure" in cfg:
+        sp.mg = mg_params(cfg)
     return sp
 
 
 class Sim:
     """Oracle NSSolver / NSTempTurbSolver with TimeIntegration's per-step tail."""
 
-    def __init__(self, dom, cfg, u=None, v=None, w=None, T=None):
+    def __init__(self, dom, cfg, u=None, v=None, w=None, T=None, p=None):
         self.dom = dom
         self._p = sim_params(cfg)
-        self.h = dom.L.or_sim_create(dom.h, C.byref(self._p), _p(u), _p(v), _p(w), _p(T))
+        self.h = dom.L.or_sim_create(dom.h, C.byref(self._p), _p(u), _p(v), _p(w), _p(T), _p(p))
         self.n = dom.size(0)
 
     def __del__(self):

@@ -15,7 +15,8 @@ Every oracle function is compared bit for bit (np.array_equal on float64):
   divergence, projection (a12)  test_divergence, test_projection
   ConstSmagorinsky (a13)        test_smagorinsky
   sources (a17)                 test_sources (add_source scalar/vector, buoyancy, Gauss)
-  solver drivers (a18, a19)     test_run (NSSolver Taylor-Green, NSTempTurb plume Jacobi/CGS/explicit)
+  solver drivers (a18, a19)     test_run (NSSolver Taylor-Green, NSTempTurb plume Jacobi/CGS/explicit),
+                                test_run_drivers (the other solvers and the reference's initial conditions)
 
 The reference's committed tests/*/*_ref.dat files are not used: they do not
 reproduce with the reference's current sources (their initial conditions
@@ -204,6 +205,31 @@ def test_sources(ops_jacobi):
     dom.gauss_spatial(st, cfg["temperature"]["source"]["gauss"])
     st *= math.tanh(0.37 / cfg["temperature"]["source"]["tau"])  # libm, as GaussFunction.cpp:122
     same(st, z["sources/gauss_st"])
+
+
+@pytest.mark.parametrize("case", ["run_nstemp_dissipation", "run_nsturb_tg", "run_diffusion_tg",
+                                  "run_diffusion_turb_tg", "run_advection_diffusion_tg", "run_advection_gauss",
+                                  "run_diffusion_expsinusprod", "run_diffusion_hat", "run_burgers_expsinussum",
+                                  "run_ns_mcdermott", "run_ns_vortex", "run_ns_drift", "run_nstemp_layers"])
+def test_run_drivers(case):
+    """the other solver drivers, started from the reference's own initial
+    fields (the golden run's step 0; the product computes those itself and
+    is checked on them in test_gpu_parity.py)"""
+    z, cfg = load(case)
+    dom = O.Domain(cfg)
+    ic = {f: z["run/%s_0" % f] for f in "uvwTp"}
+    sim = O.Sim(dom, cfg, ic["u"], ic["v"], ic["w"], ic["T"], ic["p"])
+    fields = ["u", "v", "w", "p", "T"]
+    for f in fields:
+        same(sim.field(f), z["run/%s_0" % f])
+    nsteps = int(round(cfg["t_end"] / cfg["dt"]))
+    for step in range(1, nsteps + 1):
+        sim.step(step * cfg["dt"])
+        if step in (1, 2):
+            for f in fields + ["rhs"]:
+                same(sim.field(f), z["run/%s_%d" % (f, step)])
+    for f in fields:
+        same(sim.field(f), z["run/%s_-1" % f])
 
 
 @pytest.mark.parametrize("case", ["run_ns_tg", "run_plume_jacobi", "run_plume_cgs", "run_plume_explicit"])
