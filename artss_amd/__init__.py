@@ -86,7 +86,7 @@ def lib():
         "artss_solver_set_pressure_method": (C.c_int, [vp, C.c_int, C.c_int]),
         "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
         "artss_solver_set_warm_start": (C.c_int, [vp, C.c_int, vp, C.c_int]),
-        "artss_solver_set_mixed_sweeps": (C.c_int, [vp, C.c_int, C.c_int]),
+        "artss_solver_set_mixed_sweeps": (C.c_int, [vp, C.c_int, C.c_int, C.c_double]),
         "artss_solver_snapshot": (C.c_int, [vp, C.c_double, ip]),
         "artss_solver_restore": (C.c_int, [vp, C.c_int, dp]),
         "artss_solver_drop_snapshot": (C.c_int, [vp, C.c_int]),
@@ -464,9 +464,9 @@ class Solver:
         _check(self.L.artss_solver_load(self.h, os.fspath(path).encode(), C.byref(t)))
         return t.value
 
-    def set_mixed_sweeps(self, pre=0, post=0):
-        """method 2: smoothing sweeps of the FP32 levels (0: the XML's)"""
-        _check(self.L.artss_solver_set_mixed_sweeps(self.h, int(pre), int(post)))
+    def set_mixed_sweeps(self, pre=0, post=0, weight=0.0):
+        """method 2: smoothing sweeps of the FP32 levels and their Jacobi weight (0: the XML's)"""
+        _check(self.L.artss_solver_set_mixed_sweeps(self.h, int(pre), int(post), float(weight)))
 
     def set_mg_schedule(self, mode=2, tail_cells=4096):
         _check(self.L.artss_solver_set_mg_schedule(self.h, int(mode), int(tail_cells)))

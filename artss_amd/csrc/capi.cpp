@@ -666,12 +666,13 @@ int artss_solver_initial_residual(artss_solver *s, double *residual) {
     });
 }
 
-int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post) {
+int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post, double weight) {
     return guard([&] {
-        require(s && pre >= 0 && post >= 0, "bad argument");
+        require(s && pre >= 0 && post >= 0 && weight >= 0.0 && weight < 2.0, "bad argument");
         PressureMode m = s->solver->pressure_mode();
         m.mixed_pre = pre;
         m.mixed_post = post;
+        m.mixed_weight = weight;
         s->solver->set_pressure_mode(m);
         return ARTSS_OK;
     });

@@ -83,12 +83,11 @@ void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int
     const BCView V = d.bc_view(level, P, true);
     const int nx = m_s.n_relax;
     const int n = sweeps > 0 ? sweeps : ((nx % 2 == 0 && nx > 0) ? nx - 1 : nx);
+    const double w = m_mixed_weight > 0.0 ? m_mixed_weight : m_s.smoother.w;
     auto sweep = [&](bool first, float *dst, const float *src) {
         d.timed("jacobi_mg32", level, [&] {
-            if (b64) k::jacobi_bc<float, double>(d.stream(), Ld, V, first, dst, src, b64, ax, ay, az, beta, kDsign,
-                                                 m_s.smoother.w);
-            else k::jacobi_bc<float, float>(d.stream(), Ld, V, first, dst, src, b32, ax, ay, az, beta, kDsign,
-                                            m_s.smoother.w);
+            if (b64) k::jacobi_bc<float, double>(d.stream(), Ld, V, first, dst, src, b64, ax, ay, az, beta, kDsign, w);
+            else k::jacobi_bc<float, float>(d.stream(), Ld, V, first, dst, src, b32, ax, ay, az, beta, kDsign, w);
         });
     };
     float *src = m_tmpf[level];
@@ -105,11 +104,13 @@ void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int
     }
 }
 
-void VCycleMG::set_mixed_sweeps(int pre, int post) {
+void VCycleMG::set_mixed_sweeps(int pre, int post, double weight) {
     if (pre < 0 || post < 0) throw config_error("sweep counts must not be negative");
+    if (!(weight >= 0.0 && weight < 2.0)) throw config_error("the Jacobi weight must lie in [0, 2)");
     drop_graphs();
     m_mixed_pre = pre;
     m_mixed_post = post;
+    m_mixed_weight = weight;
 }
 
 void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {

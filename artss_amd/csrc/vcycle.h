@@ -53,8 +53,9 @@ class VCycleMG : public IPressure {
     bool mixed_supported() const;
     void prepare_mixed() { alloc_mixed(); }  // allocates; call before capturing vcycle_mixed
     // pre- and post-smoothing sweeps of the mixed V-cycle on the levels
-    // above its single-workgroup kernel (0: the XML's effective count)
-    void set_mixed_sweeps(int pre, int post);
+    // above its single-workgroup kernel (0: the XML's effective count) and
+    // their Jacobi weight (0: the XML's)
+    void set_mixed_sweeps(int pre, int post, double weight = 0.0);
     // coarse_max_iter > 0 caps the coarse solve's Jacobi iterations
     void vcycle_mixed(const double *r, int coarse_max_iter = 0);
     float *mixed_result() const { return m_err1f.empty() ? nullptr : m_err1f[0]; }
@@ -106,6 +107,7 @@ class VCycleMG : public IPressure {
     // FP32 level arrays of the mixed V-cycle (levels above the tail)
     std::vector<float *> m_err1f, m_tmpf, m_scrf, m_res0f, m_res1f;
     int m_mixed_pre = 0, m_mixed_post = 0;  // sweeps of the mixed V-cycle (0: the XML's)
+    double m_mixed_weight = 0.0;            // their Jacobi weight (0: the XML's)
     void alloc_mixed();
     void free_mixed();
     void smooth_fast_f(int level, const double *b64, const float *b32, int sweeps);

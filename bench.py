@@ -75,8 +75,10 @@ def parse():
                     help="residual-target iteration: V-cycles alone (reference) or V-cycle-preconditioned GCR")
     ap.add_argument("--coarse-iterations", type=int, default=0,
                     help="mixed: cap of the preconditioner's coarse Jacobi solve (0: the XML's max_iter)")
-    ap.add_argument("--sweeps", default="0,0",
+    ap.add_argument("--sweeps", default="1,1",
                     help="mixed: pre,post smoothing sweeps of the FP32 levels (0: the XML's effective count)")
+    ap.add_argument("--weight", type=float, default=0.8,
+                    help="mixed: Jacobi weight of those sweeps (0: the XML's)")
     ap.add_argument("--tail-cells", type=int, default=4096,
                     help="levels of at most this many cells run in one single-workgroup kernel")
     ap.add_argument("--warm", choices=("none", "extrapolate"), default="extrapolate",
@@ -125,7 +127,7 @@ def make_solver(artss_amd, args, pressure, warm="none"):
             solver.set_warm_start(1)
         if pressure == "mixed":
             pre, post = (int(v) for v in args.sweeps.split(","))
-            solver.set_mixed_sweeps(pre, post)
+            solver.set_mixed_sweeps(pre, post, args.weight)
     return solver
 
 
@@ -288,7 +290,7 @@ def main():
                    "path": PATHS[args.pressure], "mg_schedule": args.mg_schedule, "tail_cells": args.tail_cells,
                    "warm_start": ("previous p" if args.warm == "none" or args.pressure == "vcycle" else
                                   "p + (p - p_prev), linear extrapolation in time"),
-                   "coarse_iterations": args.coarse_iterations, "sweeps": args.sweeps,
+                   "coarse_iterations": args.coarse_iterations, "sweeps": args.sweeps, "weight": args.weight,
                    "pressure": ("reference V-cycle schedule" if args.reference_schedule else
                                 "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance),
                    "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(

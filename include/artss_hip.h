@@ -163,9 +163,10 @@ int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cell
 
 /* Method 2 only (no reference counterpart): weighted-Jacobi sweeps before
  * and after the coarse correction on the FP32 levels of the preconditioner
- * (0: the XML's n_relax with the reference's effective count).  The
- * iteration still meets the same FP64 residual target. */
-int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post);
+ * (0: the XML's n_relax with the reference's effective count) and their
+ * weight (0: the XML's).  The iteration still meets the same FP64 residual
+ * target. */
+int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post, double weight);
 
 /* Start of the residual-target GCR pressure iteration (methods 1 and 2; no
  * reference counterpart: the reference's iteration starts from the previous
