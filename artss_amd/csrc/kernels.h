@@ -13,6 +13,14 @@ namespace k {
 
 // ---- element-wise over the whole array (Field.h:64-135) ----
 void copy(hipStream_t s, double *dst, const double *src, long long n);
+// dst0[q] = dst1[q] = src[q] for q < fields (1..4), n elements each, one pass
+struct CopyFan {
+    const double *src[4];
+    double *dst0[4];
+    double *dst1[4];
+    int fields;
+};
+void copy_fan(hipStream_t s, const CopyFan &f, long long n);
 void fill(hipStream_t s, double *dst, double v, long long n);
 void fill32(hipStream_t s, float *dst, float v, long long n);
 void scale(hipStream_t s, double *dst, double v, long long n);           // dst *= v
@@ -78,6 +86,7 @@ void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, dou
 // Iterates of type T (double: the reference's arithmetic; float: the mixed
 // V-cycle), right-hand sides / coarse errors of type TB / Tc.
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src);
+// src == nullptr (first sweeps only): the sweep starts from zero everywhere
 template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
                double ax, double ay, double az, double beta, double dsign, double w);

@@ -15,6 +15,7 @@
 #include "stencil.h"
 
 #include <cmath>
+#include <stdexcept>
 
 namespace artss {
 namespace k {
@@ -86,6 +87,16 @@ __device__ inline void write_partial(double *partial, double v) {
 // ------------------------------------------------------------------ element-wise
 __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long long n) {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i];
+}
+// each source into two destinations, for up to four fields at once
+__global__ void k_copy_fan(CopyFan f, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+        for (int q = 0; q < f.fields; q++) {
+            const double v = f.src[q][i];
+            f.dst0[q][i] = v;
+            f.dst1[q][i] = v;
+        }
+    }
 }
 __global__ void k_fill32(float *__restrict__ d, float v, long long n) {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = v;
@@ -507,13 +518,20 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict_
 // initial copy is folded in, every cell of dst is written (inner cells with
 // the update of src, filled cells by the fill, all others with src);
 // first = 0: cells that are neither inner nor filled keep dst's own values.
-template <typename T, typename TB, bool FIRST>
+// ZERO (with FIRST): src is zero everywhere and is not read, the update is
+// w beta dsign b, the value the sweep gives for src = 0.
+template <typename T, typename TB, bool FIRST, bool ZERO>
 __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, T ax, T ay, T az, T beta, T dsign, T w,
                             T *__restrict__ dst, const T *__restrict__ src, const TB *__restrict__ b) {
+    static_assert(FIRST || !ZERO, "a sweep from zero writes every cell");
     if (static_cast<int>(blockIdx.z) >= main_z) {
         auto val = [&](long long c) -> T {
             int ci, cj, ck;
-            if (is_inner(L, c, ci, cj, ck)) return dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, c);
+            if (is_inner(L, c, ci, cj, ck)) {
+                if (ZERO) return w * (beta * (dsign * static_cast<T>(b[c])));
+                return dev::jacobi_t<T, TB>(L, ax, ay, az, beta, dsign, w, src, b, c);
+            }
+            if (ZERO) return T(0);
             return FIRST ? src[c] : dst[c];
         };
         dev::bc_item<T, decltype(val), true>(L, V, item_index(main_z), dst, val);
@@ -536,15 +554,19 @@ __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, T ax, T ay, T az,
     // (the expression of JacobiDiffuse.cpp:292-300, as dev::jacobi_t)
     T v = T(0);
     if (in_inner_box) {
-        const T out_h = beta * (dsign * static_cast<T>(b[idx]) + ax * (src[idx + 1] + src[idx - 1]) +
-                                ay * (src[idx + nj] + src[idx - nj]) + az * (src[idx + nk] + src[idx - nk]));
-        v = (1 - w) * src[idx] + w * out_h;
+        if (ZERO) {
+            v = w * (beta * (dsign * static_cast<T>(b[idx])));
+        } else {
+            const T out_h = beta * (dsign * static_cast<T>(b[idx]) + ax * (src[idx + 1] + src[idx - 1]) +
+                                    ay * (src[idx + nj] + src[idx - nj]) + az * (src[idx + nk] + src[idx - nk]));
+            v = (1 - w) * src[idx] + w * out_h;
+        }
     }
     if (in_inner_box && !in_obstacle(L, i, j, k)) {
         dst[idx] = v;
         dev::face_ghosts(L, V, dst, static_cast<long long>(idx), i, j, k, v);
     } else if (FIRST && !dev::written(V, idx)) {
-        dst[idx] = src[idx];
+        dst[idx] = ZERO ? T(0) : src[idx];
     }
 }
 
@@ -740,6 +762,10 @@ __global__ void k_buoyancy(LevelDesc L, double beta, double g, double *__restric
 void copy(hipStream_t s, double *dst, const double *src, long long n) {
     k_copy<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n);
 }
+void copy_fan(hipStream_t s, const CopyFan &f, long long n) {
+    if (f.fields < 1 || f.fields > 4) throw std::invalid_argument("copy_fan: 1 to 4 fields");
+    k_copy_fan<<<grid1d(n, 256), 256, 0, s>>>(f, n);
+}
 void fill(hipStream_t s, double *dst, double v, long long n) { k_fill<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
 void fill32(hipStream_t s, float *dst, float v, long long n) { k_fill32<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
 void scale(hipStream_t s, double *dst, double v, long long n) { k_scale<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
@@ -839,16 +865,20 @@ void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, co
 template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
                double ax, double ay, double az, double beta, double dsign, double w) {
+    if (src == nullptr && !first) throw std::invalid_argument("jacobi_bc: a sweep from zero must be a first sweep");
     const dim3 g = first ? full_grid(L) : inner_grid(L);
     const dim3 gi = with_items(g, dev::bc_items_grid(V));
     const T ax_ = static_cast<T>(ax), ay_ = static_cast<T>(ay), az_ = static_cast<T>(az), beta_ = static_cast<T>(beta),
             ds_ = static_cast<T>(dsign), w_ = static_cast<T>(w);
-    if (first)
-        k_jacobi_bc<T, TB, true><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_, w_,
-                                                        dst, src, b);
+    if (first && src == nullptr)
+        k_jacobi_bc<T, TB, true, true><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_,
+                                                              w_, dst, src, b);
+    else if (first)
+        k_jacobi_bc<T, TB, true, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_,
+                                                               w_, dst, src, b);
     else
-        k_jacobi_bc<T, TB, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_, w_,
-                                                         dst, src, b);
+        k_jacobi_bc<T, TB, false, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_,
+                                                                ds_, w_, dst, src, b);
 }
 template void jacobi_bc<double, double>(hipStream_t, const LevelDesc &, const BCView &, bool, double *,
                                         const double *, const double *, double, double, double, double, double,

@@ -19,6 +19,8 @@ GcrPressure::GcrPressure(Domain &d, VCycleMG &preconditioner, bool mixed, int co
     ARTSS_HIP(hipMalloc(&m_coef, sizeof(double) * 4));
     ARTSS_HIP(hipMalloc(&m_sum, sizeof(double)));
     ARTSS_HIP(hipHostMalloc(&m_sum_host, sizeof(double)));
+    ARTSS_HIP(hipMalloc(&m_gate, sizeof(kk::GcrGate)));
+    ARTSS_HIP(hipHostMalloc(&m_gate_host, sizeof(kk::GcrGate)));
     ARTSS_HIP(hipMalloc(&m_zp32, sizeof(float) * static_cast<size_t>(L.size)));
     ARTSS_HIP(hipMalloc(&m_wp32, sizeof(float) * static_cast<size_t>(L.size)));
     ARTSS_HIP(hipMemset(m_zp32, 0, sizeof(float) * static_cast<size_t>(L.size)));
@@ -32,13 +34,17 @@ GcrPressure::~GcrPressure() {
     (void)hipFree(m_coef);
     (void)hipFree(m_sum);
     (void)hipHostFree(m_sum_host);
+    (void)hipFree(m_gate);
+    (void)hipHostFree(m_gate_host);
     (void)hipFree(m_zp32);
     (void)hipFree(m_wp32);
 }
 
 // One iteration: z = M r (one V-cycle from zero), the linear part of the
 // fill on z, w = A z with the four dot products, the coefficients, and the
-// update of p, r and the kept direction, ending with (r, r) in pinned memory.
+// update of p, r and the kept direction, ending with (r, r) in the gate.
+// The coefficient, update and sum kernels do nothing once the gate is
+// closed; the preconditioner and lap_dots write only scratch buffers.
 void GcrPressure::iteration(Field &p) {
     Domain &d = m_domain;
     const LevelDesc &L = d.level(0);
@@ -49,22 +55,21 @@ void GcrPressure::iteration(Field &p) {
         m_pre.vcycle_mixed(m_r.data, m_coarse_max_iter);
         const float *z = m_pre.mixed_result();
         d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, z, m_wp32, m_r.data, m_partial); });
-        kk::gcr_coef(s, m_partial, kk::blocks_dots(L), m_coef);
+        kk::gcr_coef(s, m_partial, kk::blocks_dots(L), m_coef, m_gate);
         d.timed("gcr_update", 0, [&] {
-            kk::gcr_update(s, L, m_coef, p.data, m_r.data, z, m_zp32, m_wp32, m_partial);
+            kk::gcr_update(s, L, m_coef, p.data, m_r.data, z, m_zp32, m_wp32, m_partial, m_gate);
         });
     } else {
         m_z.set_value(0.0);
         m_pre.vcycle_eager(m_z, m_r);
         d.apply_boundary_linear(m_z.data, P, 0);
         d.timed("gcr_lap", 0, [&] { kk::lap_dots(s, L, m_z.data, m_wp.data, m_r.data, m_partial); });
-        kk::gcr_coef(s, m_partial, kk::blocks_dots(L), m_coef);
+        kk::gcr_coef(s, m_partial, kk::blocks_dots(L), m_coef, m_gate);
         d.timed("gcr_update", 0, [&] {
-            kk::gcr_update(s, L, m_coef, p.data, m_r.data, m_z.data, m_zp.data, m_wp.data, m_partial);
+            kk::gcr_update(s, L, m_coef, p.data, m_r.data, m_z.data, m_zp.data, m_wp.data, m_partial, m_gate);
         });
     }
-    kk::sum(s, m_partial, kk::blocks_full(L), m_sum);
-    ARTSS_HIP(hipMemcpyAsync(m_sum_host, m_sum, sizeof(double), hipMemcpyDeviceToHost, s));
+    kk::sum_gate(s, m_partial, kk::blocks_full(L), m_gate);
 }
 
 void GcrPressure::launch_iteration(Field &p, const Field &b) {
@@ -110,6 +115,8 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
     if (m_mixed) m_pre.prepare_mixed();
     const LevelDesc &L = d.level(0);
     hipStream_t s = d.stream();
+    // convergence is rr <= tol2 on the host and in the gate alike
+    const double tol2 = tol * tol;
     auto restart = [&]() {
         d.apply_boundary(p.data, P, 0);
         kk::residual_vec(s, L, m_r.data, p.data, b.data, m_partial);
@@ -119,36 +126,38 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
         m_wp.set_value(0.0);
         k::fill32(s, m_zp32, 0.0f, L.size);
         k::fill32(s, m_wp32, 0.0f, L.size);
+        kk::gate_open(s, m_gate, tol2);
         ARTSS_HIP(hipStreamSynchronize(s));
-        return std::sqrt(*m_sum_host);
+        return *m_sum_host;
     };
-    double res = restart();
-    m_initial = res;
-    int it = 0;
+    double rr = restart();
+    m_initial = std::sqrt(rr);
+    int it = 0, it_base = 0;
     bool confirmed_once = false;
     while (true) {
-        if (res <= tol) {
+        if (rr <= tol2) {
             const double t = true_residual(p, b);
-            if (t <= tol) {
+            if (t <= tol || confirmed_once) {
                 *residual = t;
+                m_predicted = std::max(0, it - 1);
                 return it;
             }
             // the recursive residual drifted below the true one: restart
             // from the true residual (once per solve is plenty)
-            if (confirmed_once) {
-                *residual = t;
-                return it;
-            }
             confirmed_once = true;
-            res = restart();
+            rr = restart();
+            it_base = it;
             continue;
         }
         if (it >= max_iter) break;
-        launch_iteration(p, b);
+        const int batch = it < m_predicted ? std::min(m_predicted - it, max_iter - it) : 1;
+        for (int q = 0; q < batch; q++) launch_iteration(p, b);
+        ARTSS_HIP(hipMemcpyAsync(m_gate_host, m_gate, sizeof(kk::GcrGate), hipMemcpyDeviceToHost, s));
         ARTSS_HIP(hipStreamSynchronize(s));
-        res = std::sqrt(*m_sum_host);
-        it++;
+        rr = m_gate_host->rr;
+        it = it_base + m_gate_host->count;
     }
+    m_predicted = std::max(0, it - 1);
     *residual = true_residual(p, b);
     return it;
 }

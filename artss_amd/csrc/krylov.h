@@ -13,6 +13,7 @@
 #include <memory>
 
 #include "field.h"
+#include "krylov_kernels.h"
 #include "vcycle.h"
 
 namespace artss {
@@ -30,7 +31,10 @@ class GcrPressure {
     GcrPressure &operator=(const GcrPressure &) = delete;
 
     // iterate until ||b - lap p||_2 <= tol (or max_iter); returns the
-    // iterations (V-cycles) taken, *residual the final true residual
+    // iterations (V-cycles) taken, *residual the final true residual.
+    // Up to one fewer iterations than the previous solve took are enqueued
+    // at once behind the convergence gate (kk::GcrGate), the rest one at a
+    // time: the result is the same as with a check after every iteration.
     int solve(Field &p, const Field &b, double tol, int max_iter, double *residual);
     // ||b - lap p||_2 at the start of the last solve
     double initial_residual() const { return m_initial; }
@@ -50,6 +54,9 @@ class GcrPressure {
     double *m_coef = nullptr;          // device: beta, 1/|w'|, alpha
     double *m_sum = nullptr;           // device scalar
     double *m_sum_host = nullptr;      // pinned
+    kk::GcrGate *m_gate = nullptr;     // device
+    kk::GcrGate *m_gate_host = nullptr;  // pinned
+    int m_predicted = 0;               // iterations enqueued at once in the next solve
     double m_initial = -1.0;
     hipGraphExec_t m_graph = nullptr;
     double *m_graph_p = nullptr;

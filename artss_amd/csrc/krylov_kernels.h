@@ -9,6 +9,16 @@
 namespace artss {
 namespace kk {
 
+// Convergence gate of a GCR solve in device memory: iterations enqueued
+// after the one that met the tolerance change nothing, so several can be
+// enqueued without waiting for each residual.
+struct GcrGate {
+    double rr;    // (r, r) after the last iteration that ran
+    double tol2;  // tolerance squared
+    int count;    // iterations that ran since gate_open
+    int done;     // 1 once rr <= tol2
+};
+
 int blocks_inner(const LevelDesc &L);  // partial slots per sum of the inner-grid kernels
 int blocks_full(const LevelDesc &L);   // ... of the whole-array kernels
 int blocks_dots(const LevelDesc &L);   // ... of lap_dots
@@ -19,14 +29,20 @@ void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x,
 template <typename Tz, typename Tw>
 void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tw *wp, const double *r, double *partial);
 // GCR coefficients from the 4 sums: coef[0] beta, coef[1] 1/|w - beta wp|, coef[2] alpha
-void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef);
+// (gate: nothing when closed; nullptr: always)
+void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef, const GcrGate *gate);
 // the GCR step over the whole array (w = lap(z) recomputed); 1 sum: (r, r).
 // The kept direction (zp, wp) is stored as Tw.
 template <typename Tz, typename Tw>
 void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, Tw *zp,
-                Tw *wp, double *partial);
+                Tw *wp, double *partial, const GcrGate *gate);
 // out[0] = fixed-order sum of n partials
 void sum(hipStream_t s, const double *partial, int n, double *out);
+// the same sum into gate->rr, counting the iteration and closing the gate
+// when rr <= tol2; nothing when the gate is closed
+void sum_gate(hipStream_t s, const double *partial, int n, GcrGate *gate);
+// rr = 0, count = 0, done = 0, tol2 = tol2
+void gate_open(hipStream_t s, GcrGate *gate, double tol2);
 
 }  // namespace kk
 }  // namespace artss

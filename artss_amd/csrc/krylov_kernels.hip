@@ -128,7 +128,9 @@ __global__ void k_lap_dots(LevelDesc L, const Tz *__restrict__ z, const Tw *__re
 }
 
 // coef[0] = beta, coef[1] = 1/|w - beta wp|, coef[2] = alpha
-__global__ void k_gcr_coef(const double *__restrict__ partial, int blocks, double *__restrict__ coef) {
+__global__ void k_gcr_coef(const double *__restrict__ partial, int blocks, double *__restrict__ coef,
+                           const GcrGate *__restrict__ gate) {
+    if (gate != nullptr && gate->done) return;
     __shared__ double s[4][16];
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int t = threadIdx.x; t < blocks; t += 1024)
@@ -164,7 +166,8 @@ __global__ void k_gcr_coef(const double *__restrict__ partial, int blocks, doubl
 template <typename Tz, typename Tw>
 __global__ void k_gcr_update(LevelDesc L, const double *__restrict__ coef, double *__restrict__ x,
                              double *__restrict__ r, const Tz *__restrict__ z, Tw *__restrict__ zp,
-                             Tw *__restrict__ wp, double *__restrict__ partial) {
+                             Tw *__restrict__ wp, double *__restrict__ partial, const GcrGate *__restrict__ gate) {
+    if (gate != nullptr && gate->done) return;  // uniform over the grid
     int bx, by, bz;
     xcd_block(bx, by, bz);
     const int i = bx * BX + static_cast<int>(threadIdx.x);
@@ -200,6 +203,35 @@ __global__ void k_sum(const double *__restrict__ partial, int n, double *__restr
         double r = 0.0;
         for (int w = 0; w < 16; w++) r += s[w];
         out[0] = r;
+    }
+}
+
+// the sum of the update's partials closes the gate when (r, r) <= tol^2;
+// nothing runs once it is closed, so the state stays that of the first
+// iteration that met the tolerance
+__global__ void k_sum_gate(const double *__restrict__ partial, int n, GcrGate *__restrict__ gate) {
+    if (gate->done) return;
+    __shared__ double s[16];
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += partial[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < 16; w++) r += s[w];
+        gate->rr = r;
+        gate->count += 1;
+        gate->done = r <= gate->tol2 ? 1 : 0;
+    }
+}
+
+__global__ void k_gate_open(GcrGate *__restrict__ gate, double tol2) {
+    if (threadIdx.x == 0) {
+        gate->rr = 0.0;
+        gate->tol2 = tol2;
+        gate->count = 0;
+        gate->done = 0;
     }
 }
 
@@ -311,19 +343,23 @@ template void lap_dots<double, double>(hipStream_t, const LevelDesc &, const dou
                                        const double *, double *);
 template void lap_dots<float, float>(hipStream_t, const LevelDesc &, const float *, const float *, const double *,
                                      double *);
-void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef) {
-    k_gcr_coef<<<1, 1024, 0, s>>>(partial, blocks, coef);
+void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef, const GcrGate *gate) {
+    k_gcr_coef<<<1, 1024, 0, s>>>(partial, blocks, coef, gate);
 }
 template <typename Tz, typename Tw>
 void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, Tw *zp,
-                Tw *wp, double *partial) {
-    k_gcr_update<Tz, Tw><<<full_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, wp, partial);
+                Tw *wp, double *partial, const GcrGate *gate) {
+    k_gcr_update<Tz, Tw><<<full_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, wp, partial, gate);
 }
 template void gcr_update<double, double>(hipStream_t, const LevelDesc &, const double *, double *, double *,
-                                         const double *, double *, double *, double *);
+                                         const double *, double *, double *, double *, const GcrGate *);
 template void gcr_update<float, float>(hipStream_t, const LevelDesc &, const double *, double *, double *,
-                                       const float *, float *, float *, double *);
+                                       const float *, float *, float *, double *, const GcrGate *);
 void sum(hipStream_t s, const double *partial, int n, double *out) { k_sum<<<1, 1024, 0, s>>>(partial, n, out); }
+void sum_gate(hipStream_t s, const double *partial, int n, GcrGate *gate) {
+    k_sum_gate<<<1, 1024, 0, s>>>(partial, n, gate);
+}
+void gate_open(hipStream_t s, GcrGate *gate, double tol2) { k_gate_open<<<1, 64, 0, s>>>(gate, tol2); }
 
 }  // namespace kk
 }  // namespace artss

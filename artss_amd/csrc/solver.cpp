@@ -17,29 +17,24 @@ FieldController::FieldController(Domain &d)
       nu_t(d, NU), kappa_t(d, artss::T), p(d, P), rhs(d, P), T(d, artss::T), T0(d, artss::T), T_tmp(d, artss::T),
       T_ambient(d, artss::T), f_x(d, U), f_y(d, V), f_z(d, W), S_T(d, artss::T) {}
 
+// FieldController.cpp couple_vector / couple_scalar / update_data: the copies
+// into the previous-step and temporary fields, one pass per call
 void FieldController::couple_vector() {
-    u0.copy_data(u);
-    u_tmp.copy_data(u);
-    v0.copy_data(v);
-    v_tmp.copy_data(v);
-    w0.copy_data(w);
-    w_tmp.copy_data(w);
+    k::CopyFan f{{u.data, v.data, w.data}, {u0.data, v0.data, w0.data}, {u_tmp.data, v_tmp.data, w_tmp.data}, 3};
+    k::copy_fan(u.domain().stream(), f, u.get_size());
 }
 
 void FieldController::couple_scalar() {
-    T0.copy_data(T);
-    T_tmp.copy_data(T);
+    k::CopyFan f{{T.data}, {T0.data}, {T_tmp.data}, 1};
+    k::copy_fan(T.domain().stream(), f, T.get_size());
 }
 
 void FieldController::update_data() {
-    u0.copy_data(u);
-    v0.copy_data(v);
-    w0.copy_data(w);
-    u_tmp.copy_data(u);
-    v_tmp.copy_data(v);
-    w_tmp.copy_data(w);
-    T0.copy_data(T);
-    T_tmp.copy_data(T);
+    k::CopyFan f{{u.data, v.data, w.data, T.data},
+                 {u0.data, v0.data, w0.data, T0.data},
+                 {u_tmp.data, v_tmp.data, w_tmp.data, T_tmp.data},
+                 4};
+    k::copy_fan(u.domain().stream(), f, u.get_size());
 }
 
 std::vector<std::pair<const char *, Field *>> FieldController::all() {
@@ -198,8 +193,7 @@ void Solver::warm_begin() {
     FieldController &f = *m_fields;
     const LevelDesc &L = d.level(0);
     hipStream_t s = d.stream();
-    ARTSS_HIP(hipMemcpyAsync(m_p_start, f.p.data, sizeof(double) * static_cast<size_t>(L.size),
-                             hipMemcpyDeviceToDevice, s));
+    k::copy(s, m_p_start, f.p.data, L.size);
     if (m_warm_history == 0) return;
     const double fluid = static_cast<double>(d.inner_cells(0));
     kk::warm_sum(s, L, f.p.data, m_p_prev, m_warm_partial, m_warm_sum);

@@ -75,7 +75,7 @@ void VCycleMG::alloc_mixed() {
 // effective count).  An even count starts with one sweep into a third
 // buffer, so the result always lands in err1; the first sweep into a buffer
 // copies the cells it does not update, so err1 holds tmp's values there.
-void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int sweeps) {
+void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int sweeps, bool from_zero) {
     Domain &d = m_domain;
     double ax, ay, az, beta;
     coeffs(level, &ax, &ay, &az, &beta);
@@ -90,7 +90,9 @@ void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int
             else k::jacobi_bc<float, float>(d.stream(), Ld, V, first, dst, src, b32, ax, ay, az, beta, kDsign, w);
         });
     };
-    float *src = m_tmpf[level];
+    // from_zero: tmp is zero and the first sweep does not read it; after
+    // the first sweep tmp is free and the sweeps alternate err1 and tmp
+    float *src = from_zero ? nullptr : m_tmpf[level];
     int m = n;
     if (n % 2 == 0 && n > 0) {
         sweep(true, m_scrf[level], src);
@@ -100,7 +102,8 @@ void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int
     float *dst = m_err1f[level];
     for (int i = 0; i < m; i++) {
         sweep(i == 0, dst, src);
-        std::swap(src, dst);
+        src = dst;
+        dst = dst == m_err1f[level] ? m_tmpf[level] : m_err1f[level];
     }
 }
 
@@ -119,12 +122,12 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
     Domain &d = m_domain;
     const int f = m_tail_first;
     hipStream_t s = d.stream();
-    k::fill32(s, m_tmpf[0], 0.0f, d.level(0).size);
+    // every level's pre-smoothing starts from zero
     for (int level = 0; level < f; ++level) {
         const BCView V = d.bc_view(level, P, true);
         const double *b64 = level == 0 ? r : nullptr;
         const float *b32 = level == 0 ? nullptr : m_res1f[level];
-        smooth_fast_f(level, b64, b32, m_mixed_pre);
+        smooth_fast_f(level, b64, b32, m_mixed_pre, true);
         d.timed("residuum32", level, [&] {
             if (b64) k::residuum_faces<float, double>(s, d.level(level), V, m_res0f[level], m_err1f[level], b64);
             else k::residuum_faces<float, float>(s, d.level(level), V, m_res0f[level], m_err1f[level], b32);
@@ -150,7 +153,7 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
                 k::prolongate_add_bc<float, float>(s, d.level(level), d.level(fine), V, m_tmpf[fine], m_err1f[fine],
                                                    m_err1f[level]);
         });
-        smooth_fast_f(fine, fine == 0 ? r : nullptr, fine == 0 ? nullptr : m_res1f[fine], m_mixed_post);
+        smooth_fast_f(fine, fine == 0 ? r : nullptr, fine == 0 ? nullptr : m_res1f[fine], m_mixed_post, false);
     }
 }
 

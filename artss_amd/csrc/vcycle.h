@@ -110,7 +110,7 @@ class VCycleMG : public IPressure {
     double m_mixed_weight = 0.0;            // their Jacobi weight (0: the XML's)
     void alloc_mixed();
     void free_mixed();
-    void smooth_fast_f(int level, const double *b64, const float *b32, int sweeps);
+    void smooth_fast_f(int level, const double *b64, const float *b32, int sweeps, bool from_zero);
     bool m_use_graphs = true;
     CycleGraph m_graph[2];  // [0]: V-cycle, [1]: V-cycle + residual
 };
