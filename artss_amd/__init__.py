@@ -87,6 +87,8 @@ def lib():
         "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
         "artss_solver_set_warm_start": (C.c_int, [vp, C.c_int, vp, C.c_int]),
         "artss_solver_set_mixed_sweeps": (C.c_int, [vp, C.c_int, C.c_int, C.c_double]),
+        "artss_solver_set_coarse_solve": (C.c_int, [vp, C.c_int]),
+        "artss_solver_set_mixed_tail": (C.c_int, [vp, C.c_int]),
         "artss_solver_snapshot": (C.c_int, [vp, C.c_double, ip]),
         "artss_solver_restore": (C.c_int, [vp, C.c_int, dp]),
         "artss_solver_drop_snapshot": (C.c_int, [vp, C.c_int]),
@@ -467,6 +469,14 @@ class Solver:
     def set_mixed_sweeps(self, pre=0, post=0, weight=0.0):
         """method 2: smoothing sweeps of the FP32 levels and their Jacobi weight (0: the XML's)"""
         _check(self.L.artss_solver_set_mixed_sweeps(self.h, int(pre), int(post), float(weight)))
+
+    def set_coarse_solve(self, direct=True):
+        """method 2: dense exact solve of the preconditioner's coarse levels (True) or their V-cycle"""
+        _check(self.L.artss_solver_set_coarse_solve(self.h, 1 if direct else 0))
+
+    def set_mixed_tail(self, n_relax=0):
+        """method 2: n_relax of the preconditioner's coarse-level kernel (0: the XML's)"""
+        _check(self.L.artss_solver_set_mixed_tail(self.h, int(n_relax)))
 
     def set_mg_schedule(self, mode=2, tail_cells=4096):
         _check(self.L.artss_solver_set_mg_schedule(self.h, int(mode), int(tail_cells)))

@@ -666,6 +666,26 @@ int artss_solver_initial_residual(artss_solver *s, double *residual) {
     });
 }
 
+int artss_solver_set_mixed_tail(artss_solver *s, int n_relax) {
+    return guard([&] {
+        require(s && n_relax >= 0, "bad argument");
+        PressureMode m = s->solver->pressure_mode();
+        m.mixed_tail = n_relax;
+        s->solver->set_pressure_mode(m);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_set_coarse_solve(artss_solver *s, int direct) {
+    return guard([&] {
+        require(s && (direct == 0 || direct == 1), "bad argument");
+        PressureMode m = s->solver->pressure_mode();
+        m.coarse_direct = direct == 1;
+        s->solver->set_pressure_mode(m);
+        return ARTSS_OK;
+    });
+}
+
 int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post, double weight) {
     return guard([&] {
         require(s && pre >= 0 && post >= 0 && weight >= 0.0 && weight < 2.0, "bad argument");

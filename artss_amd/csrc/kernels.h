@@ -13,6 +13,10 @@ namespace k {
 
 // ---- element-wise over the whole array (Field.h:64-135) ----
 void copy(hipStream_t s, double *dst, const double *src, long long n);
+// x[cells[r]] = sum_j M[r * n + j] b[cells[j]] for r < n (n <= kMaxDenseRows:
+// the gathered b is staged in 16 KiB of LDS per workgroup)
+constexpr int kMaxDenseRows = 2048;
+void dense_apply(hipStream_t s, int n, const double *M, const uint32_t *cells, const double *b, double *x);
 // dst0[q] = dst1[q] = src[q] for q < fields (1..4), n elements each, one pass
 struct CopyFan {
     const double *src[4];

@@ -32,6 +32,15 @@ inline dim3 full_grid(const LevelDesc &L) {
     return dim3((L.N[0] + BX - 1) / BX, (L.N[1] + BY - 1) / BY, L.N[2]);
 }
 inline dim3 block() { return dim3(BX, BY, 1); }
+// The fused stencil kernels give each thread KZ consecutive cells along z:
+// with one cell per thread their waves live about a thousand cycles and the
+// workgroup launch rate, not memory, bounds a level-0 launch.
+// Small levels keep one cell per thread (their launches are short of waves
+// as it is); the count is a kernel argument.
+constexpr int KZ = 4;
+constexpr long long kBatchCells = 1LL << 20;  // levels with at least this many cells batch
+inline int kz_of(const LevelDesc &L) { return L.size >= kBatchCells ? KZ : 1; }
+inline dim3 zbatch(dim3 g, int kz) { return dim3(g.x, g.y, (g.z + kz - 1) / kz); }
 inline unsigned grid1d(long long n, int threads) {
     long long g = (n + threads - 1) / threads;
     return static_cast<unsigned>(g > 65536 ? 65536 : (g < 1 ? 1 : g));
@@ -492,7 +501,7 @@ __device__ inline bool is_inner(const LevelDesc &L, long long c, int &i, int &j,
 }
 
 // dst = src with the fill applied (copy of the whole array + fill in one launch)
-__global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict__ dst,
+__global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, int kz, double *__restrict__ dst,
                           const double *__restrict__ src) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
         auto val = [&](long long c) { return src[c]; };
@@ -503,14 +512,18 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict_
     main_block(main_z, bx, by, bz);
     const int i = bx * BX + static_cast<int>(threadIdx.x);
     const int j = by * BY + static_cast<int>(threadIdx.y);
-    const int k = bz;
     if (i >= L.N[0] || j >= L.N[1]) return;
-    const long long idx = lin(L, i, j, k);
-    if (dev::written(V, idx)) return;
-    const double v = src[idx];
-    dst[idx] = v;
-    if (i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k))
-        dev::face_ghosts(L, V, dst, idx, i, j, k, v);
+#pragma unroll
+    for (int q = 0; q < kz; q++) {
+        const int k = bz * kz + q;
+        if (k >= L.N[2]) break;
+        const long long idx = lin(L, i, j, k);
+        if (dev::written(V, idx)) continue;
+        const double v = src[idx];
+        dst[idx] = v;
+        if (i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k))
+            dev::face_ghosts(L, V, dst, idx, i, j, k, v);
+    }
 }
 
 // One Jacobi sweep of the multigrid smoother and the fill after it
@@ -521,7 +534,7 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, double *__restrict_
 // ZERO (with FIRST): src is zero everywhere and is not read, the update is
 // w beta dsign b, the value the sweep gives for src = 0.
 template <typename T, typename TB, bool FIRST, bool ZERO>
-__global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, T ax, T ay, T az, T beta, T dsign, T w,
+__global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int kz, T ax, T ay, T az, T beta, T dsign, T w,
                             T *__restrict__ dst, const T *__restrict__ src, const TB *__restrict__ b) {
     static_assert(FIRST || !ZERO, "a sweep from zero writes every cell");
     if (static_cast<int>(blockIdx.z) >= main_z) {
@@ -543,37 +556,42 @@ __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, T ax, T ay, T az,
     const int o = FIRST ? 0 : 1;
     const int i = o + bx * BX + static_cast<int>(threadIdx.x);
     const int j = o + by * BY + static_cast<int>(threadIdx.y);
-    const int k = o + bz;
     if (FIRST ? (i >= N0 || j >= N1) : (i > n0 || j > n1)) return;
+    const int kend = FIRST ? L.N[2] : n2 + 1;
     // 32-bit offsets (a level has fewer than 2^31 cells)
     const unsigned nj = static_cast<unsigned>(N0), nk = nj * static_cast<unsigned>(N1);
-    const unsigned idx = static_cast<unsigned>(i) + nj * static_cast<unsigned>(j) + nk * static_cast<unsigned>(k);
-    const bool in_inner_box = FIRST ? (i >= 1 && j >= 1 && k >= 1 && i <= n0 && j <= n1 && k <= n2) : true;
-    // the update is computed for every cell of the inner box before the
-    // obstacle test, so the stencil loads are in flight while it runs
-    // (the expression of JacobiDiffuse.cpp:292-300, as dev::jacobi_t)
-    T v = T(0);
-    if (in_inner_box) {
-        if (ZERO) {
-            v = w * (beta * (dsign * static_cast<T>(b[idx])));
-        } else {
-            const T out_h = beta * (dsign * static_cast<T>(b[idx]) + ax * (src[idx + 1] + src[idx - 1]) +
-                                    ay * (src[idx + nj] + src[idx - nj]) + az * (src[idx + nk] + src[idx - nk]));
-            v = (1 - w) * src[idx] + w * out_h;
+#pragma unroll
+    for (int q = 0; q < kz; q++) {
+        const int k = o + bz * kz + q;
+        if (k >= kend) break;
+        const unsigned idx = static_cast<unsigned>(i) + nj * static_cast<unsigned>(j) + nk * static_cast<unsigned>(k);
+        const bool in_inner_box = FIRST ? (i >= 1 && j >= 1 && k >= 1 && i <= n0 && j <= n1 && k <= n2) : true;
+        // the update is computed for every cell of the inner box before the
+        // obstacle test, so the stencil loads are in flight while it runs
+        // (the expression of JacobiDiffuse.cpp:292-300, as dev::jacobi_t)
+        T v = T(0);
+        if (in_inner_box) {
+            if (ZERO) {
+                v = w * (beta * (dsign * static_cast<T>(b[idx])));
+            } else {
+                const T out_h = beta * (dsign * static_cast<T>(b[idx]) + ax * (src[idx + 1] + src[idx - 1]) +
+                                        ay * (src[idx + nj] + src[idx - nj]) + az * (src[idx + nk] + src[idx - nk]));
+                v = (1 - w) * src[idx] + w * out_h;
+            }
         }
-    }
-    if (in_inner_box && !in_obstacle(L, i, j, k)) {
-        dst[idx] = v;
-        dev::face_ghosts(L, V, dst, static_cast<long long>(idx), i, j, k, v);
-    } else if (FIRST && !dev::written(V, idx)) {
-        dst[idx] = ZERO ? T(0) : src[idx];
+        if (in_inner_box && !in_obstacle(L, i, j, k)) {
+            dst[idx] = v;
+            dev::face_ghosts(L, V, dst, static_cast<long long>(idx), i, j, k, v);
+        } else if (FIRST && !dev::written(V, idx)) {
+            dst[idx] = ZERO ? T(0) : src[idx];
+        }
     }
 }
 
 // Residuum (VCycleMG.cpp:346-354) and the obstacle-face part of the fill
 // after it (VCycleMG.cpp:245); the ghosts of the residuum are never read.
 template <typename T, typename TB>
-__global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, T *__restrict__ out, const T *__restrict__ x,
+__global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, int kz, T *__restrict__ out, const T *__restrict__ x,
                                  const TB *__restrict__ b) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
         const long long t = item_index(main_z);
@@ -589,10 +607,15 @@ __global__ void k_residuum_faces(LevelDesc L, BCView V, int main_z, T *__restric
     main_block(main_z, bx, by, bz);
     const int i = 1 + bx * BX + static_cast<int>(threadIdx.x);
     const int j = 1 + by * BY + static_cast<int>(threadIdx.y);
-    const int k = 1 + bz;
-    if (i > L.n[0] || j > L.n[1] || in_obstacle(L, i, j, k)) return;
-    const long long idx = lin(L, i, j, k);
-    out[idx] = static_cast<T>(b[idx]) - dev::laplace_t<T>(L, x, idx);
+    if (i > L.n[0] || j > L.n[1]) return;
+#pragma unroll
+    for (int q = 0; q < kz; q++) {
+        const int k = 1 + bz * kz + q;
+        if (k > L.n[2]) break;
+        if (in_obstacle(L, i, j, k)) continue;
+        const long long idx = lin(L, i, j, k);
+        out[idx] = static_cast<T>(b[idx]) - dev::laplace_t<T>(L, x, idx);
+    }
 }
 
 // The correction of the ascent (VCycleMG.cpp:266-270) and the two fills at
@@ -622,7 +645,7 @@ __device__ inline Tf corrected(const LevelDesc &Lc, const LevelDesc &Lf, const T
                                 static_cast<int>(u / (Nx * Ny)), c);
 }
 template <typename Tf, typename Tc>
-__global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, Tf *__restrict__ dst,
+__global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, int kz, Tf *__restrict__ dst,
                                     const Tf *__restrict__ fine, const Tc *__restrict__ coarse) {
     if (static_cast<int>(blockIdx.z) >= main_z) {
         auto val = [&](long long c) { return corrected<Tf, Tc>(Lc, Lf, fine, coarse, c); };
@@ -633,30 +656,52 @@ __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int ma
     main_block(main_z, bx, by, bz);
     const int i = bx * BX + static_cast<int>(threadIdx.x);
     const int j = by * BY + static_cast<int>(threadIdx.y);
-    const int k = bz;
     if (i >= Lf.N[0] || j >= Lf.N[1]) return;
-    // 32-bit offsets (a level has fewer than 2^31 cells); the coarse and
-    // fine loads are issued before the obstacle and fill tests
-    const unsigned idx = static_cast<unsigned>(i) +
-                         static_cast<unsigned>(Lf.N[0]) * (static_cast<unsigned>(j) +
-                                                           static_cast<unsigned>(Lf.N[1]) * static_cast<unsigned>(k));
-    const int ci = (i + 1) / 2, cj = (j + 1) / 2, ck = (k + 1) / 2;
-    const bool reach = i >= 1 && j >= 1 && k >= 1 && ci <= Lc.n[0] && cj <= Lc.n[1] && ck <= Lc.n[2];
-    const Tf f = fine[idx];
-    Tf p = 0.0;
-    if (reach) {
-        const int q = ((i & 1) ? 4 : 0) | ((j & 1) ? 2 : 0) | ((k & 1) ? 1 : 0);
-        const unsigned cidx = static_cast<unsigned>(ci) +
-                              static_cast<unsigned>(Lc.N[0]) * (static_cast<unsigned>(cj) +
-                                                                static_cast<unsigned>(Lc.N[1]) * static_cast<unsigned>(ck));
-        const Tf pc = static_cast<Tf>(dev::prolong_t<Tc>(Lc, coarse, cidx, q));
-        if (!in_obstacle(Lc, ci, cj, ck)) p = pc;
+#pragma unroll
+    for (int qz = 0; qz < kz; qz++) {
+        const int k = bz * kz + qz;
+        if (k >= Lf.N[2]) break;
+        // 32-bit offsets (a level has fewer than 2^31 cells); the coarse and
+        // fine loads are issued before the obstacle and fill tests
+        const unsigned idx = static_cast<unsigned>(i) +
+                             static_cast<unsigned>(Lf.N[0]) * (static_cast<unsigned>(j) +
+                                                               static_cast<unsigned>(Lf.N[1]) * static_cast<unsigned>(k));
+        const int ci = (i + 1) / 2, cj = (j + 1) / 2, ck = (k + 1) / 2;
+        const bool reach = i >= 1 && j >= 1 && k >= 1 && ci <= Lc.n[0] && cj <= Lc.n[1] && ck <= Lc.n[2];
+        const Tf f = fine[idx];
+        Tf p = 0.0;
+        if (reach) {
+            const int q = ((i & 1) ? 4 : 0) | ((j & 1) ? 2 : 0) | ((k & 1) ? 1 : 0);
+            const unsigned cidx = static_cast<unsigned>(ci) +
+                                  static_cast<unsigned>(Lc.N[0]) *
+                                      (static_cast<unsigned>(cj) + static_cast<unsigned>(Lc.N[1]) * static_cast<unsigned>(ck));
+            const Tf pc = static_cast<Tf>(dev::prolong_t<Tc>(Lc, coarse, cidx, q));
+            if (!in_obstacle(Lc, ci, cj, ck)) p = pc;
+        }
+        if (dev::written(V, idx)) continue;
+        const Tf v = f + p;  // corrected_at's value
+        dst[idx] = v;
+        if (i >= 1 && j >= 1 && k >= 1 && i <= Lf.n[0] && j <= Lf.n[1] && k <= Lf.n[2] && !in_obstacle(Lf, i, j, k))
+            dev::face_ghosts(Lf, V, dst, static_cast<long long>(idx), i, j, k, v);
     }
-    if (dev::written(V, idx)) return;
-    const Tf v = f + p;  // corrected_at's value
-    dst[idx] = v;
-    if (i >= 1 && j >= 1 && k >= 1 && i <= Lf.n[0] && j <= Lf.n[1] && k <= Lf.n[2] && !in_obstacle(Lf, i, j, k))
-        dev::face_ghosts(Lf, V, dst, static_cast<long long>(idx), i, j, k, v);
+}
+
+// x[cells[r]] = sum_j M[r][j] b[cells[j]]: one wavefront per row, the
+// gathered right-hand side staged in LDS, a fixed summation order
+__global__ void k_dense_apply(int n, const double *__restrict__ M, const uint32_t *__restrict__ cells,
+                              const double *__restrict__ b, double *__restrict__ x) {
+    extern __shared__ double sb[];
+    for (int j = static_cast<int>(threadIdx.x); j < n; j += static_cast<int>(blockDim.x)) sb[j] = b[cells[j]];
+    __syncthreads();
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int row = static_cast<int>(blockIdx.x) * (static_cast<int>(blockDim.x) >> 6) +
+                    (static_cast<int>(threadIdx.x) >> 6);
+    if (row >= n) return;
+    const double *m = M + static_cast<size_t>(row) * static_cast<size_t>(n);
+    double acc = 0.0;
+    for (int j = lane; j < n; j += 64) acc += m[j] * sb[j];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) x[cells[row]] = acc;
 }
 
 // One idempotent fill in a single launch (faces, ghosts, dependent ghosts)
@@ -762,6 +807,10 @@ __global__ void k_buoyancy(LevelDesc L, double beta, double g, double *__restric
 void copy(hipStream_t s, double *dst, const double *src, long long n) {
     k_copy<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n);
 }
+void dense_apply(hipStream_t s, int n, const double *M, const uint32_t *cells, const double *b, double *x) {
+    if (n < 1 || n > kMaxDenseRows) throw std::invalid_argument("dense_apply: 1 to kMaxDenseRows rows");
+    k_dense_apply<<<(n + 3) / 4, 256, sizeof(double) * static_cast<size_t>(n), s>>>(n, M, cells, b, x);
+}
 void copy_fan(hipStream_t s, const CopyFan &f, long long n) {
     if (f.fields < 1 || f.fields > 4) throw std::invalid_argument("copy_fan: 1 to 4 fields");
     k_copy_fan<<<grid1d(n, 256), 256, 0, s>>>(f, n);
@@ -859,25 +908,27 @@ void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, dou
 }
 
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src) {
-    const dim3 g = full_grid(L);
-    k_copy_bc<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), dst, src);
+    const int kz = kz_of(L);
+    const dim3 g = zbatch(full_grid(L), kz);
+    k_copy_bc<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, dst, src);
 }
 template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
                double ax, double ay, double az, double beta, double dsign, double w) {
     if (src == nullptr && !first) throw std::invalid_argument("jacobi_bc: a sweep from zero must be a first sweep");
-    const dim3 g = first ? full_grid(L) : inner_grid(L);
+    const int kz = kz_of(L);
+    const dim3 g = zbatch(first ? full_grid(L) : inner_grid(L), kz);
     const dim3 gi = with_items(g, dev::bc_items_grid(V));
     const T ax_ = static_cast<T>(ax), ay_ = static_cast<T>(ay), az_ = static_cast<T>(az), beta_ = static_cast<T>(beta),
             ds_ = static_cast<T>(dsign), w_ = static_cast<T>(w);
     if (first && src == nullptr)
-        k_jacobi_bc<T, TB, true, true><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_,
+        k_jacobi_bc<T, TB, true, true><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, ax_, ay_, az_, beta_, ds_,
                                                               w_, dst, src, b);
     else if (first)
-        k_jacobi_bc<T, TB, true, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_, ds_,
+        k_jacobi_bc<T, TB, true, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, ax_, ay_, az_, beta_, ds_,
                                                                w_, dst, src, b);
     else
-        k_jacobi_bc<T, TB, false, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), ax_, ay_, az_, beta_,
+        k_jacobi_bc<T, TB, false, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, ax_, ay_, az_, beta_,
                                                                 ds_, w_, dst, src, b);
 }
 template void jacobi_bc<double, double>(hipStream_t, const LevelDesc &, const BCView &, bool, double *,
@@ -890,8 +941,9 @@ template void jacobi_bc<float, float>(hipStream_t, const LevelDesc &, const BCVi
 
 template <typename T, typename TB>
 void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, T *out, const T *x, const TB *b) {
-    const dim3 g = inner_grid(L);
-    k_residuum_faces<T, TB><<<with_items(g, V.n_faces), block(), 0, s>>>(L, V, static_cast<int>(g.z), out, x, b);
+    const int kz = kz_of(L);
+    const dim3 g = zbatch(inner_grid(L), kz);
+    k_residuum_faces<T, TB><<<with_items(g, V.n_faces), block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, out, x, b);
 }
 template void residuum_faces<double, double>(hipStream_t, const LevelDesc &, const BCView &, double *,
                                              const double *, const double *);
@@ -903,8 +955,9 @@ template void residuum_faces<float, float>(hipStream_t, const LevelDesc &, const
 template <typename Tf, typename Tc>
 void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, Tf *dst,
                        const Tf *fine, const Tc *coarse) {
-    const dim3 g = full_grid(Lf);
-    k_prolongate_add_bc<Tf, Tc><<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z),
+    const int kz = kz_of(Lf);
+    const dim3 g = zbatch(full_grid(Lf), kz);
+    k_prolongate_add_bc<Tf, Tc><<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z), kz,
                                                                                      dst, fine, coarse);
 }
 template void prolongate_add_bc<double, double>(hipStream_t, const LevelDesc &, const LevelDesc &, const BCView &,

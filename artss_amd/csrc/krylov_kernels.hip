@@ -25,6 +25,13 @@ inline dim3 inner_grid(const LevelDesc &L) {
 inline dim3 full_grid(const LevelDesc &L) {
     return dim3((L.N[0] + BX - 1) / BX, (L.N[1] + BY - 1) / BY, L.N[2]);
 }
+// the update gives each thread kUpdateZ consecutive cells along z (waves of
+// one cell each are bound by the workgroup launch rate at level 0)
+constexpr int kUpdateZ = 4;
+inline dim3 update_grid(const LevelDesc &L) {
+    const dim3 g = full_grid(L);
+    return dim3(g.x, g.y, (g.z + kUpdateZ - 1) / kUpdateZ);
+}
 
 __device__ inline int block_id() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 __device__ inline int block_count() { return gridDim.x * gridDim.y * gridDim.z; }
@@ -172,22 +179,26 @@ __global__ void k_gcr_update(LevelDesc L, const double *__restrict__ coef, doubl
     xcd_block(bx, by, bz);
     const int i = bx * BX + static_cast<int>(threadIdx.x);
     const int j = by * BY + static_cast<int>(threadIdx.y);
-    const int k = bz;
     double v[1] = {0.0};
     if (i < L.N[0] && j < L.N[1]) {
-        const long long idx = dev::lin(L, i, j, k);
-        const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
-                           !in_obstacle(L, i, j, k);
-        const double w = inner ? lap(L, z, idx) : 0.0;
         const double beta = coef[0], inv = coef[1], alpha = coef[2];
-        const double zn = (double(z[idx]) - beta * double(zp[idx])) * inv;
-        const double wn = (w - beta * double(wp[idx])) * inv;
-        x[idx] += alpha * zn;
-        const double rn = r[idx] - alpha * wn;
-        r[idx] = rn;
-        zp[idx] = static_cast<Tw>(zn);
-        wp[idx] = static_cast<Tw>(wn);
-        v[0] = rn * rn;
+#pragma unroll
+        for (int q = 0; q < kUpdateZ; q++) {
+            const int k = bz * kUpdateZ + q;
+            if (k >= L.N[2]) break;
+            const long long idx = dev::lin(L, i, j, k);
+            const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
+                               !in_obstacle(L, i, j, k);
+            const double w = inner ? lap(L, z, idx) : 0.0;
+            const double zn = (double(z[idx]) - beta * double(zp[idx])) * inv;
+            const double wn = (w - beta * double(wp[idx])) * inv;
+            x[idx] += alpha * zn;
+            const double rn = r[idx] - alpha * wn;
+            r[idx] = rn;
+            zp[idx] = static_cast<Tw>(zn);
+            wp[idx] = static_cast<Tw>(wn);
+            v[0] += rn * rn;
+        }
     }
     write_partials<1>(v, partial);
 }
@@ -328,7 +339,7 @@ int blocks_dots(const LevelDesc &L) {
     return static_cast<int>(g.x * g.y * g.z);
 }
 int blocks_full(const LevelDesc &L) {
-    const dim3 g = full_grid(L);
+    const dim3 g = update_grid(L);
     return static_cast<int>(g.x * g.y * g.z);
 }
 
@@ -349,7 +360,7 @@ void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef, co
 template <typename Tz, typename Tw>
 void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, Tw *zp,
                 Tw *wp, double *partial, const GcrGate *gate) {
-    k_gcr_update<Tz, Tw><<<full_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, wp, partial, gate);
+    k_gcr_update<Tz, Tw><<<update_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, wp, partial, gate);
 }
 template void gcr_update<double, double>(hipStream_t, const LevelDesc &, const double *, double *, double *,
                                          const double *, double *, double *, double *, const GcrGate *);

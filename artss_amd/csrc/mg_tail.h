@@ -11,7 +11,10 @@ namespace artss {
 // (device-resident plan) with `b_first` as the right-hand side of the first
 // level; one workgroup, enqueued on `s`.
 // max_iter > 0 replaces the coarse solve's iteration cap of the plan
-void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter = 0);
+// max_iter, n_relax, w > 0 replace the plan's coarse-solve cap, smoothing
+// count and Jacobi weight for this launch
+void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter = 0,
+             int n_relax = 0, double w = 0.0);
 // dynamic LDS (bytes) a workgroup of the kernel can use for level arrays
 int mg_tail_lds_limit();
 // allow launches with this much dynamic LDS (once, before the first launch)

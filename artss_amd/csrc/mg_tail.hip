@@ -313,7 +313,7 @@ __device__ void t_solve(const TailLevel &T, const TailPlan &P, double *out, doub
 }
 
 __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__restrict__ plan, const double *b_first,
-                                                           int max_iter) {
+                                                           int max_iter, int n_relax, double w) {
     __shared__ TailPlan P;
     __shared__ double red[kTailThreads / 64 + 1];
     extern __shared__ double lds[];
@@ -324,6 +324,8 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__rest
         for (int t = threadIdx.x; t < words; t += kTailThreads) dst[t] = src[t];
         __syncthreads();
         if (max_iter > 0 && threadIdx.x == 0) P.max_iter = max_iter;
+        if (n_relax > 0 && threadIdx.x == 0) P.n_relax = n_relax;
+        if (w > 0.0 && threadIdx.x == 0) P.w = w;
         __syncthreads();
     }
     const int f = P.first, Lm = P.last;
@@ -397,8 +399,9 @@ void mg_tail_reserve_lds(int lds_bytes) {
         reserved = lds_bytes;
 }
 
-void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter) {
-    k_mg_tail<<<1, kTailThreads, lds_bytes, s>>>(plan, b_first, max_iter);
+void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter, int n_relax,
+             double w) {
+    k_mg_tail<<<1, kTailThreads, lds_bytes, s>>>(plan, b_first, max_iter, n_relax, w);
 }
 
 }  // namespace artss

@@ -276,15 +276,21 @@ void Solver::pressure_step(double t) {
         const bool mixed = m_pmode.method == 2;
         if (!m_gcr || m_gcr_mixed != mixed || m_gcr_coarse != m_pmode.coarse_max_iter ||
             m_gcr_pre != m_pmode.mixed_pre || m_gcr_post != m_pmode.mixed_post ||
-            m_gcr_weight != m_pmode.mixed_weight) {
+            m_gcr_weight != m_pmode.mixed_weight || m_gcr_direct != m_pmode.coarse_direct ||
+            m_gcr_tail != m_pmode.mixed_tail) {
             m_gcr.reset();
             m_pres->set_mixed_sweeps(m_pmode.mixed_pre, m_pmode.mixed_post, m_pmode.mixed_weight);
+            m_pres->set_mixed_tail(m_pmode.mixed_tail);
+            if (m_gcr_direct != m_pmode.coarse_direct || !m_gcr_built) m_pres->set_direct_coarse(m_pmode.coarse_direct);
             m_gcr = std::make_unique<GcrPressure>(*m_domain, *m_pres, mixed, m_pmode.coarse_max_iter);
             m_gcr_mixed = mixed;
             m_gcr_coarse = m_pmode.coarse_max_iter;
             m_gcr_pre = m_pmode.mixed_pre;
             m_gcr_post = m_pmode.mixed_post;
             m_gcr_weight = m_pmode.mixed_weight;
+            m_gcr_direct = m_pmode.coarse_direct;
+            m_gcr_tail = m_pmode.mixed_tail;
+            m_gcr_built = true;
         }
         const bool warm = m_warm.mode != 0 || m_warm.record;
         if (warm) warm_begin();

@@ -58,6 +58,8 @@ struct PressureMode {
     int coarse_max_iter = 0;  // method 2: cap of the preconditioner's coarse solve (0: the XML's)
     int mixed_pre = 0, mixed_post = 0;  // method 2: smoothing sweeps above the coarse levels (0: the XML's)
     double mixed_weight = 0.0;          // method 2: their Jacobi weight (0: the XML's)
+    bool coarse_direct = true;          // method 2: dense exact coarse solve where it applies (VCycleMG)
+    int mixed_tail = 0;                 // method 2: n_relax of the coarse levels' kernel (0: the XML's)
 };
 
 // Where the residual-target GCR pressure iteration starts (methods 1, 2),
@@ -150,6 +152,8 @@ class Solver {
     bool m_gcr_mixed = false;
     int m_gcr_coarse = 0, m_gcr_pre = 0, m_gcr_post = 0;
     double m_gcr_weight = 0.0;
+    bool m_gcr_direct = true, m_gcr_built = false;
+    int m_gcr_tail = 0;
     std::unique_ptr<ExplicitEulerSource> m_src_vel, m_src_temp;
     std::unique_ptr<GaussFunction> m_gauss;
     // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver, 3 NSTempSolver,

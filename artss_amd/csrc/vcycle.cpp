@@ -1,7 +1,9 @@
 #include "vcycle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "mg_tail.h"
 
@@ -33,6 +35,7 @@ VCycleMG::VCycleMG(Domain &d, const MultigridSettings &s) : IPressure(d), m_s(s)
 VCycleMG::~VCycleMG() {
     drop_graphs();
     free_mixed();
+    free_direct();
     if (m_plan_dev) (void)hipFree(m_plan_dev);
 }
 
@@ -107,6 +110,164 @@ void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int
     }
 }
 
+namespace {
+
+// In-place LU with partial pivoting of the n x n row-major A, then its
+// inverse by n forward/back substitutions.  false when a pivot falls below
+// 1e-12 of the largest entry (singular for this purpose).
+bool invert_dense(std::vector<double> &A, int n, std::vector<double> &inv) {
+    const size_t N = static_cast<size_t>(n);
+    double scale = 0.0;
+    for (double v : A) scale = std::max(scale, std::fabs(v));
+    if (!(scale > 0.0)) return false;
+    std::vector<int> piv(N);
+    for (size_t k = 0; k < N; k++) {
+        size_t p = k;
+        for (size_t i = k + 1; i < N; i++)
+            if (std::fabs(A[i * N + k]) > std::fabs(A[p * N + k])) p = i;
+        if (!(std::fabs(A[p * N + k]) > 1e-12 * scale)) return false;
+        piv[k] = static_cast<int>(p);
+        if (p != k)
+            for (size_t j = 0; j < N; j++) std::swap(A[k * N + j], A[p * N + j]);
+        const double d = A[k * N + k];
+        const double *rk = &A[k * N];
+        for (size_t i = k + 1; i < N; i++) {
+            double *ri = &A[i * N];
+            const double l = ri[k] / d;
+            ri[k] = l;
+            for (size_t j = k + 1; j < N; j++) ri[j] -= l * rk[j];
+        }
+    }
+    // columns of the inverse: P A = L U, A^-1 e_c = U^-1 L^-1 P e_c
+    inv.assign(N * N, 0.0);
+    std::vector<double> y(N);
+    for (size_t c = 0; c < N; c++) {
+        std::fill(y.begin(), y.end(), 0.0);
+        y[c] = 1.0;
+        for (size_t k = 0; k < N; k++) std::swap(y[k], y[static_cast<size_t>(piv[k])]);
+        for (size_t i = 0; i < N; i++) {
+            double v = y[i];
+            const double *ri = &A[i * N];
+            for (size_t j = 0; j < i; j++) v -= ri[j] * y[j];
+            y[i] = v;
+        }
+        for (size_t i = N; i-- > 0;) {
+            double v = y[i];
+            const double *ri = &A[i * N];
+            for (size_t j = i + 1; j < N; j++) v -= ri[j] * y[j];
+            y[i] = v / ri[i];
+        }
+        for (size_t r = 0; r < N; r++) inv[r * N + c] = y[r];
+    }
+    return true;
+}
+
+}  // namespace
+
+void VCycleMG::set_direct_coarse(bool on) {
+    drop_graphs();
+    free_direct();
+    m_direct_enabled = on;
+    m_direct_tried = false;
+}
+
+void VCycleMG::free_direct() {
+    if (m_direct) (void)hipFree(m_direct);
+    if (m_direct_cells) (void)hipFree(m_direct_cells);
+    m_direct = nullptr;
+    m_direct_cells = nullptr;
+    m_direct_n = 0;
+}
+
+void VCycleMG::prepare_mixed() {
+    alloc_mixed();
+    if (m_direct_enabled && !m_direct_tried) {
+        m_direct_tried = true;
+        if (!build_direct()) free_direct();
+    }
+}
+
+// Column j of K is (e_j - J e_j) / beta on the fluid cells, J e_j one
+// Jacobi sweep (weight 1, b = 0) of e_j carrying its linear fill: the sweep
+// gives beta N e_j, N the neighbour sum, and the Jacobi iteration's fixed
+// point solves (I / beta - N) x = dsign b.  With a constant null space
+// (every row sum zero: Neumann or periodic everywhere) the mean diagonal
+// times the averaging matrix is added, which leaves K^-1 b the solution
+// with the mean of the cells removed as the null-space component.
+bool VCycleMG::build_direct() {
+    Domain &d = m_domain;
+    const int f = m_tail_first;
+    if (f < 1 || f > m_s.n_level || !fast_level(f)) return false;
+    const std::vector<uint64_t> cells = d.inner_list(f);
+    const int n = static_cast<int>(cells.size());
+    if (n < 1 || n > kMaxDirectCells || n > k::kMaxDenseRows) return false;
+    const LevelDesc &L = d.level(f);
+    // obstacles at this level: their coarse faces make the level's operator
+    // a poor stand-in for the finer ones near them, and the exact solve of it
+    // (unlike the smoothed single-workgroup V-cycle) stalls the iteration
+    if (static_cast<long long>(n) != static_cast<long long>(L.n[0]) * L.n[1] * L.n[2]) return false;
+    const BCView V = d.bc_view(f, P, true);
+    double ax, ay, az, beta;
+    coeffs(f, &ax, &ay, &az, &beta);
+    const size_t size = static_cast<size_t>(L.size);
+    hipStream_t s = d.stream();
+    double *src = nullptr, *dst = nullptr, *zero = nullptr;
+    ARTSS_HIP(hipMalloc(&src, sizeof(double) * size));
+    ARTSS_HIP(hipMalloc(&dst, sizeof(double) * size));
+    ARTSS_HIP(hipMalloc(&zero, sizeof(double) * size));
+    const size_t N = static_cast<size_t>(n);
+    std::vector<double> K(N * N), col(size);
+    try {
+        k::fill(s, zero, 0.0, L.size);
+        for (size_t j = 0; j < N; j++) {
+            k::fill(s, src, 0.0, L.size);
+            k::fill(s, src + cells[j], 1.0, 1);
+            k::fill_bc<double>(s, L, V, src);
+            k::jacobi_bc<double, double>(s, L, V, true, dst, src, zero, ax, ay, az, beta, kDsign, 1.0);
+            ARTSS_HIP(hipMemcpyAsync(col.data(), dst, sizeof(double) * size, hipMemcpyDeviceToHost, s));
+            ARTSS_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < N; i++) K[i * N + j] = ((i == j ? 1.0 : 0.0) - col[cells[i]]) / beta;
+        }
+    } catch (...) {
+        (void)hipFree(src);
+        (void)hipFree(dst);
+        (void)hipFree(zero);
+        throw;
+    }
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    (void)hipFree(zero);
+    double diag = 0.0, rows = 0.0;
+    for (size_t i = 0; i < N; i++) {
+        double r = 0.0;
+        for (size_t j = 0; j < N; j++) r += K[i * N + j];
+        rows = std::max(rows, std::fabs(r));
+        diag += K[i * N + i];
+    }
+    diag /= static_cast<double>(n);
+    if (rows <= 1e-9 * diag) {
+        const double c = diag / static_cast<double>(n);
+        for (double &v : K) v += c;
+    }
+    std::vector<double> inv;
+    if (!invert_dense(K, n, inv)) return false;
+    for (double &v : inv) v *= kDsign;
+    std::vector<uint32_t> idx(N);
+    for (size_t i = 0; i < N; i++) idx[i] = static_cast<uint32_t>(cells[i]);
+    ARTSS_HIP(hipMalloc(&m_direct, sizeof(double) * N * N));
+    ARTSS_HIP(hipMalloc(&m_direct_cells, sizeof(uint32_t) * N));
+    ARTSS_HIP(hipMemcpy(m_direct, inv.data(), sizeof(double) * N * N, hipMemcpyHostToDevice));
+    ARTSS_HIP(hipMemcpy(m_direct_cells, idx.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
+    m_direct_n = n;
+    return true;
+}
+
+void VCycleMG::set_mixed_tail(int n_relax) {
+    if (n_relax < 0) throw config_error("sweep counts must not be negative");
+    drop_graphs();
+    m_mixed_tail = n_relax;
+}
+
 void VCycleMG::set_mixed_sweeps(int pre, int post, double weight) {
     if (pre < 0 || post < 0) throw config_error("sweep counts must not be negative");
     if (!(weight >= 0.0 && weight < 2.0)) throw config_error("the Jacobi weight must lie in [0, 2)");
@@ -141,7 +302,18 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
                                                         m_res0f[level], m_err1[level + 1]->data);
         });
     }
-    d.timed("mg_tail", f, [&] { mg_tail(s, m_plan_dev, m_res1[f]->data, m_tail_lds_bytes, coarse_max_iter); });
+    if (m_direct) {
+        // restrict_fill zeroed err1[f]; the solve writes its fluid cells, the fill the rest it sets
+        d.timed("coarse_direct", f, [&] {
+            k::dense_apply(s, m_direct_n, m_direct, m_direct_cells, m_res1[f]->data, m_err1[f]->data);
+            k::fill_bc<double>(s, d.level(f), d.bc_view(f, P, true), m_err1[f]->data);
+        });
+    } else {
+        d.timed("mg_tail", f, [&] {
+            mg_tail(s, m_plan_dev, m_res1[f]->data, m_tail_lds_bytes, coarse_max_iter, m_mixed_tail,
+                    m_mixed_tail > 0 ? m_mixed_weight : 0.0);
+        });
+    }
     for (int level = f; level > 0; --level) {
         const int fine = level - 1;
         const BCView V = d.bc_view(fine, P, true);
@@ -160,6 +332,8 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
 void VCycleMG::set_schedule(bool fused, long long tail_cells) {
     drop_graphs();
     free_mixed();
+    free_direct();
+    m_direct_tried = false;
     m_fused = fused;
     m_tail_cells = tail_cells < 0 ? 0 : tail_cells;
     build_tail();

@@ -51,14 +51,28 @@ class VCycleMG : public IPressure {
     // restriction, prolongation), the coarse levels and their solve in FP64.
     // The result stays in mixed_result() (FP32, level 0) until the next call.
     bool mixed_supported() const;
-    void prepare_mixed() { alloc_mixed(); }  // allocates; call before capturing vcycle_mixed
+    void prepare_mixed();  // allocates (and assembles the direct solve); call before capturing vcycle_mixed
     // pre- and post-smoothing sweeps of the mixed V-cycle on the levels
     // above its single-workgroup kernel (0: the XML's effective count) and
     // their Jacobi weight (0: the XML's)
     void set_mixed_sweeps(int pre, int post, double weight = 0.0);
+    // n_relax of the single-workgroup kernel in the mixed V-cycle (0: the
+    // XML's); when set, its sweeps use the mixed Jacobi weight as well
+    void set_mixed_tail(int n_relax);
     // coarse_max_iter > 0 caps the coarse solve's Jacobi iterations
     void vcycle_mixed(const double *r, int coarse_max_iter = 0);
     float *mixed_result() const { return m_err1f.empty() ? nullptr : m_err1f[0]; }
+    // The coarse part of the mixed V-cycle (the levels of the single-workgroup
+    // kernel) replaced by the exact solve on its first level, x = K^-1 dsign b
+    // with K the level's operator and linear fill, applied as a dense matrix
+    // (k::dense_apply) when the level has at most kMaxDirectCells fluid cells,
+    // uses the fused fill, holds no obstacle cells and K is not singular
+    // beyond a constant null space.
+    // On by default; set before the first mixed V-cycle (prepare_mixed
+    // assembles it; a GcrPressure built afterwards captures the choice).
+    static constexpr int kMaxDirectCells = 1536;
+    void set_direct_coarse(bool on);
+    bool direct_coarse() const { return m_direct != nullptr; }
 
     // graphs need the fused schedule with the single-workgroup kernel (no
     // host round trip inside a V-cycle); while a kernel timer is armed the
@@ -108,9 +122,17 @@ class VCycleMG : public IPressure {
     std::vector<float *> m_err1f, m_tmpf, m_scrf, m_res0f, m_res1f;
     int m_mixed_pre = 0, m_mixed_post = 0;  // sweeps of the mixed V-cycle (0: the XML's)
     double m_mixed_weight = 0.0;            // their Jacobi weight (0: the XML's)
+    int m_mixed_tail = 0;                   // n_relax of the tail kernel in the mixed V-cycle (0: the XML's)
     void alloc_mixed();
     void free_mixed();
     void smooth_fast_f(int level, const double *b64, const float *b32, int sweeps, bool from_zero);
+    bool build_direct();
+    void free_direct();
+    bool m_direct_enabled = true;
+    bool m_direct_tried = false;
+    double *m_direct = nullptr;           // n x n row-major, dsign K^-1
+    uint32_t *m_direct_cells = nullptr;   // the level's fluid cells (row/column order)
+    int m_direct_n = 0;
     bool m_use_graphs = true;
     CycleGraph m_graph[2];  // [0]: V-cycle, [1]: V-cycle + residual
 };

@@ -79,6 +79,10 @@ def parse():
                     help="mixed: pre,post smoothing sweeps of the FP32 levels (0: the XML's effective count)")
     ap.add_argument("--weight", type=float, default=0.8,
                     help="mixed: Jacobi weight of those sweeps (0: the XML's)")
+    ap.add_argument("--coarse", choices=("direct", "tail"), default="direct",
+                    help="mixed: exact dense solve of the coarse levels, or their single-workgroup V-cycle")
+    ap.add_argument("--tail-sweeps", type=int, default=0,
+                    help="mixed: n_relax of the coarse levels' single-workgroup kernel (0: the XML's)")
     ap.add_argument("--tail-cells", type=int, default=4096,
                     help="levels of at most this many cells run in one single-workgroup kernel")
     ap.add_argument("--warm", choices=("none", "extrapolate"), default="extrapolate",
@@ -128,6 +132,8 @@ def make_solver(artss_amd, args, pressure, warm="none"):
         if pressure == "mixed":
             pre, post = (int(v) for v in args.sweeps.split(","))
             solver.set_mixed_sweeps(pre, post, args.weight)
+            solver.set_coarse_solve(args.coarse == "direct")
+            solver.set_mixed_tail(args.tail_sweeps)
     return solver
 
 
@@ -291,6 +297,7 @@ def main():
                    "warm_start": ("previous p" if args.warm == "none" or args.pressure == "vcycle" else
                                   "p + (p - p_prev), linear extrapolation in time"),
                    "coarse_iterations": args.coarse_iterations, "sweeps": args.sweeps, "weight": args.weight,
+                   "coarse": args.coarse, "tail_sweeps": args.tail_sweeps,
                    "pressure": ("reference V-cycle schedule" if args.reference_schedule else
                                 "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance),
                    "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(

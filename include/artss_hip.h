@@ -168,6 +168,21 @@ int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cell
  * target. */
 int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post, double weight);
 
+/* Method 2 only (no reference counterpart): direct = 1 (the default)
+ * replaces the coarse levels of the preconditioner (those of the
+ * single-workgroup kernel) by the exact solve on their finest level, a dense
+ * inverse assembled once on the host when that level has at most 1536 cells
+ * and no obstacle cells (otherwise, and with direct = 0, the single-workgroup V-cycle and
+ * Jacobi coarse solve run).  The iteration still meets the same FP64
+ * residual target. */
+int artss_solver_set_coarse_solve(artss_solver *s, int direct);
+
+/* Method 2 only (no reference counterpart): n_relax of the preconditioner's
+ * coarse levels (the single-workgroup kernel), whose sweeps then use the
+ * weight of artss_solver_set_mixed_sweeps; 0 (the default) keeps the XML's
+ * n_relax and weight. */
+int artss_solver_set_mixed_tail(artss_solver *s, int n_relax);
+
 /* Start of the residual-target GCR pressure iteration (methods 1 and 2; no
  * reference counterpart: the reference's iteration starts from the previous
  * step's p, mode 0).  With p_n the previous solution, p_prev the one before

@@ -59,3 +59,47 @@ def test_gcr_graph_replay_matches_eager(method):
     assert c1 == c2
     for f in f1:
         assert np.array_equal(f1[f], f2[f]), f
+
+
+def _run_coarse(xml, direct, tol, steps):
+    s = artss_amd.Solver(xml)
+    s.set_mg_schedule(2, 4096)
+    s.set_pressure_mode(True, tol, 400)
+    s.set_pressure_method(2)
+    s.set_coarse_solve(direct)
+    counts = []
+    for step in range(1, steps + 1):
+        s.step(step * 0.01)
+        counts.append(s.counters())
+    return {f: s.field(f) for f in ("u", "v", "w", "p", "T")}, counts
+
+
+@pytest.mark.parametrize("open_top", [True, False], ids=["open_top", "closed_box"])
+@pytest.mark.parametrize("obstacles", [OBST, []], ids=["obstacle", "open"])
+def test_direct_coarse_solve_meets_target(obstacles, open_top):
+    """The dense exact coarse solve (1024 cells at the 16x8x8 level; a
+    constant null space in the closed box) against the single-workgroup
+    V-cycle: both reach the target, the fields agree to what it allows and
+    the exact solve needs fewer iterations.  With an obstacle cell on that
+    level the exact solve does not apply and the V-cycle runs unchanged."""
+    xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=4, steps=4, dt=0.01, obstacles=obstacles, hrr=3.0,
+                      source_x=1.0, open_top=open_top)
+    tol = 1e-9
+    f_t, c_t = _run_coarse(xml, False, tol, 4)
+    f_d, c_d = _run_coarse(xml, True, tol, 4)
+    cyc_t = [c["vcycles"] for c in c_t]
+    cyc_d = [c["vcycles"] for c in c_d]
+    errs = {f: float(np.abs(f_d[f] - f_t[f]).max() / max(np.abs(f_t[f]).max(), 1e-30)) for f in f_t}
+    print("tail", cyc_t, "direct", cyc_d, "residuals", [c["residual"] for c in c_d], "rel err", errs)
+    if obstacles:
+        # (the closed box with this obstacle does not reach 1e-9 by either
+        # coarse solve: its restricted right-hand sides are not consistent)
+        assert c_d == c_t
+        for f in f_t:
+            assert np.array_equal(f_d[f], f_t[f]), f
+    else:
+        for c in c_d:
+            assert 0 <= c["residual"] <= tol
+        assert sum(cyc_d) < sum(cyc_t)
+        for f in f_t:
+            assert errs[f] < 1e-5, f
