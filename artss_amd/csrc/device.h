@@ -84,6 +84,10 @@ struct BCView {
     // the dependent ones
     const uint32_t *rest;
     long long n_rest;
+    // every ghost cell is written by the fill: kernels that must write every
+    // cell (first sweeps, the out-of-place prolongation, copy_bc) can then
+    // run over the inner box only
+    int shell_filled;
 };
 
 #if defined(__HIPCC__)

@@ -112,6 +112,18 @@ Domain::Domain(const Settings &s, int device) : m_settings(s), m_device(device) 
     if (m_levels < 0 || m_levels > 16) throw config_error("multigrid levels out of range");
     build_levels();
     build_obstacles();
+    // fluid cells per level, counted once (the pressure solve's warm start
+    // asks for them every step)
+    m_inner_count.resize(m_level.size());
+    for (size_t l = 0; l < m_level.size(); l++) {
+        const LevelDesc &L = m_level[l];
+        size_t n = 0;
+        for (int k = 1; k <= L.n[2]; k++)
+            for (int j = 1; j <= L.n[1]; j++)
+                for (int i = 1; i <= L.n[0]; i++)
+                    if (!in_obstacle(L, i, j, k)) n++;
+        m_inner_count[l] = n;
+    }
     if (device >= 0) {
         ARTSS_HIP(hipSetDevice(device));
         ARTSS_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking));
@@ -313,6 +325,7 @@ void Domain::build_boundary_tables() {
     m_idem.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 0));
     m_face_values_zero.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 1));
     m_view_bits.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
+    m_shell.assign(m_levels + 1, std::vector<char>(NUMBER_OF_FIELD_TYPES, 0));
     m_view_deps.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
     m_view_ndeps.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
     m_view_rest.assign(m_levels + 1, std::vector<long long>(NUMBER_OF_FIELD_TYPES, 0));
@@ -491,6 +504,18 @@ void Domain::build_boundary_tables() {
                 bits.resize(b0 + words, 0u);
                 for (size_t c = 0; c < written.size(); c++)
                     if (written[c]) bits[b0 + c / 32] |= 1u << (c % 32);
+                bool shell = true;
+                for (long long k = 0; k < L.N[2] && shell; k++)
+                    for (long long j = 0; j < L.N[1] && shell; j++)
+                        for (long long i = 0; i < L.N[0]; i++) {
+                            const bool ghost = i == 0 || j == 0 || k == 0 || i == L.N[0] - 1 || j == L.N[1] - 1 ||
+                                               k == L.N[2] - 1;
+                            if (ghost && !written[static_cast<size_t>(i + L.N[0] * (j + L.N[1] * k))]) {
+                                shell = false;
+                                break;
+                            }
+                        }
+                m_shell[l][f] = shell ? 1 : 0;
                 if (!m_waves[l][f].empty()) {
                     std::unordered_map<uint64_t, long long> entry_of;
                     const Wave &W0 = m_waves[l][f][0];
@@ -590,6 +615,7 @@ BCView Domain::bc_view(int level, int ft, bool homogeneous) const {
     if (homogeneous)
         for (double &x : v.bc.value) x = 0.0;
     v.n_ghosts = v.bc.active ? ghost_cells(level) : 0;
+    v.shell_filled = m_shell[level][ft];
     return v;
 }
 
@@ -665,15 +691,7 @@ double Domain::reduce(const double *partial, int n) {
     return *m_sum_host;
 }
 
-size_t Domain::inner_cells(int level) const {
-    const LevelDesc &L = m_level[level];
-    size_t n = 0;
-    for (int k = 1; k <= L.n[2]; k++)
-        for (int j = 1; j <= L.n[1]; j++)
-            for (int i = 1; i <= L.n[0]; i++)
-                if (!in_obstacle(L, i, j, k)) n++;
-    return n;
-}
+size_t Domain::inner_cells(int level) const { return m_inner_count.at(static_cast<size_t>(level)); }
 
 std::vector<uint64_t> Domain::inner_list(int level) const {
     const LevelDesc &L = m_level[level];

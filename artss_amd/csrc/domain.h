@@ -141,6 +141,7 @@ class Domain {
     int m_device = 0;
     int m_levels = 0;
     std::vector<LevelDesc> m_level;
+    std::vector<size_t> m_inner_count;  // fluid cells per level
     std::vector<std::vector<ObstacleLevel>> m_obst;  // [level][id]
     hipStream_t m_stream = nullptr;
 
@@ -154,6 +155,7 @@ class Domain {
     std::vector<std::vector<long long>> m_param_offset;  // into m_params_dev
     std::vector<std::vector<char>> m_simple;             // [level][field type]
     std::vector<std::vector<char>> m_idem;               // [level][field type]
+    std::vector<std::vector<char>> m_shell;              // [level][field type]: the fill writes every ghost
     std::vector<std::vector<char>> m_face_values_zero;   // [level][field type]
     FaceParam *m_params_hom_dev = nullptr;
     std::vector<std::vector<long long>> m_view_bits, m_view_deps, m_view_ndeps, m_view_rest, m_view_nrest;

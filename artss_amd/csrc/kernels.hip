@@ -510,13 +510,14 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, int kz, double *__r
     }
     int bx, by, bz;
     main_block(main_z, bx, by, bz);
-    const int i = bx * BX + static_cast<int>(threadIdx.x);
-    const int j = by * BY + static_cast<int>(threadIdx.y);
-    if (i >= L.N[0] || j >= L.N[1]) return;
+    const int o = V.shell_filled ? 1 : 0;  // the inner box when the fill writes every ghost
+    const int i = o + bx * BX + static_cast<int>(threadIdx.x);
+    const int j = o + by * BY + static_cast<int>(threadIdx.y);
+    if (i >= L.N[0] - o || j >= L.N[1] - o) return;
 #pragma unroll
     for (int q = 0; q < kz; q++) {
-        const int k = bz * kz + q;
-        if (k >= L.N[2]) break;
+        const int k = o + bz * kz + q;
+        if (k >= L.N[2] - o) break;
         const long long idx = lin(L, i, j, k);
         if (dev::written(V, idx)) continue;
         const double v = src[idx];
@@ -553,11 +554,14 @@ __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int kz, T ax, T a
     int bx, by, bz;
     main_block(main_z, bx, by, bz);
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2], N0 = L.N[0], N1 = L.N[1];
-    const int o = FIRST ? 0 : 1;
+    // a first sweep covers the whole array unless the fill writes every
+    // ghost cell (then the inner box holds every cell it must write)
+    const bool full = FIRST && !V.shell_filled;
+    const int o = full ? 0 : 1;
     const int i = o + bx * BX + static_cast<int>(threadIdx.x);
     const int j = o + by * BY + static_cast<int>(threadIdx.y);
-    if (FIRST ? (i >= N0 || j >= N1) : (i > n0 || j > n1)) return;
-    const int kend = FIRST ? L.N[2] : n2 + 1;
+    if (full ? (i >= N0 || j >= N1) : (i > n0 || j > n1)) return;
+    const int kend = full ? L.N[2] : n2 + 1;
     // 32-bit offsets (a level has fewer than 2^31 cells)
     const unsigned nj = static_cast<unsigned>(N0), nk = nj * static_cast<unsigned>(N1);
 #pragma unroll
@@ -565,7 +569,7 @@ __global__ void k_jacobi_bc(LevelDesc L, BCView V, int main_z, int kz, T ax, T a
         const int k = o + bz * kz + q;
         if (k >= kend) break;
         const unsigned idx = static_cast<unsigned>(i) + nj * static_cast<unsigned>(j) + nk * static_cast<unsigned>(k);
-        const bool in_inner_box = FIRST ? (i >= 1 && j >= 1 && k >= 1 && i <= n0 && j <= n1 && k <= n2) : true;
+        const bool in_inner_box = full ? (i >= 1 && j >= 1 && k >= 1 && i <= n0 && j <= n1 && k <= n2) : true;
         // the update is computed for every cell of the inner box before the
         // obstacle test, so the stencil loads are in flight while it runs
         // (the expression of JacobiDiffuse.cpp:292-300, as dev::jacobi_t)
@@ -654,13 +658,14 @@ __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int ma
     }
     int bx, by, bz;
     main_block(main_z, bx, by, bz);
-    const int i = bx * BX + static_cast<int>(threadIdx.x);
-    const int j = by * BY + static_cast<int>(threadIdx.y);
-    if (i >= Lf.N[0] || j >= Lf.N[1]) return;
+    const int o = V.shell_filled ? 1 : 0;  // the inner box when the fill writes every ghost
+    const int i = o + bx * BX + static_cast<int>(threadIdx.x);
+    const int j = o + by * BY + static_cast<int>(threadIdx.y);
+    if (i >= Lf.N[0] - o || j >= Lf.N[1] - o) return;
 #pragma unroll
     for (int qz = 0; qz < kz; qz++) {
-        const int k = bz * kz + qz;
-        if (k >= Lf.N[2]) break;
+        const int k = o + bz * kz + qz;
+        if (k >= Lf.N[2] - o) break;
         // 32-bit offsets (a level has fewer than 2^31 cells); the coarse and
         // fine loads are issued before the obstacle and fill tests
         const unsigned idx = static_cast<unsigned>(i) +
@@ -909,7 +914,7 @@ void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, dou
 
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src) {
     const int kz = kz_of(L);
-    const dim3 g = zbatch(full_grid(L), kz);
+    const dim3 g = zbatch(V.shell_filled ? inner_grid(L) : full_grid(L), kz);
     k_copy_bc<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, dst, src);
 }
 template <typename T, typename TB>
@@ -917,7 +922,7 @@ void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T
                double ax, double ay, double az, double beta, double dsign, double w) {
     if (src == nullptr && !first) throw std::invalid_argument("jacobi_bc: a sweep from zero must be a first sweep");
     const int kz = kz_of(L);
-    const dim3 g = zbatch(first ? full_grid(L) : inner_grid(L), kz);
+    const dim3 g = zbatch(first && !V.shell_filled ? full_grid(L) : inner_grid(L), kz);
     const dim3 gi = with_items(g, dev::bc_items_grid(V));
     const T ax_ = static_cast<T>(ax), ay_ = static_cast<T>(ay), az_ = static_cast<T>(az), beta_ = static_cast<T>(beta),
             ds_ = static_cast<T>(dsign), w_ = static_cast<T>(w);
@@ -956,7 +961,7 @@ template <typename Tf, typename Tc>
 void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, Tf *dst,
                        const Tf *fine, const Tc *coarse) {
     const int kz = kz_of(Lf);
-    const dim3 g = zbatch(full_grid(Lf), kz);
+    const dim3 g = zbatch(V.shell_filled ? inner_grid(Lf) : full_grid(Lf), kz);
     k_prolongate_add_bc<Tf, Tc><<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z), kz,
                                                                                      dst, fine, coarse);
 }
