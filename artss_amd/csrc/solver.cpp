@@ -326,7 +326,7 @@ void Solver::do_step_ns(double t) {
         m_counters.diffusion[1] = m_dif->last_iterations();
         m_dif->diffuse(f.w, f.w0, f.w_tmp, nu, false);
         m_counters.diffusion[2] = m_dif->last_iterations();
-        f.couple_vector();
+        if (!m_add_source) f.couple_vector();  // else overwritten below before any read
     }
     if (m_add_source) {
         m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
@@ -353,7 +353,7 @@ void Solver::do_step_nstempturb(double t) {
     m_counters.diffusion[1] = m_dif->last_iterations();
     m_dif->diffuse(f.w, f.w0, f.w_tmp, s.nu, f.nu_t, false);
     m_counters.diffusion[2] = m_dif->last_iterations();
-    f.couple_vector();
+    if (!m_add_source) f.couple_vector();  // else overwritten below before any read
     if (m_add_source) {
         m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
         f.couple_vector();
@@ -368,11 +368,11 @@ void Solver::do_step_nstempturb(double t) {
         k::scale_to(d.stream(), f.kappa_t.data, f.nu_t.data, rPr_T, f.kappa_t.get_size());
         m_dif_temp->diffuse(f.T, f.T0, f.T_tmp, s.kappa, f.kappa_t, false);
         m_counters.diffusion[3] = m_dif_temp->last_iterations();
-        f.couple_scalar();
+        if (!m_add_temp_source) f.couple_scalar();  // else overwritten below before any read
     } else if (s.kappa != 0.) {
         m_dif_temp->diffuse(f.T, f.T0, f.T_tmp, s.kappa, false);
         m_counters.diffusion[3] = m_dif_temp->last_iterations();
-        f.couple_scalar();
+        if (!m_add_temp_source) f.couple_scalar();
     }
     if (m_add_temp_source) {
         m_src_temp->add_source(f.T, f.S_T, false);
@@ -397,7 +397,7 @@ void Solver::do_step_nstemp(double t) {
         m_counters.diffusion[1] = m_dif->last_iterations();
         m_dif->diffuse(f.w, f.w0, f.w_tmp, s.nu, false);
         m_counters.diffusion[2] = m_dif->last_iterations();
-        f.couple_vector();
+        if (!m_add_source) f.couple_vector();  // else overwritten below before any read
     }
     if (m_add_source) {
         m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
@@ -438,7 +438,7 @@ void Solver::do_step_nsturb(double t) {
     m_counters.diffusion[1] = m_dif->last_iterations();
     m_dif->diffuse(f.w, f.w0, f.w_tmp, s.nu, f.nu_t, false);
     m_counters.diffusion[2] = m_dif->last_iterations();
-    f.couple_vector();
+    if (!m_add_source) f.couple_vector();  // else overwritten below before any read
     if (m_add_source) {
         m_src_vel->add_source(f.u, f.v, f.w, f.f_x, f.f_y, f.f_z, false);
         f.couple_vector();

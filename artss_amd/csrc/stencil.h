@@ -175,6 +175,11 @@ template <typename T>
 __device__ inline void face_ghosts(const LevelDesc &L, const BCView &V, T *dst, long long idx, int i, int j, int k,
                                    T v) {
     if (V.n_ghosts == 0) return;
+    // one test for the common case of a cell away from every face (j and k
+    // are uniform over a wavefront of the sweep kernels, i is not)
+    if (i != V.bc.target[0] && i != V.bc.target[1] && j != V.bc.target[2] && j != V.bc.target[3] &&
+        k != V.bc.target[4] && k != V.bc.target[5])
+        return;
     const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
     if (i == V.bc.target[0]) dst[idx - i] = static_cast<T>(V.bc.sign[0]) * v + static_cast<T>(V.bc.value[0]);
     if (i == V.bc.target[1])
