@@ -632,14 +632,14 @@ long long Domain::ghost_cells(int level) const {
     return 2 * Ny * Nz + 2 * (Nx - 2) * Nz + 2 * (Nx - 2) * (Ny - 2);
 }
 
-void Domain::apply_boundary(double *f, int ft, int level) {
+void Domain::apply_boundary(double *f, int ft, int level, const int *gate) {
     require_device();
     if (ft < 0 || ft >= NUMBER_OF_FIELD_TYPES) return;  // UNKNOWN_FIELD has no boundary data
     for (const Wave &w : m_waves[level][ft])
         timed("bc_faces", level, [&] {
-            k::bc_faces(m_stream, m_faces_dev + w.offset, w.count, m_params_dev + m_param_offset[level][ft], f);
+            k::bc_faces(m_stream, m_faces_dev + w.offset, w.count, m_params_dev + m_param_offset[level][ft], f, gate);
         });
-    timed("bc_domain", level, [&] { k::bc_domain(m_stream, m_level[level], m_dom_bc[level][ft], f); });
+    timed("bc_domain", level, [&] { k::bc_domain(m_stream, m_level[level], m_dom_bc[level][ft], f, gate); });
 }
 
 void Domain::apply_boundary_merged(double *f, int ft, int level) {

@@ -51,7 +51,8 @@ class Domain {
 
     // DomainController::apply_boundary (DomainController.cpp:145): obstacle
     // faces first, then the six domain patches.
-    void apply_boundary(double *f, int field_type, int level);
+    // gate (optional): the fill's kernels do nothing while *gate != 0 (k::IterGate)
+    void apply_boundary(double *f, int field_type, int level, const int *gate = nullptr);
 
     // A fill is simple when no write of it reads a cell that any of its
     // writes changes: it then needs one wave of face writes, faces and

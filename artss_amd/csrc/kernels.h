@@ -12,7 +12,8 @@ namespace artss {
 namespace k {
 
 // ---- element-wise over the whole array (Field.h:64-135) ----
-void copy(hipStream_t s, double *dst, const double *src, long long n);
+// gate (optional): nothing runs while *gate != 0 (IterGate::done)
+void copy(hipStream_t s, double *dst, const double *src, long long n, const int *gate = nullptr);
 // x[cells[r]] = sum_j M[r * n + j] b[cells[j]] for r < n (n <= kMaxDenseRows:
 // the gathered b is staged in 16 KiB of LDS per workgroup)
 constexpr int kMaxDenseRows = 2048;
@@ -34,8 +35,8 @@ void scale_to(hipStream_t s, double *dst, const double *src, double v, long long
 void copy_domain(hipStream_t s, const LevelDesc &L, double *dst, const double *src);
 
 // ---- boundary fill (Multigrid.cpp:555-572) ----
-void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p, double *f);
-void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f);
+void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p, double *f, const int *gate = nullptr);
+void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f, const int *gate = nullptr);
 // faces (one wave) and ghosts in one launch; only for a fill none of whose
 // writes reads a cell another write changes
 void bc_merged(hipStream_t s, const LevelDesc &L, const DomainBC &bc, const FaceEntry *e, long long n_faces,
@@ -67,10 +68,11 @@ void jacobi_mg_first(hipStream_t s, const LevelDesc &L, double *dst, const doubl
 // ---- colored Gauss-Seidel (ColoredGaussSeidelDiffuse.cpp:190-341), one color per launch ----
 // ev == nullptr: constant ax,ay,az,beta; else per-cell coefficients from D, ev, dt
 void cgs_color(hipStream_t s, const LevelDesc &L, int color, double *out, const double *b, double ax, double ay,
-               double az, double beta, double dsign, double w, double D, const double *ev, double dt);
+               double az, double beta, double dsign, double w, double D, const double *ev, double dt,
+               const int *gate = nullptr);
 // (rb*(out-in))^2 partials over inner cells, ColoredGaussSeidelDiffuse.cpp:77-82, :147-157
 void update_norm_partials(hipStream_t s, const LevelDesc &L, const double *out, const double *in, double rb,
-                          double D, const double *ev, double dt, double *partial);
+                          double D, const double *ev, double dt, double *partial, const int *gate = nullptr);
 
 // ---- multigrid (VCycleMG.cpp) ----
 void residuum(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *b);   // :346-354
@@ -105,6 +107,21 @@ void fill_bc(hipStream_t s, const LevelDesc &L, const BCView &V, T *f);
 // ---- reductions ----
 int partial_count(const LevelDesc &L);  // number of partial slots a level-wide kernel writes
 void reduce_sum(hipStream_t s, const double *partial, int n, double *out);  // fixed-order sum
+// Loop state of an iteration whose test runs on the device, so that several
+// iterations can be enqueued without a readback each: kernels given &done do
+// nothing once it is set, and the state after the batch is that of a loop
+// that tested after every iteration.
+struct IterGate {
+    double sum;    // the last sum reduced (the residual is its square root)
+    double tol;
+    int count;     // iterations that ran since gate_open
+    int max_iter;
+    int done;      // 1: the loop would have stopped
+};
+void gate_open(hipStream_t s, IterGate *g, double tol, int max_iter);
+// reduce_sum into g->sum, then count the iteration and set done as the
+// loop `while (sqrt(sum) > tol && count < max_iter)` decides; nothing once done
+void reduce_gate(hipStream_t s, const double *partial, int n, IterGate *g);
 
 // ---- IPressure (IPressure.cpp) ----
 void divergence(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *y, const double *z);

@@ -94,7 +94,8 @@ __device__ inline void write_partial(double *partial, double v) {
 }
 
 // ------------------------------------------------------------------ element-wise
-__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long long n) {
+__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long long n, const int *gate) {
+    if (gate != nullptr && *gate) return;
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i];
 }
 // each source into two destinations, for up to four fields at once
@@ -138,7 +139,8 @@ __global__ void k_copy_domain(LevelDesc L, double *__restrict__ d, const double 
 // ------------------------------------------------------------------ boundary
 // ObstacleBoundary.cpp:42-48: f[idx] = sign * f[idx +- ref] + value
 __global__ void k_bc_faces(const FaceEntry *__restrict__ e, long long n, const FaceParam *__restrict__ p,
-                           double *__restrict__ f) {
+                           double *__restrict__ f, const int *gate) {
+    if (gate != nullptr && *gate) return;
     const long long t = blockIdx.x * 256LL + threadIdx.x;
     if (t >= n) return;
     dev::face_write(e[t], p, f);
@@ -148,7 +150,8 @@ __global__ void k_bc_faces(const FaceEntry *__restrict__ e, long long n, const F
 // owned by one thread; its final value is the composition of the affine maps
 // of the patches containing it, applied in patch order to the inner cell the
 // chain of reference offsets ends on.
-__global__ void k_bc_domain(LevelDesc L, DomainBC bc, double *__restrict__ f) {
+__global__ void k_bc_domain(LevelDesc L, DomainBC bc, double *__restrict__ f, const int *gate) {
+    if (gate != nullptr && *gate) return;
     dev::domain_ghost(L, bc, f, blockIdx.x * 256LL + threadIdx.x);
 }
 
@@ -355,7 +358,8 @@ __global__ void k_dissipate(LevelDesc L, double rdx, double rdy, double rdz, dou
 // Thread t of a row handles i = i_first + 2t.
 __global__ void k_cgs_color(LevelDesc L, int color, double ax, double ay, double az, double beta, double dsign,
                             double w, double D, const double *__restrict__ ev, double dt, double rdx, double rdy,
-                            double rdz, double *__restrict__ out, const double *__restrict__ b) {
+                            double rdz, double *__restrict__ out, const double *__restrict__ b, const int *gate) {
+    if (gate != nullptr && *gate) return;
     int bx, by, bz;
     main_block(static_cast<int>(gridDim.z), bx, by, bz);
     const int t = bx * BX + static_cast<int>(threadIdx.x);
@@ -384,7 +388,8 @@ __global__ void k_cgs_color(LevelDesc L, int color, double ax, double ay, double
 }
 
 __global__ void k_update_norm(LevelDesc L, DiffCoef c, const double *__restrict__ out, const double *__restrict__ in,
-                              const double *__restrict__ ev, double *__restrict__ partial) {
+                              const double *__restrict__ ev, double *__restrict__ partial, const int *gate) {
+    if (gate != nullptr && *gate) return;  // uniform over the grid
     int i, j, k;
     double sq = 0.0;
     if (inner_cell(L, i, j, k)) {
@@ -715,6 +720,36 @@ __global__ void k_fill_bc(LevelDesc L, BCView V, T *__restrict__ f) {
     dev::bc_item(L, V, blockIdx.x * 256LL + threadIdx.x, f, [&](long long c) { return f[c]; });
 }
 
+// the fixed-order sum of k_reduce_sum, then the loop test of the diffusion
+// iteration (ColoredGaussSeidelDiffuse.cpp:60-98): run again while
+// sqrt(sum) > tol and fewer than max_iter iterations ran
+__global__ void k_reduce_gate(const double *__restrict__ partial, int n, IterGate *__restrict__ g) {
+    if (g->done) return;
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += partial[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __shared__ double wsum[16];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < 16; w++) r += wsum[w];
+        const int c = g->count + 1;
+        g->sum = r;
+        g->count = c;
+        g->done = (sqrt(r) > g->tol && c < g->max_iter) ? 0 : 1;
+    }
+}
+__global__ void k_gate_open_iter(IterGate *__restrict__ g, double tol, int max_iter) {
+    if (threadIdx.x == 0) {
+        g->sum = 1.0;
+        g->tol = tol;
+        g->count = 0;
+        g->max_iter = max_iter;
+        g->done = (1.0 > tol && 0 < max_iter) ? 0 : 1;
+    }
+}
+
 __global__ void k_reduce_sum(const double *__restrict__ partial, int n, double *__restrict__ out) {
     double v = 0.0;
     for (int t = threadIdx.x; t < n; t += 1024) v += partial[t];
@@ -809,8 +844,8 @@ __global__ void k_buoyancy(LevelDesc L, double beta, double g, double *__restric
 }  // namespace
 
 // ================================================================== launchers
-void copy(hipStream_t s, double *dst, const double *src, long long n) {
-    k_copy<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n);
+void copy(hipStream_t s, double *dst, const double *src, long long n, const int *gate) {
+    k_copy<<<grid1d(n, 256), 256, 0, s>>>(dst, src, n, gate);
 }
 void dense_apply(hipStream_t s, int n, const double *M, const uint32_t *cells, const double *b, double *x) {
     if (n < 1 || n > kMaxDenseRows) throw std::invalid_argument("dense_apply: 1 to kMaxDenseRows rows");
@@ -833,14 +868,14 @@ void copy_domain(hipStream_t s, const LevelDesc &L, double *dst, const double *s
     k_copy_domain<<<full_grid(L), block(), 0, s>>>(L, dst, src);
 }
 
-void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p, double *f) {
+void bc_faces(hipStream_t s, const FaceEntry *e, long long n, const FaceParam *p, double *f, const int *gate) {
     if (n <= 0) return;
-    k_bc_faces<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(e, n, p, f);
+    k_bc_faces<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(e, n, p, f, gate);
 }
-void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f) {
+void bc_domain(hipStream_t s, const LevelDesc &L, const DomainBC &bc, double *f, const int *gate) {
     if (!bc.active) return;
     const long long n = dev::ghost_count(L);
-    k_bc_domain<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, bc, f);
+    k_bc_domain<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(L, bc, f, gate);
 }
 
 void bc_merged(hipStream_t s, const LevelDesc &L, const DomainBC &bc, const FaceEntry *e, long long n_faces,
@@ -872,16 +907,17 @@ void jacobi_mg_first(hipStream_t s, const LevelDesc &L, double *dst, const doubl
 }
 
 void cgs_color(hipStream_t s, const LevelDesc &L, int color, double *out, const double *b, double ax, double ay,
-               double az, double beta, double dsign, double w, double D, const double *ev, double dt) {
+               double az, double beta, double dsign, double w, double D, const double *ev, double dt,
+               const int *gate) {
     const int half = (L.n[0] + 1) / 2;
     dim3 g((half + BX - 1) / BX, (L.n[1] + BY - 1) / BY, L.n[2]);
     k_cgs_color<<<g, block(), 0, s>>>(L, color, ax, ay, az, beta, dsign, w, D, ev, dt, 1. / L.h[0], 1. / L.h[1],
-                                      1. / L.h[2], out, b);
+                                      1. / L.h[2], out, b, gate);
 }
 void update_norm_partials(hipStream_t s, const LevelDesc &L, const double *out, const double *in, double rb,
-                          double D, const double *ev, double dt, double *partial) {
+                          double D, const double *ev, double dt, double *partial, const int *gate) {
     DiffCoef c{0, 0, 0, rb, D, dt, 1. / L.h[0], 1. / L.h[1], 1. / L.h[2]};
-    k_update_norm<<<inner_grid(L), block(), 0, s>>>(L, c, out, in, ev, partial);
+    k_update_norm<<<inner_grid(L), block(), 0, s>>>(L, c, out, in, ev, partial, gate);
 }
 
 void residuum(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *b) {
@@ -985,6 +1021,10 @@ int partial_count(const LevelDesc &L) {
     const dim3 g = inner_grid(L);
     return static_cast<int>(g.x * g.y * g.z);
 }
+void reduce_gate(hipStream_t s, const double *partial, int n, IterGate *g) {
+    k_reduce_gate<<<1, 1024, 0, s>>>(partial, n, g);
+}
+void gate_open(hipStream_t s, IterGate *g, double tol, int max_iter) { k_gate_open_iter<<<1, 64, 0, s>>>(g, tol, max_iter); }
 void reduce_sum(hipStream_t s, const double *partial, int n, double *out) {
     k_reduce_sum<<<1, 1024, 0, s>>>(partial, n, out);
 }

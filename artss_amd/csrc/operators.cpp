@@ -121,22 +121,47 @@ void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b,
     const double *ev = EV ? EV->data : nullptr;
     const int n_part = k::partial_count(L);
     out.copy_data(in);
-    int it = 0;
-    double res = 1.;
-    while (res > m_settings.tol_res && it < m_settings.max_iter) {
-        in.copy_data(out);
-        for (int color = 0; color < 2; color++)
-            dm.timed("cgs_diffuse", 0, [&] {
-                k::cgs_color(dm.stream(), L, color, out.data, b.data, ax, ay, az, beta, 1.0, m_settings.w, D, ev,
-                             dt);
-            });
-        dm.apply_boundary(out.data, out.get_type(), 0);
-        k::update_norm_partials(dm.stream(), L, out.data, in.data, rb, D, ev, dt, dm.partial_buffer());
-        res = std::sqrt(dm.reduce(dm.partial_buffer(), n_part));
-        it++;
+    // The loop test runs on the device (k::IterGate): as many iterations as
+    // the previous call took are enqueued at once, then one at a time; the
+    // kernels of iterations past the test's stop do nothing, so the fields
+    // are those of the reference's loop, which tests after every iteration.
+    hipStream_t s = dm.stream();
+    if (!m_gate) {
+        ARTSS_HIP(hipMalloc(&m_gate, sizeof(k::IterGate)));
+        ARTSS_HIP(hipHostMalloc(&m_gate_host, sizeof(k::IterGate)));
     }
-    m_iterations = it;
-    m_residual = res;
+    const int *stop = &m_gate->done;
+    k::gate_open(s, m_gate, m_settings.tol_res, m_settings.max_iter);
+    int launched = 0;
+    while (true) {
+        const int batch = launched < m_batch_hint ? m_batch_hint - launched : 1;
+        for (int q = 0; q < batch; q++) {
+            k::copy(s, in.data, out.data, in.get_size(), stop);
+            for (int color = 0; color < 2; color++)
+                dm.timed("cgs_diffuse", 0, [&] {
+                    k::cgs_color(s, L, color, out.data, b.data, ax, ay, az, beta, 1.0, m_settings.w, D, ev, dt,
+                                 stop);
+                });
+            dm.apply_boundary(out.data, out.get_type(), 0, stop);
+            k::update_norm_partials(s, L, out.data, in.data, rb, D, ev, dt, dm.partial_buffer(), stop);
+            k::reduce_gate(s, dm.partial_buffer(), n_part, m_gate);
+        }
+        launched += batch;
+        ARTSS_HIP(hipMemcpyAsync(m_gate_host, m_gate, sizeof(k::IterGate), hipMemcpyDeviceToHost, s));
+        ARTSS_HIP(hipStreamSynchronize(s));
+        if (m_gate_host->done) break;
+    }
+    m_iterations = m_gate_host->count;
+    m_residual = m_iterations > 0 ? std::sqrt(m_gate_host->sum) : 1.;
+    m_batch_hint = m_iterations;
+}
+
+ColoredGaussSeidelDiffuse::~ColoredGaussSeidelDiffuse() {
+    if (m_gate) {
+        (void)hipStreamSynchronize(m_domain.stream());
+        (void)hipFree(m_gate);
+        (void)hipHostFree(m_gate_host);
+    }
 }
 
 void ColoredGaussSeidelDiffuse::diffuse(Field &out, const Field &in, const Field &b, double D, bool sync) {

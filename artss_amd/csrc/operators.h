@@ -114,6 +114,9 @@ class ExplicitDiffuse : public IDiffusion {
 class ColoredGaussSeidelDiffuse : public IDiffusion {
   public:
     ColoredGaussSeidelDiffuse(Domain &d, const DiffusionSettings &s) : m_domain(d), m_settings(s) {}
+    ~ColoredGaussSeidelDiffuse() override;
+    ColoredGaussSeidelDiffuse(const ColoredGaussSeidelDiffuse &) = delete;
+    ColoredGaussSeidelDiffuse &operator=(const ColoredGaussSeidelDiffuse &) = delete;
     void diffuse(Field &out, const Field &in, const Field &b, double D, bool sync) override;
     void diffuse(Field &out, const Field &in, const Field &b, double D, const Field &EV, bool sync) override;
 
@@ -121,6 +124,9 @@ class ColoredGaussSeidelDiffuse : public IDiffusion {
     void run(Field &out, const Field &in, const Field &b, double D, const Field *EV);
     Domain &m_domain;
     DiffusionSettings m_settings;
+    k::IterGate *m_gate = nullptr;       // device loop state
+    k::IterGate *m_gate_host = nullptr;  // pinned
+    int m_batch_hint = 0;                // iterations of the previous call, enqueued at once
 };
 
 // turbulence/ConstSmagorinsky.cpp:30-97
