@@ -12,9 +12,11 @@ namespace artss {
 // level; one workgroup, enqueued on `s`.
 // max_iter > 0 replaces the coarse solve's iteration cap of the plan
 // max_iter, n_relax, w > 0 replace the plan's coarse-solve cap, smoothing
-// count and Jacobi weight for this launch
+// count and Jacobi weight for this launch; skip_dead leaves out the
+// reference's work on the coarsest level, whose error stays zero (the
+// result differs at most in the sign of zeros)
 void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter = 0,
-             int n_relax = 0, double w = 0.0);
+             int n_relax = 0, double w = 0.0, bool skip_dead = false);
 // dynamic LDS (bytes) a workgroup of the kernel can use for level arrays
 int mg_tail_lds_limit();
 // allow launches with this much dynamic LDS (once, before the first launch)

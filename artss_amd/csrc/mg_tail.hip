@@ -313,7 +313,7 @@ __device__ void t_solve(const TailLevel &T, const TailPlan &P, double *out, doub
 }
 
 __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__restrict__ plan, const double *b_first,
-                                                           int max_iter, int n_relax, double w) {
+                                                           int max_iter, int n_relax, double w, int skip_dead) {
     __shared__ TailPlan P;
     __shared__ double red[kTailThreads / 64 + 1];
     extern __shared__ double lds[];
@@ -356,6 +356,9 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__rest
         const TailLevel &Tc = P.lv[level + 1 - f];
         const double *rhs = level == f ? b_first : T.res1;
         t_smooth(T, P, T.err1, T.tmp, rhs);
+        // the coarsest level only ever holds a zero error (VCycleMG.cpp:228-282
+        // restrict to it, then prolongate its zero back): skipped on request
+        if (skip_dead && level + 1 == Lm) break;
         t_residuum(T, T.res0, T.err1, rhs);
         t_bc(T, T.res0);
         t_restrict(Tc, T, Tc.res1, T.res0);
@@ -367,9 +370,11 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__rest
         const TailLevel &T = P.lv[level - 1 - f];
         const TailLevel &Tc = P.lv[level - f];
         const double *rhs = level - 1 == f ? b_first : T.res1;
-        t_prolongate(Tc, T, T.err0, Tc.err1);
-        t_bc(T, T.err0);
-        t_add(T.err1, T.err0, static_cast<int>(T.L.size));
+        if (!(skip_dead && level == Lm)) {
+            t_prolongate(Tc, T, T.err0, Tc.err1);
+            t_bc(T, T.err0);
+            t_add(T.err1, T.err0, static_cast<int>(T.L.size));
+        }
         if (level == Lm) t_solve(T, P, T.err1, T.tmp, rhs, red);
         else t_smooth(T, P, T.err1, T.tmp, rhs);
     }
@@ -400,8 +405,8 @@ void mg_tail_reserve_lds(int lds_bytes) {
 }
 
 void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter, int n_relax,
-             double w) {
-    k_mg_tail<<<1, kTailThreads, lds_bytes, s>>>(plan, b_first, max_iter, n_relax, w);
+             double w, bool skip_dead) {
+    k_mg_tail<<<1, kTailThreads, lds_bytes, s>>>(plan, b_first, max_iter, n_relax, w, skip_dead ? 1 : 0);
 }
 
 }  // namespace artss

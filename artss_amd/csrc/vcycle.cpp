@@ -311,7 +311,7 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
     } else {
         d.timed("mg_tail", f, [&] {
             mg_tail(s, m_plan_dev, m_res1[f]->data, m_tail_lds_bytes, coarse_max_iter, m_mixed_tail,
-                    m_mixed_tail > 0 ? m_mixed_weight : 0.0);
+                    m_mixed_tail > 0 ? m_mixed_weight : 0.0, true);
         });
     }
     for (int level = f; level > 0; --level) {
