@@ -87,7 +87,7 @@ def parse():
                     help="mixed: Jacobi weight of those sweeps (0: the XML's)")
     ap.add_argument("--coarse", choices=("direct", "tail"), default="direct",
                     help="mixed: exact dense solve of the coarse levels, or their single-workgroup V-cycle")
-    ap.add_argument("--tail-sweeps", type=int, default=0,
+    ap.add_argument("--tail-sweeps", type=int, default=3,
                     help="mixed: n_relax of the coarse levels' single-workgroup kernel (0: the XML's)")
     ap.add_argument("--tail-cells", type=int, default=4096,
                     help="levels of at most this many cells run in one single-workgroup kernel")
