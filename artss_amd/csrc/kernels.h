@@ -92,6 +92,11 @@ void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, dou
 // Iterates of type T (double: the reference's arithmetic; float: the mixed
 // V-cycle), right-hand sides / coarse errors of type TB / Tc.
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src);
+// restrict_fill<float, float, float> into rhs followed by the coarse level's
+// first sweep from zero into err with its fill (jacobi_bc with src == nullptr),
+// in one launch; the values are those of the two launches
+void restrict_sweep0(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, float *rhs, float *err,
+                     const float *fine, double ax, double ay, double az, double beta, double dsign, double w);
 // src == nullptr (first sweeps only): the sweep starts from zero everywhere
 template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,

@@ -714,6 +714,40 @@ __global__ void k_dense_apply(int n, const double *__restrict__ M, const uint32_
     if (lane == 0) x[cells[row]] = acc;
 }
 
+// Restriction of the fine residuum to a coarse level fused with that level's
+// first smoothing sweep from zero (k_restrict_fill, then k_jacobi_bc with
+// ZERO): rhs = restrict(fine) on the coarse fluid cells, err = w beta dsign
+// rhs there, err's fill, zero on every other cell.  Fill items restrict the
+// cell they read themselves, so no cell waits for another workgroup.
+__global__ void k_restrict_sweep0(LevelDesc Lc, LevelDesc Lf, BCView V, int main_z, float w, float beta, float dsign,
+                                  float *__restrict__ rhs, float *__restrict__ err, const float *__restrict__ fine) {
+    if (static_cast<int>(blockIdx.z) >= main_z) {
+        auto val = [&](long long c) -> float {
+            int ci, cj, ck;
+            if (!is_inner(Lc, c, ci, cj, ck)) return 0.0f;
+            return w * (beta * (dsign * dev::restrict_t<float>(Lf, fine, ci, cj, ck)));
+        };
+        dev::bc_item<float, decltype(val), true>(Lc, V, item_index(main_z), err, val);
+        return;
+    }
+    int bx, by, bz;
+    main_block(main_z, bx, by, bz);
+    const int i = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = by * BY + static_cast<int>(threadIdx.y);
+    const int k = bz;
+    if (i >= Lc.N[0] || j >= Lc.N[1]) return;
+    const long long idx = lin(Lc, i, j, k);
+    if (i >= 1 && j >= 1 && k >= 1 && i <= Lc.n[0] && j <= Lc.n[1] && k <= Lc.n[2] && !in_obstacle(Lc, i, j, k)) {
+        const float r = dev::restrict_t<float>(Lf, fine, i, j, k);
+        rhs[idx] = r;
+        const float v = w * (beta * (dsign * r));
+        err[idx] = v;
+        dev::face_ghosts(Lc, V, err, idx, i, j, k, v);
+    } else if (!dev::written(V, idx)) {
+        err[idx] = 0.0f;
+    }
+}
+
 // One idempotent fill in a single launch (faces, ghosts, dependent ghosts)
 template <typename T>
 __global__ void k_fill_bc(LevelDesc L, BCView V, T *__restrict__ f) {
@@ -944,6 +978,16 @@ template void restrict_fill<float, float, float>(hipStream_t, const LevelDesc &,
                                                  const float *, float *);
 template void restrict_fill<float, double, double>(hipStream_t, const LevelDesc &, const LevelDesc &, double *,
                                                    const float *, double *);
+void restrict_sweep0(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, float *rhs, float *err,
+                     const float *fine, double ax, double ay, double az, double beta, double dsign, double w) {
+    (void)ax;
+    (void)ay;
+    (void)az;
+    const dim3 g = full_grid(Lc);
+    k_restrict_sweep0<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(
+        Lc, Lf, V, static_cast<int>(g.z), static_cast<float>(w), static_cast<float>(beta), static_cast<float>(dsign), rhs,
+        err, fine);
+}
 void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse) {
     k_prolongate_add<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, fine, coarse);
 }

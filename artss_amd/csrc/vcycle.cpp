@@ -78,14 +78,24 @@ void VCycleMG::alloc_mixed() {
 // effective count).  An even count starts with one sweep into a third
 // buffer, so the result always lands in err1; the first sweep into a buffer
 // copies the cells it does not update, so err1 holds tmp's values there.
-void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int sweeps, bool from_zero) {
+int VCycleMG::mixed_sweep_count(int sweeps) const {
+    const int nx = m_s.n_relax;
+    return sweeps > 0 ? sweeps : ((nx % 2 == 0 && nx > 0) ? nx - 1 : nx);
+}
+
+float *VCycleMG::first_sweep_target(int level, int sweeps) const {
+    const int n = mixed_sweep_count(sweeps);
+    return (n % 2 == 0 && n > 0) ? m_scrf[level] : m_err1f[level];
+}
+
+void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int sweeps, bool from_zero,
+                             bool first_done) {
     Domain &d = m_domain;
     double ax, ay, az, beta;
     coeffs(level, &ax, &ay, &az, &beta);
     const LevelDesc &Ld = d.level(level);
     const BCView V = d.bc_view(level, P, true);
-    const int nx = m_s.n_relax;
-    const int n = sweeps > 0 ? sweeps : ((nx % 2 == 0 && nx > 0) ? nx - 1 : nx);
+    const int n = mixed_sweep_count(sweeps);
     const double w = m_mixed_weight > 0.0 ? m_mixed_weight : m_s.smoother.w;
     auto sweep = [&](bool first, float *dst, const float *src) {
         d.timed("jacobi_mg32", level, [&] {
@@ -94,17 +104,20 @@ void VCycleMG::smooth_fast_f(int level, const double *b64, const float *b32, int
         });
     };
     // from_zero: tmp is zero and the first sweep does not read it; after
-    // the first sweep tmp is free and the sweeps alternate err1 and tmp
+    // the first sweep tmp is free and the sweeps alternate err1 and tmp.
+    // first_done: the first sweep's result is already in
+    // first_sweep_target(level) (restrict_sweep0)
     float *src = from_zero ? nullptr : m_tmpf[level];
     int m = n;
     if (n % 2 == 0 && n > 0) {
-        sweep(true, m_scrf[level], src);
+        if (!first_done) sweep(true, m_scrf[level], src);
         src = m_scrf[level];
         m = n - 1;
+        first_done = false;
     }
     float *dst = m_err1f[level];
     for (int i = 0; i < m; i++) {
-        sweep(i == 0, dst, src);
+        if (!(i == 0 && first_done)) sweep(i == 0, dst, src);
         src = dst;
         dst = dst == m_err1f[level] ? m_tmpf[level] : m_err1f[level];
     }
@@ -283,18 +296,28 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
     Domain &d = m_domain;
     const int f = m_tail_first;
     hipStream_t s = d.stream();
-    // every level's pre-smoothing starts from zero
+    // every level's pre-smoothing starts from zero; below level 0 its first
+    // sweep is fused into the restriction that produces the level's rhs
+    const bool fused_first = mixed_sweep_count(m_mixed_pre) > 0;
     for (int level = 0; level < f; ++level) {
         const BCView V = d.bc_view(level, P, true);
         const double *b64 = level == 0 ? r : nullptr;
         const float *b32 = level == 0 ? nullptr : m_res1f[level];
-        smooth_fast_f(level, b64, b32, m_mixed_pre, true);
+        smooth_fast_f(level, b64, b32, m_mixed_pre, true, level > 0 && fused_first);
         d.timed("residuum32", level, [&] {
             if (b64) k::residuum_faces<float, double>(s, d.level(level), V, m_res0f[level], m_err1f[level], b64);
             else k::residuum_faces<float, float>(s, d.level(level), V, m_res0f[level], m_err1f[level], b32);
         });
         d.timed("restrict32", level, [&] {
-            if (level + 1 < f)
+            if (level + 1 < f && fused_first) {
+                // the next level's first sweep from zero rides along
+                double cx, cy, cz, cb;
+                coeffs(level + 1, &cx, &cy, &cz, &cb);
+                const double w = m_mixed_weight > 0.0 ? m_mixed_weight : m_s.smoother.w;
+                k::restrict_sweep0(s, d.level(level + 1), d.level(level), d.bc_view(level + 1, P, true),
+                                   m_res1f[level + 1], first_sweep_target(level + 1, m_mixed_pre), m_res0f[level], cx,
+                                   cy, cz, cb, kDsign, w);
+            } else if (level + 1 < f)
                 k::restrict_fill<float, float, float>(s, d.level(level + 1), d.level(level), m_res1f[level + 1],
                                                       m_res0f[level], m_tmpf[level + 1]);
             else
@@ -325,7 +348,7 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
                 k::prolongate_add_bc<float, float>(s, d.level(level), d.level(fine), V, m_tmpf[fine], m_err1f[fine],
                                                    m_err1f[level]);
         });
-        smooth_fast_f(fine, fine == 0 ? r : nullptr, fine == 0 ? nullptr : m_res1f[fine], m_mixed_post, false);
+        smooth_fast_f(fine, fine == 0 ? r : nullptr, fine == 0 ? nullptr : m_res1f[fine], m_mixed_post, false, false);
     }
 }
 
