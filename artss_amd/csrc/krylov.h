@@ -41,7 +41,7 @@ class GcrPressure {
 
   private:
     void iteration(Field &p);          // enqueue one GCR iteration
-    void launch_iteration(Field &p, const Field &b);
+    void launch_iteration(Field &p, const Field &b, int count = 1);
     double true_residual(Field &p, const Field &b);
 
     Domain &m_domain;
@@ -58,7 +58,7 @@ class GcrPressure {
     kk::GcrGate *m_gate_host = nullptr;  // pinned
     int m_predicted = 0;               // iterations enqueued at once in the next solve
     double m_initial = -1.0;
-    hipGraphExec_t m_graph = nullptr;
+    hipGraphExec_t m_graph = nullptr, m_graph2 = nullptr;  // one and two iterations
     double *m_graph_p = nullptr;
     const double *m_graph_b = nullptr;
 };
