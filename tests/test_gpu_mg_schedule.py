@@ -48,6 +48,10 @@ CASES = {
                                                        smoother="ColoredGaussSeidel", obstacles=OBST3[:1],
                                                        hrr=3.0, open_top=True), n_relax=2),
     "tg_periodic": lambda: cases.ns_taylor_green(n=32, levels=3, steps=2),
+    # level 0 above 2^20 cells: the kernels that give each thread four cells
+    # along z (and the prolongation that shares its coarse planes among them)
+    "plume_obst3_large": lambda: cases.plume(n=(128, 64, 128), X=(0.0, 2.0), levels=4, steps=2, dt=0.01,
+                                             obstacles=OBST3, hrr=3.0, source_x=1.0, open_top=True),
     "tg_periodic_relax1": lambda: _with(cases.ns_taylor_green(n=32, levels=2, steps=2), n_relax=1),
 }
 
