@@ -524,11 +524,12 @@ __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, int kz, double *__r
         const int k = o + bz * kz + q;
         if (k >= L.N[2] - o) break;
         const long long idx = lin(L, i, j, k);
-        if (dev::written(V, idx)) continue;
+        const bool fluid =
+            i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k);
+        if (!fluid && dev::written(V, idx)) continue;  // fluid cells are never written by the fill
         const double v = src[idx];
         dst[idx] = v;
-        if (i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k))
-            dev::face_ghosts(L, V, dst, idx, i, j, k, v);
+        if (fluid) dev::face_ghosts(L, V, dst, idx, i, j, k, v);
     }
 }
 
@@ -688,11 +689,14 @@ __global__ void k_prolongate_add_bc(LevelDesc Lc, LevelDesc Lf, BCView V, int ma
             const Tf pc = static_cast<Tf>(dev::prolong_t<Tc>(Lc, coarse, cidx, q));
             if (!in_obstacle(Lc, ci, cj, ck)) p = pc;
         }
-        if (dev::written(V, idx)) continue;
+        // fluid cells are never written by the fill: the write-bit lookup
+        // is only needed for the others
+        const bool fluid =
+            i >= 1 && j >= 1 && k >= 1 && i <= Lf.n[0] && j <= Lf.n[1] && k <= Lf.n[2] && !in_obstacle(Lf, i, j, k);
+        if (!fluid && dev::written(V, idx)) continue;
         const Tf v = f + p;  // corrected_at's value
         dst[idx] = v;
-        if (i >= 1 && j >= 1 && k >= 1 && i <= Lf.n[0] && j <= Lf.n[1] && k <= Lf.n[2] && !in_obstacle(Lf, i, j, k))
-            dev::face_ghosts(Lf, V, dst, static_cast<long long>(idx), i, j, k, v);
+        if (fluid) dev::face_ghosts(Lf, V, dst, static_cast<long long>(idx), i, j, k, v);
     }
 }
 
