@@ -103,3 +103,26 @@ def test_direct_coarse_solve_meets_target(obstacles, open_top):
         assert sum(cyc_d) < sum(cyc_t)
         for f in f_t:
             assert errs[f] < 1e-5, f
+
+
+def test_gcr_batch_respects_iteration_cap():
+    """After solves of many iterations the next solve enqueues a batch; a cap
+    below the batch stops it exactly at the cap, with the true residual of
+    that iterate reported (above the target)."""
+    xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=4, steps=5, dt=0.01, obstacles=OBST, hrr=3.0,
+                      source_x=1.0, open_top=True)
+    s = artss_amd.Solver(xml)
+    s.set_mg_schedule(2, 4096)
+    s.set_pressure_mode(True, 1e-9, 400)
+    s.set_pressure_method(2)
+    for step in range(1, 4):
+        s.step(step * 0.01)
+    assert s.counters()["vcycles"] > 5
+    s.set_pressure_mode(True, 1e-12, 3)
+    s.step(0.04)
+    c = s.counters()
+    assert c["vcycles"] == 3
+    assert c["residual"] > 1e-12
+    s.set_pressure_mode(True, 1e-9, 400)
+    s.step(0.05)
+    assert 0 <= s.counters()["residual"] <= 1e-9

@@ -268,3 +268,30 @@ def test_taylor_green_100_steps_vs_oracle():
                 worst = max(worst, e)
                 assert e <= 1e-10, (f, step, e)
     assert worst <= 1e-8
+
+
+@pytest.mark.parametrize("max_iter", [1, 2])
+def test_cgs_diffusion_iteration_cap_vs_oracle(max_iter):
+    """Colored Gauss-Seidel diffusion stopped by max_iter (tolerance out of
+    reach) inside the NSSolver step: from the second step on the product
+    enqueues the previous step's iteration count at once behind its device
+    loop test; the fields stay those of the oracle's loop, which tests after
+    every iteration."""
+    xml_ref = cases.ns_taylor_green(n=32, levels=3, steps=6, reference=True, diffusion="ColoredGaussSeidel")
+    xml_ref = xml_ref.replace("<max_iter> 100 </max_iter><tol_res> 1e-07 </tol_res>",
+                              "<max_iter> %d </max_iter><tol_res> 1e-30 </tol_res>" % max_iter, 1)
+    assert "<tol_res> 1e-30 </tol_res>" in xml_ref
+    cfg = artss_xml.parse(xml_ref)
+    odom = O.Domain(cfg)
+    u, v, w = cases.taylor_green_ic(odom)
+    osim = O.Sim(odom, cfg, u, v, w, None)
+    s = artss_amd.Solver(xml_ref.replace('usr_fct="Zero"', 'usr_fct="TaylorGreen"'))
+    for step in range(1, 7):
+        t = step * cfg["dt"]
+        s.step(t)
+        osim.step(t)
+        counts = s.counters()["diffusion"][:3]
+        assert counts == osim.counters()[:3], step
+        assert counts[:2] == [max_iter] * 2, step  # w stays zero: its first update is zero and ends the loop
+        for f in "uvwp":
+            assert _rel(s.field(f), osim.field(f)) <= 1e-12, (f, step)
