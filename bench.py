@@ -94,7 +94,7 @@ def parse():
     ap.add_argument("--warm", choices=("none", "extrapolate"), default="extrapolate",
                     help="start of each step's pressure iteration: the previous p (the reference's start) or "
                          "p extrapolated linearly in time")
-    ap.add_argument("--ensemble-members", type=int, default=16,
+    ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the ensemble measurement (uneven plume boxes; 0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=5, help="timed steps of the ensemble measurement")
     ap.add_argument("--ensemble-threads", type=int, default=8, help="host threads issuing the members' steps")
