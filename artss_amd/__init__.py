@@ -89,6 +89,7 @@ def lib():
         "artss_solver_set_mixed_sweeps": (C.c_int, [vp, C.c_int, C.c_int, C.c_double]),
         "artss_solver_set_coarse_solve": (C.c_int, [vp, C.c_int]),
         "artss_solver_set_mixed_tail": (C.c_int, [vp, C.c_int]),
+        "artss_tuned_cells_per_thread": (C.c_int, [C.c_longlong]),
         "artss_solver_snapshot": (C.c_int, [vp, C.c_double, ip]),
         "artss_solver_restore": (C.c_int, [vp, C.c_int, dp]),
         "artss_solver_drop_snapshot": (C.c_int, [vp, C.c_int]),
@@ -121,6 +122,11 @@ def lib():
         fn.argtypes = args
     _lib = L
     return L
+
+
+def tuned_cells_per_thread(level_size):
+    """cells per thread along z the autotuner picked for levels of this array size (0: not tuned)"""
+    return int(lib().artss_tuned_cells_per_thread(int(level_size)))
 
 
 def _check(rc):

@@ -666,6 +666,8 @@ int artss_solver_initial_residual(artss_solver *s, double *residual) {
     });
 }
 
+int artss_tuned_cells_per_thread(long long level_size) { return artss::k::tuned_kz(level_size); }
+
 int artss_solver_set_mixed_tail(artss_solver *s, int n_relax) {
     return guard([&] {
         require(s && n_relax >= 0, "bad argument");

@@ -97,6 +97,10 @@ void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, co
 // in one launch; the values are those of the two launches
 void restrict_sweep0(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, float *rhs, float *err,
                      const float *fine, double ax, double ay, double az, double beta, double dsign, double w);
+// autotuner of the fused level kernels' cells per thread along z (1, 2 or 4)
+// for a level size, timed on scratch arrays; call outside stream capture
+void tune_level(hipStream_t s, const LevelDesc &L, const BCView &V);
+int tuned_kz(long long level_size);  // 0: not tuned (the size rule applies)
 // src == nullptr (first sweeps only): the sweep starts from zero everywhere
 template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,

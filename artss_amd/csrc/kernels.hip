@@ -12,9 +12,12 @@
 // a single workgroup adds in a fixed order, so results are deterministic run
 // to run (their rounding differs from the serial left-to-right sum).
 #include "kernels.h"
+#include "domain.h"
 #include "stencil.h"
 
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 
 namespace artss {
@@ -39,7 +42,18 @@ inline dim3 block() { return dim3(BX, BY, 1); }
 // as it is); the count is a kernel argument.
 constexpr int KZ = 4;
 constexpr long long kBatchCells = 1LL << 20;  // levels with at least this many cells batch
-inline int kz_of(const LevelDesc &L) { return L.size >= kBatchCells ? KZ : 1; }
+// cells per thread along z chosen by tune_level for a level size (guarded by
+// kz_lock), else the size rule above
+std::mutex kz_lock;
+std::map<long long, int> kz_tuned;
+inline int kz_of(const LevelDesc &L) {
+    {
+        std::lock_guard<std::mutex> g(kz_lock);
+        const auto it = kz_tuned.find(L.size);
+        if (it != kz_tuned.end()) return it->second;
+    }
+    return L.size >= kBatchCells ? KZ : 1;
+}
 inline dim3 zbatch(dim3 g, int kz) { return dim3(g.x, g.y, (g.z + kz - 1) / kz); }
 inline unsigned grid1d(long long n, int threads) {
     long long g = (n + threads - 1) / threads;
@@ -994,6 +1008,58 @@ void restrict_sweep0(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, co
 }
 void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, double *fine, const double *coarse) {
     k_prolongate_add<<<inner_grid(Lc), block(), 0, s>>>(Lc, Lf, fine, coarse);
+}
+
+// Times the level's FP32 first sweep with 1, 2 and 4 cells per thread along
+// z on scratch arrays of the level's size and keeps the fastest for every
+// fused level kernel of that size (the values do not depend on the choice).
+// Not during stream capture: it synchronizes on events.
+void tune_level(hipStream_t s, const LevelDesc &L, const BCView &V) {
+    {
+        std::lock_guard<std::mutex> g(kz_lock);
+        if (kz_tuned.count(L.size)) return;
+    }
+    const size_t n = static_cast<size_t>(L.size);
+    float *buf = nullptr;
+    ARTSS_HIP(hipMalloc(&buf, sizeof(float) * 3 * n));
+    ARTSS_HIP(hipMemsetAsync(buf, 0, sizeof(float) * 3 * n, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ARTSS_HIP(hipEventCreate(&e0));
+    ARTSS_HIP(hipEventCreate(&e1));
+    const int cand[3] = {1, 2, KZ};
+    int best = L.size >= kBatchCells ? KZ : 1;
+    float best_ms = 0.0f;
+    const dim3 g0 = V.shell_filled ? inner_grid(L) : full_grid(L);
+    for (int c : cand) {
+        const dim3 g = zbatch(g0, c);
+        const dim3 gi = with_items(g, dev::bc_items_grid(V));
+        auto launch = [&] {
+            k_jacobi_bc<float, float, true, false><<<gi, block(), 0, s>>>(L, V, static_cast<int>(g.z), c, 1.f, 1.f, 1.f,
+                                                                         0.16f, -1.f, 0.8f, buf, buf + n, buf + 2 * n);
+        };
+        launch();  // warm-up
+        ARTSS_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 5; r++) launch();
+        ARTSS_HIP(hipEventRecord(e1, s));
+        ARTSS_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        ARTSS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (c == cand[0] || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ARTSS_HIP(hipStreamSynchronize(s));
+    ARTSS_HIP(hipFree(buf));
+    std::lock_guard<std::mutex> g(kz_lock);
+    kz_tuned.emplace(L.size, best);
+}
+int tuned_kz(long long level_size) {
+    std::lock_guard<std::mutex> g(kz_lock);
+    const auto it = kz_tuned.find(level_size);
+    return it == kz_tuned.end() ? 0 : it->second;
 }
 
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src) {

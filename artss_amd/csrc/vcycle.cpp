@@ -57,6 +57,8 @@ void VCycleMG::free_mixed() {
 void VCycleMG::alloc_mixed() {
     if (!m_err1f.empty()) return;
     const int f = m_tail_first;
+    // pick the cells per thread of the FP32 levels' kernels by measurement
+    for (int l = 0; l < f; l++) k::tune_level(m_domain.stream(), m_domain.level(l), m_domain.bc_view(l, P, true));
     auto alloc = [&](int level) {
         float *p = nullptr;
         const size_t bytes = sizeof(float) * static_cast<size_t>(m_domain.level(level).size);

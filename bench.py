@@ -252,6 +252,8 @@ def main():
     dom.profile_arm(args.profile_kernel)
     t_secs, _, _, _ = run.timed(timing_steps)
     launches, kernel_ms = dom.profile_read()
+    # cells per thread along z the autotuner picked for the fused kernels of levels 0..3
+    tuned = [artss_amd.tuned_cells_per_thread(dom.info(l)["size"]) for l in range(4)]
     kname, klevel = args.profile_kernel.split("@")
     kinfo = dom.info(int(klevel))
     per_launch_bytes = BYTES_PER_CELL.get(kname, 0) * (kinfo["size"] if kname in WHOLE_ARRAY else kinfo["inner_cells"])
@@ -304,6 +306,7 @@ def main():
                                   "p + (p - p_prev), linear extrapolation in time"),
                    "coarse_iterations": args.coarse_iterations, "sweeps": args.sweeps, "weight": args.weight,
                    "coarse": args.coarse, "tail_sweeps": args.tail_sweeps,
+                   "tuned_cells_per_thread": tuned,
                    "pressure": ("reference V-cycle schedule" if args.reference_schedule else
                                 "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance),
                    "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(

@@ -183,6 +183,12 @@ int artss_solver_set_coarse_solve(artss_solver *s, int direct);
  * n_relax and weight. */
 int artss_solver_set_mixed_tail(artss_solver *s, int n_relax);
 
+/* No reference counterpart: the cells per thread along z (1, 2 or 4) that the
+ * autotuner picked for the fused level kernels of levels with this many
+ * cells (array size with ghosts), or 0 when no level of that size has been
+ * tuned yet (the mixed pressure path tunes its FP32 levels on first use). */
+int artss_tuned_cells_per_thread(long long level_size);
+
 /* Start of the residual-target GCR pressure iteration (methods 1 and 2; no
  * reference counterpart: the reference's iteration starts from the previous
  * step's p, mode 0).  With p_n the previous solution, p_prev the one before
