@@ -224,8 +224,8 @@ class Run:
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: every box here has one MI355X and this build has no "
-              "multi-GPU path (the domain-decomposition design is in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the step would shard "
+              "as z-slabs with halo exchanges; not implemented)", file=sys.stderr)
         sys.exit(2)
     import artss_amd
 

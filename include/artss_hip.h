@@ -7,7 +7,7 @@
  * IPressure, ITurbulence and ISource from the solver classes.  Each entry
  * point below is what a drop-in for one of those calls binds; the comment
  * names the reference interface it replaces (file:line in the reference's
- * src/).  INTEGRATION.md shows the reference-side adapter classes.
+ * src/).
  *
  * Conventions
  *  - Fields are float64 device arrays (HBM) in the reference layout: x

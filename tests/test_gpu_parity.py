@@ -10,7 +10,7 @@ stated otherwise:
    ones included;
 3. whole solver steps against the golden runs and, over 100 steps, against
    the oracle (tolerance 1e-10 relative per checked step, 1e-8 at the end,
-   the bounds the north star sets for FP64 fields; residual norms are summed
+   the bounds required of FP64 fields; residual norms are summed
    in a different order than the serial build, so an iteration count can in
    principle flip at a tolerance threshold -- the tests report counts).
 """
