@@ -22,7 +22,7 @@ struct InitialFields {
 
 // the initial condition the XML names (usr_fct and its parameters); throws
 // config_error for a function the reference cannot run either (it has no
-// parsing implementation for it) or one this build does not carry
+// parsing implementation for it) or one this library does not carry
 InitialFields initial_fields(const Domain &d);
 
 }  // namespace artss

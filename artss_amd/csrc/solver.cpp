@@ -86,7 +86,7 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     for (int k = 0; k < 9; k++)
         if (s.solver == kinds[k]) m_kind = k;
     if (m_kind < 0)
-        throw config_error("solver '" + s.solver + "' is not part of this build (NSSolver, NSTurbSolver, NSTempSolver, "
+        throw config_error("solver '" + s.solver + "' is not supported by this library (NSSolver, NSTurbSolver, NSTempSolver, "
                            "NSTempTurbSolver, PressureSolver, DiffusionSolver, DiffusionTurbSolver, "
                            "AdvectionDiffusionSolver, AdvectionSolver)");
     if (m_kind == 8 && s.ic != "GaussBubble")
@@ -99,7 +99,7 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     if (s.has_temperature && s.dissipation && m_kind != 3)
         throw config_error("dissipation is only part of NSTempSolver (NSTempSolver.cpp:181-185)");
     if (s.has_turbulence && s.turbulence_type != "ConstSmagorinsky")
-        throw config_error("turbulence model '" + s.turbulence_type + "' is not part of this build");
+        throw config_error("turbulence model '" + s.turbulence_type + "' is not supported by this library");
     if (s.ic_random) throw config_error("random initial conditions are not supported");
 
     m_fields = std::make_unique<FieldController>(d);

@@ -224,7 +224,7 @@ class Run:
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (the step would shard "
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (the step would shard "
               "as z-slabs with halo exchanges; not implemented)", file=sys.stderr)
         sys.exit(2)
     import artss_amd

@@ -212,7 +212,7 @@ int artss_solver_initial_residual(artss_solver *s, double *residual);
  * run bit for bit.  Snapshots stay in HBM (slot ids, freed by
  * artss_solver_drop_snapshot or with the solver); save/load write one file
  * of this library's own format (magic, version, cell counts, time, dt, field
- * names, raw FP64 arrays; HDF5 is not part of this build) and load checks
+ * names, raw FP64 arrays; HDF5 is not supported) and load checks
  * that the grid matches. */
 int artss_solver_snapshot(artss_solver *s, double t, int *slot);
 int artss_solver_restore(artss_solver *s, int slot, double *t);

@@ -1,7 +1,7 @@
 /* CPU oracle for the ARTSS solver hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a serial C restatement of the reference
- * algorithm, used as the checker for the HIP implementation in artss_amd/.
+ * algorithm, used to check the HIP implementation in artss_amd/.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product never links, loads or falls back to it.
  *

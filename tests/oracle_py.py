@@ -1,6 +1,6 @@
 """ctypes bindings for the CPU oracle (oracle/liboracle.so).
 
-Test infrastructure: the oracle is the checker, never the thing measured or
+Test infrastructure: the oracle checks results, it is never the thing measured or
 shipped.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
 leg import this module.
 """
