@@ -126,3 +126,20 @@ def test_gcr_batch_respects_iteration_cap():
     s.set_pressure_mode(True, 1e-9, 400)
     s.step(0.05)
     assert 0 <= s.counters()["residual"] <= 1e-9
+
+
+def test_autotuner_picks_a_batch_per_fp32_level():
+    """After the first mixed solve every FP32 level of the preconditioner has
+    a tuned cells-per-thread count (1, 2 or 4); the choice changes no value,
+    which the graph-versus-eager and schedule tests check bit for bit."""
+    xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=4, steps=2, dt=0.01, obstacles=OBST, hrr=3.0,
+                      source_x=1.0, open_top=True)
+    s = artss_amd.Solver(xml)
+    s.set_mg_schedule(2, 4096)
+    s.set_pressure_mode(True, 1e-9, 400)
+    s.set_pressure_method(2)
+    s.step(0.01)
+    dom = s.domain
+    picks = [artss_amd.tuned_cells_per_thread(dom.info(level)["size"]) for level in range(2)]
+    assert all(p in (1, 2, 4) for p in picks), picks
+    assert artss_amd.tuned_cells_per_thread(12345) == 0
