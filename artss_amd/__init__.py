@@ -27,6 +27,13 @@ class ArtssError(RuntimeError):
     pass
 
 
+class NotConverged(ArtssError):
+    """ARTSS_ERR_NOT_CONVERGED: the step completed (fields advanced, counters
+    valid) but its residual-target pressure solve stopped above the target."""
+
+ERR_NOT_CONVERGED = 5
+
+
 class LevelInfo(C.Structure):
     _fields_ = [("N", C.c_int * 3), ("n", C.c_int * 3), ("size", C.c_longlong), ("h", C.c_double * 3),
                 ("inner_cells", C.c_longlong), ("n_obstacles", C.c_int)]
@@ -130,6 +137,8 @@ def tuned_cells_per_thread(level_size):
 
 
 def _check(rc):
+    if rc == ERR_NOT_CONVERGED:
+        raise NotConverged("artss error %d: %s" % (rc, lib().artss_last_error().decode()))
     if rc != 0:
         raise ArtssError("artss error %d: %s" % (rc, lib().artss_last_error().decode()))
 

@@ -2,6 +2,7 @@
 #include "../../include/artss_hip.h"
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -76,6 +77,16 @@ void check_type(int ft) { require(ft >= 0 && ft < NUMBER_OF_FIELD_TYPES, "field 
 void finish(Domain &D) {
     ARTSS_HIP(hipGetLastError());
     ARTSS_HIP(hipStreamSynchronize(D.stream()));
+}
+
+// status of a completed step: a residual-target pressure solve that missed
+// its target is reported, with the residual, after the step is done
+int step_status(const Solver &s, long step) {
+    const StepCounters &c = s.counters();
+    if (c.converged) return ARTSS_OK;
+    g_error = "step " + std::to_string(step) + ": pressure residual " + std::to_string(c.residual) +
+              " above the target after " + std::to_string(c.vcycles) + " iterations";
+    return ARTSS_ERR_NOT_CONVERGED;
 }
 
 std::vector<uint64_t> patch_list(const Domain &D, int level, int patch) {
@@ -486,7 +497,7 @@ int artss_solver_step(artss_solver *s, double t) {
         require(s, "null argument");
         s->solver->step(t);
         finish(s->solver->domain());
-        return ARTSS_OK;
+        return step_status(*s->solver, std::lround(t / s->solver->domain().settings().dt));
     });
 }
 
@@ -497,10 +508,18 @@ int artss_solver_run(artss_solver *s, long first_step, long count, double *secon
         const double dt = D.settings().dt;
         finish(D);
         const auto t0 = std::chrono::steady_clock::now();
-        for (long n = first_step; n < first_step + count; n++) s->solver->step(static_cast<double>(n) * dt);
+        long missed = 0;
+        for (long n = first_step; n < first_step + count; n++) {
+            s->solver->step(static_cast<double>(n) * dt);
+            if (!s->solver->counters().converged && missed == 0) missed = n;
+        }
         finish(D);
         const auto t1 = std::chrono::steady_clock::now();
         if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+        if (missed) {
+            g_error = "step " + std::to_string(missed) + " and possibly later ones: pressure residual above the target";
+            return ARTSS_ERR_NOT_CONVERGED;
+        }
         return ARTSS_OK;
     });
 }
@@ -632,6 +651,12 @@ int artss_ensemble_step(artss_ensemble *e, long step) {
     return guard([&] {
         require(e, "null argument");
         e->ensemble->step(step);
+        for (size_t m = 0; m < e->ensemble->size(); m++)
+            if (!e->ensemble->member(m).counters().converged) {
+                g_error = "member " + std::to_string(m) + ", step " + std::to_string(step) +
+                          ": pressure residual above the target";
+                return ARTSS_ERR_NOT_CONVERGED;
+            }
         return ARTSS_OK;
     });
 }

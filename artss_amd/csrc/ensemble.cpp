@@ -39,6 +39,8 @@ void Ensemble::run_members() {
         try {
             Solver &s = *m_members[i];
             s.step(static_cast<double>(n) * s.domain().settings().dt);
+            // a member's step is complete on the device when step() returns
+            ARTSS_HIP(hipStreamSynchronize(s.domain().stream()));
         } catch (...) {
             std::lock_guard<std::mutex> g(m_lock);
             if (!m_error) m_error = std::current_exception();

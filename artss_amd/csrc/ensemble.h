@@ -34,6 +34,7 @@ class Ensemble {
     // with its own time step dt_m.  Members run concurrently on `threads`
     // host threads (threads = 1: one after another on the calling thread).
     // The first exception a member throws is rethrown after all are done.
+    // Every member's stream is synchronised before step() returns.
     void step(long n);
 
   private:

@@ -140,20 +140,24 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
     double rr = restart();
     m_initial = std::sqrt(rr);
     int it = 0, it_base = 0;
-    bool confirmed_once = false;
+    // right after a restart the recursive residual is the true one, up to
+    // the summation order; at least one iteration follows, so every restart
+    // costs an iteration and the loop ends at max_iter
+    bool restarted = false;
     while (true) {
-        if (rr <= tol2) {
+        if (rr <= tol2 && !restarted) {
             const double t = true_residual(p, b);
-            if (t <= tol || confirmed_once) {
+            if (t <= tol) {
                 *residual = t;
+                m_converged = true;
                 m_predicted = std::max(0, it - 1);
                 return it;
             }
             // the recursive residual drifted below the true one: restart
-            // from the true residual (once per solve is plenty)
-            confirmed_once = true;
+            // from the true residual and keep iterating
             rr = restart();
             it_base = it;
+            restarted = true;
             continue;
         }
         if (it >= max_iter) break;
@@ -163,9 +167,11 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
         ARTSS_HIP(hipStreamSynchronize(s));
         rr = m_gate_host->rr;
         it = it_base + m_gate_host->count;
+        restarted = false;
     }
     m_predicted = std::max(0, it - 1);
     *residual = true_residual(p, b);
+    m_converged = *residual <= tol;
     return it;
 }
 

@@ -30,14 +30,18 @@ class GcrPressure {
     GcrPressure(const GcrPressure &) = delete;
     GcrPressure &operator=(const GcrPressure &) = delete;
 
-    // iterate until ||b - lap p||_2 <= tol (or max_iter); returns the
-    // iterations (V-cycles) taken, *residual the final true residual.
+    // iterate until the true residual ||b - lap p||_2 <= tol, or until
+    // max_iter iterations; returns the iterations (V-cycles) taken and in
+    // *residual the final true residual.  converged() tells the two apart:
+    // a solve is converged only when its true residual meets tol.
     // Up to one fewer iterations than the previous solve took are enqueued
     // at once behind the convergence gate (kk::GcrGate), the rest one at a
     // time: the result is the same as with a check after every iteration.
     int solve(Field &p, const Field &b, double tol, int max_iter, double *residual);
     // ||b - lap p||_2 at the start of the last solve
     double initial_residual() const { return m_initial; }
+    // the last solve ended with its true residual <= tol
+    bool converged() const { return m_converged; }
 
   private:
     void iteration(Field &p);          // enqueue one GCR iteration
@@ -58,6 +62,7 @@ class GcrPressure {
     kk::GcrGate *m_gate_host = nullptr;  // pinned
     int m_predicted = 0;               // iterations enqueued at once in the next solve
     double m_initial = -1.0;
+    bool m_converged = false;
     hipGraphExec_t m_graph = nullptr, m_graph2 = nullptr;  // one and two iterations
     double *m_graph_p = nullptr;
     const double *m_graph_b = nullptr;

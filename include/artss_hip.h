@@ -35,6 +35,11 @@ extern "C" {
 #define ARTSS_ERR_CONFIG 2   /* XML input rejected (Settings::config_error in the reference) */
 #define ARTSS_ERR_HIP 3      /* HIP runtime failure */
 #define ARTSS_ERR_INTERNAL 4
+/* the step completed (fields advanced, counters valid) but its residual-target
+ * pressure solve stopped at max_cycles with ||rhs - lap p||_2 above the
+ * target, or the residual is not finite; no reference counterpart (the
+ * reference's VCycleMG::pressure stops silently, VCycleMG.cpp:150-197) */
+#define ARTSS_ERR_NOT_CONVERGED 5
 
 typedef struct artss_domain artss_domain;
 typedef struct artss_mg artss_mg;
@@ -132,9 +137,11 @@ int artss_mg_tail_info(artss_mg *mg, int *first, int *lds_bytes);
 int artss_solver_create(const char *xml_text, int device, artss_solver **out);
 int artss_solver_destroy(artss_solver *s);
 int artss_solver_domain(artss_solver *s, artss_domain **d); /* borrowed, owned by the solver */
-/* one TimeIntegration iteration: do_step(t), update_sources(t), update_data (TimeIntegration.cpp:93,141-142) */
+/* one TimeIntegration iteration: do_step(t), update_sources(t), update_data (TimeIntegration.cpp:93,141-142);
+ * ARTSS_ERR_NOT_CONVERGED when a residual-target pressure solve missed its target (the step is done) */
 int artss_solver_step(artss_solver *s, double t);
-/* TimeIntegration::run loop for steps first..first+count-1 (t = step*dt); seconds = wall time */
+/* TimeIntegration::run loop for steps first..first+count-1 (t = step*dt); seconds = wall time;
+ * ARTSS_ERR_NOT_CONVERGED after all steps when any of them missed its pressure target */
 int artss_solver_run(artss_solver *s, long first_step, long count, double *seconds);
 /* device pointer of a named field: u v w p rhs T nu_t kappa_t f_x f_y f_z S_T T_ambient u0 ... */
 int artss_solver_field(artss_solver *s, const char *name, double **dev, size_t *n);

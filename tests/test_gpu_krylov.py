@@ -69,8 +69,16 @@ def _run_coarse(xml, direct, tol, steps):
     s.set_coarse_solve(direct)
     counts = []
     for step in range(1, steps + 1):
-        s.step(step * 0.01)
-        counts.append(s.counters())
+        try:
+            s.step(step * 0.01)
+            converged = True
+        except artss_amd.NotConverged:
+            converged = False
+        c = s.counters()
+        c["converged"] = converged
+        # a step reports success exactly when its true residual meets the target
+        assert converged == (0 <= c["residual"] <= tol), c
+        counts.append(c)
     return {f: s.field(f) for f in ("u", "v", "w", "p", "T")}, counts
 
 
@@ -119,13 +127,38 @@ def test_gcr_batch_respects_iteration_cap():
         s.step(step * 0.01)
     assert s.counters()["vcycles"] > 5
     s.set_pressure_mode(True, 1e-12, 3)
-    s.step(0.04)
+    with pytest.raises(artss_amd.NotConverged):
+        s.step(0.04)
     c = s.counters()
     assert c["vcycles"] == 3
     assert c["residual"] > 1e-12
     s.set_pressure_mode(True, 1e-9, 400)
     s.step(0.05)
     assert 0 <= s.counters()["residual"] <= 1e-9
+
+
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["vcycles", "gcr_fp64", "gcr_mixed"])
+def test_success_only_at_the_target(method):
+    """Every step that returns success has a true final residual at or below
+    the target; a step whose solve stops at the iteration cap above the
+    target completes and reports ARTSS_ERR_NOT_CONVERGED (NotConverged)."""
+    xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=4, steps=6, dt=0.01, obstacles=OBST, hrr=3.0,
+                      source_x=1.0, open_top=True)
+    s = artss_amd.Solver(xml)
+    s.set_pressure_method(method)
+    for step, (tol, cap) in enumerate([(1e-9, 400), (1e-9, 400), (1e-13, 2), (1e-9, 400), (1e-10, 400)], 1):
+        s.set_pressure_mode(True, tol, cap)
+        try:
+            s.step(step * 0.01)
+            ok = True
+        except artss_amd.NotConverged:
+            ok = False
+        c = s.counters()
+        assert ok == (0 <= c["residual"] <= tol), (step, c)
+        if cap == 2:
+            assert not ok and c["vcycles"] == 2
+        else:
+            assert ok, (step, c)
 
 
 def test_autotuner_picks_a_batch_per_fp32_level():
