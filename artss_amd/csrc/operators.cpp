@@ -221,6 +221,36 @@ void GaussFunction::update_source(Field &out, double t_cur) {
     k::scale_to(m_domain.stream(), out.data, m_spatial.data, tv, out.get_size());
 }
 
+BuoyancyMMS::BuoyancyMMS(Domain &d) : m_domain(d), m_source(d, RHO) {
+    const Settings &s = d.settings();
+    const LevelDesc &L = d.level(0);
+    std::vector<double> host(static_cast<size_t>(L.size), 0.0);
+    const double X1 = s.X[0], Y1 = s.Y[0];
+    const double dx = L.h[0], dy = L.h[1];
+    const double rbeta = 1. / s.beta;
+    const double rg = 1. / s.g;
+    const double c_nu = 2 * s.nu * M_PI * M_PI - 1;
+    const double c_kappa = 2 * s.kappa * M_PI * M_PI - 1;
+    const size_t Nx = static_cast<size_t>(L.N[0]), Ny = static_cast<size_t>(L.N[1]);
+    for (uint64_t idx : d.domain_list(0)) {
+        const size_t k = idx / (Nx * Ny);
+        const size_t j = (idx - k * Nx * Ny) / Nx;
+        const size_t i = idx - k * Nx * Ny - j * Nx;
+        // xi(i, X1, dx) = X1 + (i - 0.5) * dx (utility/GlobalMacrosTypes.h:76)
+        const double x = X1 + (static_cast<double>(i) - 0.5) * dx;
+        const double y = Y1 + (static_cast<double>(j) - 0.5) * dy;
+        host[idx] = s.rhoa * rbeta * rg * 2 * c_nu * c_kappa * std::sin(M_PI * (x + y));
+    }
+    ARTSS_HIP(hipMemcpyAsync(m_source.data, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice,
+                             d.stream()));
+    ARTSS_HIP(hipStreamSynchronize(d.stream()));
+}
+
+// BuoyancyMMS.cpp:58-61: out = source * exp(-t) over the whole array
+void BuoyancyMMS::update_source(Field &out, double t_cur) {
+    k::scale_to(m_domain.stream(), out.data, m_source.data, std::exp(-t_cur), out.get_size());
+}
+
 std::unique_ptr<IAdvection> make_advection(Domain &d, const std::string &type) {
     if (type == "SemiLagrangian") return std::make_unique<SLAdvect>(d);
     throw config_error("advection method '" + type + "' is not implemented");

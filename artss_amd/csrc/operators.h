@@ -169,6 +169,20 @@ class GaussFunction {
     Field m_spatial;
 };
 
+// source/BuoyancyMMS.cpp:11-69 (temp_fct="Buoyancy", SolverSelection.cpp:148-149):
+// the manufactured energy source rhoa/(beta g) 2 c_nu c_kappa sin(pi (x + y))
+// on the level-0 domain list, built once on the host with libm in the
+// reference's expression order; update_source scales it by exp(-t).
+class BuoyancyMMS {
+  public:
+    explicit BuoyancyMMS(Domain &d);
+    void update_source(Field &out, double t_cur);
+
+  private:
+    Domain &m_domain;
+    Field m_source;
+};
+
 // SolverSelection (solver/SolverSelection.cpp:33-213): same type strings
 std::unique_ptr<IAdvection> make_advection(Domain &d, const std::string &type);
 std::unique_ptr<IDiffusion> make_diffusion(Domain &d, const DiffusionSettings &s);

@@ -1,5 +1,8 @@
 #include "settings.h"
 
+#include <algorithm>
+#include <cctype>
+
 #include <cerrno>
 #include <cstdlib>
 #include <memory>
@@ -89,9 +92,21 @@ const std::vector<std::string> kPatchNames = {"left", "right", "bottom", "top", 
 const std::vector<std::string> kBCNames = {"neumann", "dirichlet", "periodic"};
 const std::vector<std::string> kAxisNames = {"x", "y", "z"};
 
+// Settings.cpp:716-718 with Mapping::match_axis (Mapping.cpp:17-23): each
+// comma-separated entry is matched case-insensitively; an entry that names
+// no axis ("xyz" in several of the reference's inputs) becomes UNKNOWN_AXIS,
+// which no caller ever looks for (SolverController.cpp:239-281,
+// ExplicitEulerSource.cpp:40-70 test for X, Y and Z only), so it is dropped.
+// Only membership matters to the callers, so each axis is kept once.
 std::vector<int> axes(const std::string &s) {
     std::vector<int> out;
-    for (const auto &a : split(s, ',')) out.push_back(match(kAxisNames, a, "axis"));
+    for (const auto &a : split(s, ',')) {
+        std::string up = a;
+        for (char &c : up) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+        for (size_t i = 0; i < kAxisNames.size(); i++)
+            if (kAxisNames[i] == up && std::find(out.begin(), out.end(), static_cast<int>(i)) == out.end())
+                out.push_back(static_cast<int>(i));
+    }
     return out;
 }
 
@@ -146,6 +161,7 @@ Settings parse_settings(const std::string &xml_text) {
         s.nu = req_real(pv, "nu", "physical_parameters");
         s.has_nu = true;
     }
+    if (ns) s.rhoa = opt_real(pv, "rhoa", 1);
     s.beta = opt_real(pv, "beta", 3.34e-3);
     s.g = opt_real(pv, "g", -9.81);
     if (d == "NSTempSolver" || d == "NSTempTurbSolver" || d == "NSTempTurbConSolver" || d == "NSTempConSolver") {
@@ -200,7 +216,11 @@ Settings parse_settings(const std::string &xml_text) {
             s.use_init_values = req_bool(srv, "use_init_values", "solver/source");
             if (!s.use_init_values)
                 s.ambient_temperature = req_real(srv, "ambient_temperature_value", "solver/source");
-        } else if (s.force_fct != "Zero" && s.force_fct != "Uniform") {
+        } else if (s.force_fct == "Uniform") {  // Settings.cpp:687-696
+            s.uniform_force[0] = req_real(srv, "val_x", "solver/source/Uniform");
+            s.uniform_force[1] = req_real(srv, "val_y", "solver/source/Uniform");
+            s.uniform_force[2] = req_real(srv, "val_z", "solver/source/Uniform");
+        } else if (s.force_fct != "Zero") {
             throw config_error("source type '" + s.source_type + "' has no parsing implementation.");
         }
         s.force_dir = axes(req_str(srv, "dir", "solver/source"));
@@ -246,7 +266,7 @@ Settings parse_settings(const std::string &xml_text) {
                 s.gauss.sigma[a] = req_real(tsv, std::string("sigma_") + ax[a], c);
             }
             s.gauss.tau = req_real(tsv, "tau", c);
-        } else if (s.temp_fct != "Zero") {
+        } else if (s.temp_fct != "Zero" && s.temp_fct != "Buoyancy") {  // Buoyancy: BuoyancyMMS, no parameters
             throw config_error("temperature source function '" + s.temp_fct + "' has no implementation here.");
         }
     }

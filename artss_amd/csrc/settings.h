@@ -54,7 +54,7 @@ struct GaussSettings {
 
 struct Settings {
     std::string solver;  // solver description, e.g. "NSTempTurbSolver"
-    double t_end = 0, dt = 0, nu = 0, kappa = 0, beta = 3.34e-3, g = -9.81;
+    double t_end = 0, dt = 0, nu = 0, kappa = 0, beta = 3.34e-3, g = -9.81, rhoa = 1;
     bool has_nu = false, has_kappa = false;
     double X[2] = {0, 1}, Y[2] = {0, 1}, Z[2] = {0, 1};
     int n[3] = {1, 1, 1};
@@ -71,6 +71,7 @@ struct Settings {
     std::vector<int> force_dir;
     bool use_init_values = false;
     double ambient_temperature = 0;
+    double uniform_force[3] = {0, 0, 0};  // force_fct Uniform: val_x, val_y, val_z
     bool has_pressure = false;
     MultigridSettings pressure;
 

@@ -92,9 +92,6 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     if (m_kind == 8 && s.ic != "GaussBubble")
         throw config_error("AdvectionSolver takes its velocity from the GaussBubble initial condition "
                            "(SolverController.cpp:54-56)");
-    if (s.has_source && s.force_fct == "Buoyancy" && s.use_init_values)
-        throw config_error("buoyancy with use_init_values is not supported");
-    if (s.has_source && s.force_fct == "Uniform") throw config_error("uniform momentum source is not supported");
     if (s.has_temperature && s.temp_random) throw config_error("random temperature source is not supported");
     if (s.has_temperature && s.dissipation && m_kind != 3)
         throw config_error("dissipation is only part of NSTempSolver (NSTempSolver.cpp:181-185)");
@@ -131,6 +128,7 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
         m_dif_temp = make_diffusion(d, s.diffusion);  // both solvers use the velocity settings (NSTempSolver.cpp:33)
         m_src_temp = std::make_unique<ExplicitEulerSource>(d, s.temp_dir);
         if (s.temp_fct == "Gauss") m_gauss = std::make_unique<GaussFunction>(d, s.gauss);
+        if (s.temp_fct == "Buoyancy") m_mms = std::make_unique<BuoyancyMMS>(d);  // SolverSelection.cpp:148-149
         m_add_temp_source = s.temp_fct != "Zero";
     }
     set_up_fields();
@@ -239,8 +237,26 @@ void Solver::set_up_fields() {
     }
     ARTSS_HIP(hipStreamSynchronize(d.stream()));
     for (const auto &kv : ic.constants) f.by_name(kv.first)->set_value(kv.second);
-    // SolverController::force_source (SolverController.cpp:197-202, 229-236) for the temperature solvers
-    if ((m_kind == 1 || m_kind == 3) && s.force_fct == "Buoyancy") f.T_ambient.set_value(s.ambient_temperature);
+    // SolverController::force_source (SolverController.cpp:197-202, 229-257), for the temperature
+    // solvers only.  The reference constructs T_ambient and the force fields without a value
+    // (FieldController.cpp:41,50-52); they hold zero here, so with use_init_values="Yes" T_ambient
+    // stays zero, and the Uniform force of the other solvers adds zero, as the reference's fresh
+    // allocations read in practice.
+    if (m_kind == 1 || m_kind == 3) {
+        if (s.force_fct == "Buoyancy" && !s.use_init_values) f.T_ambient.set_value(s.ambient_temperature);
+        if (s.force_fct == "Uniform") {
+            // Functions::uniform (Functions.cpp:820-832): the value on the inner cells
+            Field *out[3] = {&f.f_x, &f.f_y, &f.f_z};
+            const std::vector<uint64_t> inner = d.inner_list(0);
+            std::vector<double> host(n);
+            for (int a : s.force_dir) {
+                std::fill(host.begin(), host.end(), 0.0);
+                for (uint64_t idx : inner) host[idx] = s.uniform_force[a];
+                upload(d, out[a]->data, host.data(), n);
+                ARTSS_HIP(hipStreamSynchronize(d.stream()));
+            }
+        }
+    }
     // FieldController::set_up_boundary, FieldController.cpp:101-112
     d.apply_boundary(f.u.data, U, 0);
     d.apply_boundary(f.v.data, V, 0);
@@ -263,10 +279,12 @@ void Solver::update_sources(double t) {
         for (int a : s.force_dir) ISource::buoyancy_force(*out[a], f.T, f.T_ambient, s.beta, s.g, false);
     }
     if (m_gauss) m_gauss->update_source(f.S_T, t);
+    if (m_mms) m_mms->update_source(f.S_T, t);
 }
 
 void Solver::pressure_step(double t) {
     FieldController &f = *m_fields;
+    m_counters.converged = true;
     if (!m_pmode.to_tolerance) {
         m_pres->pressure(f.p, f.rhs, t, false);
         m_counters.vcycles = m_pres->last_cycles();
@@ -298,6 +316,7 @@ void Solver::pressure_step(double t) {
         m_counters.vcycles = m_gcr->solve(f.p, f.rhs, m_pmode.tolerance, m_pmode.max_cycles, &r);
         m_counters.residual = r;
         m_counters.initial_residual = m_gcr->initial_residual();
+        m_counters.converged = m_gcr->converged();
         if (warm) warm_end();
         return;
     }
@@ -309,6 +328,7 @@ void Solver::pressure_step(double t) {
     }
     m_counters.vcycles = cycles;
     m_counters.residual = r;
+    m_counters.converged = r <= m_pmode.tolerance;
 }
 
 // NSSolver.cpp:50-124
@@ -582,9 +602,11 @@ void Solver::save(const std::string &path, double t) {
         out.write(nm.data(), len);
     }
     std::vector<double> host(n);
+    // the copies run on the solver's stream, behind any step still in flight
     for (size_t i = 0; i < names.size(); i++) {
         const double *src = i < fields.size() ? fields[i].second->data : m_p_prev;
-        ARTSS_HIP(hipMemcpy(host.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost));
+        ARTSS_HIP(hipMemcpyAsync(host.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, d.stream()));
+        ARTSS_HIP(hipStreamSynchronize(d.stream()));
         out.write(reinterpret_cast<const char *>(host.data()), static_cast<std::streamsize>(sizeof(double) * n));
     }
     if (!out) throw config_error("writing checkpoint '" + path + "' failed");
@@ -630,7 +652,10 @@ double Solver::load(const std::string &path) {
             if (f == nullptr) throw config_error("checkpoint '" + path + "' holds an unknown field '" + nm + "'");
             dst = f->data;
         }
-        ARTSS_HIP(hipMemcpy(dst, host.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        // ordered after the kernels still in flight on the solver's stream;
+        // the synchronisation keeps `host` alive until the copy is done
+        ARTSS_HIP(hipMemcpyAsync(dst, host.data(), sizeof(double) * n, hipMemcpyHostToDevice, d.stream()));
+        ARTSS_HIP(hipStreamSynchronize(d.stream()));
     }
     m_warm_history = have_prev ? static_cast<int>(header[5]) : 0;
     m_warm_sample = false;
@@ -647,7 +672,9 @@ int Solver::write_dat(const std::string &dir) {
         if (!d.domain_bc(0, ft).active) continue;  // BoundaryDataController::get_used_fields
         Field *f = m_fields->by_name(names[ft]);
         if (f == nullptr) continue;  // no such field on this path (rho, nu)
-        ARTSS_HIP(hipMemcpy(host.data(), f->data, sizeof(double) * host.size(), hipMemcpyDeviceToHost));
+        ARTSS_HIP(hipMemcpyAsync(host.data(), f->data, sizeof(double) * host.size(), hipMemcpyDeviceToHost,
+                                 d.stream()));
+        ARTSS_HIP(hipStreamSynchronize(d.stream()));
         {
             std::ofstream out(dir + "/" + names[ft] + ".dat");
             if (!out) throw std::invalid_argument("cannot write " + dir + "/" + names[ft] + ".dat");

@@ -42,6 +42,7 @@ struct StepCounters {
     int vcycles = 0;
     double residual = -1.0;           // final pressure residual when measured
     double initial_residual = -1.0;   // pressure residual at the start of the iteration (GCR methods)
+    bool converged = true;            // a residual-target solve met its target (always true otherwise)
 };
 
 // How the pressure equation is solved each step.
@@ -156,6 +157,7 @@ class Solver {
     int m_gcr_tail = 0;
     std::unique_ptr<ExplicitEulerSource> m_src_vel, m_src_temp;
     std::unique_ptr<GaussFunction> m_gauss;
+    std::unique_ptr<BuoyancyMMS> m_mms;
     // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver, 3 NSTempSolver,
     // 4 NSTurbSolver, 5 DiffusionSolver, 6 AdvectionDiffusionSolver,
     // 7 DiffusionTurbSolver, 8 AdvectionSolver

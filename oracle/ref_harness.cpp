@@ -11,7 +11,10 @@
 // oracle/ can be pinned against the real reference.
 //
 // Modes
-//   run   <xml> <outdir> <dump_steps,comma> [ic]   whole solver time loop
+//   run   <xml> <outdir> <dump_steps,comma> [ic] [max_steps]
+//                                                   whole solver time loop
+//                                                   (ic "-" = none; stops
+//                                                   after max_steps if given)
 //   time  <xml> <steps> [ic]                        time loop, no dumps
 //   lists <xml> <outdir>                            index lists per level
 //   state <xml> <dir> <first> <count>               continue from a saved state, timed
@@ -482,7 +485,10 @@ int main(int argc, char **argv) {
     }
     const std::string mode = argv[1];
     try {
-        if (mode == "run" && argc >= 5) return mode_run(argv[2], argv[3], argv[4], argc > 5 ? argv[5] : "", true, -1);
+        if (mode == "run" && argc >= 5) {
+            const std::string ic = argc > 5 && std::string(argv[5]) != "-" ? argv[5] : "";
+            return mode_run(argv[2], argv[3], argv[4], ic, true, argc > 6 ? std::stol(argv[6]) : -1);
+        }
         if (mode == "time" && argc >= 4)
             return mode_run(argv[2], "", "", argc > 4 ? argv[4] : "", false, std::stol(argv[3]));
         if (mode == "lists" && argc >= 4) return mode_lists(argv[2], argv[3]);

@@ -15,6 +15,18 @@ BCS = {"neumann": 0, "dirichlet": 1, "periodic": 2}
 AXES = {"x": 0, "y": 1, "z": 2}
 
 
+def axes(text):
+    """Mapping::match_axis (Mapping.cpp:17-23): entries matched without
+    regard to case; one that names no axis ("xyz") is UNKNOWN_AXIS, which no
+    caller looks for, so it is dropped; each axis is kept once"""
+    out = []
+    for a in text.split(","):
+        i = AXES.get(a.lower())
+        if i is not None and i not in out:
+            out.append(i)
+    return out
+
+
 def _values(el):
     """Settings.cpp map_parameter_section: attributes + child element texts."""
     vals = dict(el.attrib)
@@ -93,7 +105,7 @@ def parse(path_or_text):
         elif name == "source":
             v = _values(el)
             s = {"type": v["type"], "force_fct": v["force_fct"],
-                 "dir": [AXES[a] for a in v["dir"].split(",")]}
+                 "dir": axes(v["dir"])}
             if v["force_fct"] == "Buoyancy":
                 s["use_init_values"] = _yes(v, "use_init_values")
                 if not s["use_init_values"]:
@@ -112,7 +124,7 @@ def parse(path_or_text):
                 t["prandtl"] = float(_values(turb)["Pr_T"])
             src = _values(el.find("source"))
             t["source"] = {"type": src["type"], "temp_fct": src["temp_fct"],
-                           "dir": [AXES[a] for a in src["dir"].split(",")],
+                           "dir": axes(src["dir"]),
                            "dissipation": _yes(src, "dissipation"), "random": _yes(src, "random")}
             if src["temp_fct"] == "Gauss":
                 t["source"]["gauss"] = [float(src[k]) for k in

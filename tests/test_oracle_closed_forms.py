@@ -189,3 +189,35 @@ def test_gauss_source_integral_is_hrr_over_cp():
     dom.gauss_spatial(out, (hrr, cp, 0.45, 0.1, 0.4, 0.1, 0.15, 0.12))
     total = out.sum() * hx * hy * hz
     assert math.isclose(total, hrr / cp, rel_tol=1e-12), total
+
+
+def test_vcycle_reproduces_reference_pressure_fixture():
+    """The reference's own pressure test (tests/pressure: 64^3 over [0, 2]^3,
+    five levels, SinSinSin right-hand side with l = 2, Dirichlet 0, one step
+    with the residual-driven cycle loop of VCycleMG::pressure) against its
+    fixture p_ref.dat, to the fixture's six digits (the file lists the inner
+    cells first, then the ghost cells)."""
+    import artss_xml
+    import oracle_py
+    import refxml
+
+    z = refxml.load("pressure")
+    cfg = artss_xml.parse(bytes(z["xml"]).decode())
+    dom = oracle_py.Domain(cfg)
+    info = dom.info(0)
+    N, h = info["N"], info["h"]
+    l = float(cfg["ic"]["values"]["l"])
+    # Functions::sin_sin_sin (Functions.cpp:781-811) on the inner cells
+    lo = [cfg["domain"][k] for k in ("X1", "Y1", "Z1")]
+    rhs = dom.field()
+    inner = dom.list(0, 0).astype(np.int64)
+    k, rem = np.divmod(inner, N[0] * N[1])
+    j, i = np.divmod(rem, N[0])
+    x = [lo[0] + (i - 0.5) * h[0], lo[1] + (j - 0.5) * h[1], lo[2] + (k - 0.5) * h[2]]
+    rhs[inner] = np.sin(l * np.pi * x[0]) * np.sin(l * np.pi * x[1]) * np.sin(l * np.pi * x[2])
+    p = dom.field()
+    dom.apply_bc(p, 4)
+    mg = oracle_py.MG(dom, cfg)
+    mg.pressure(p, rhs, cfg["dt"], cfg["dt"])
+    perm = refxml.inner_first(z["dims"])
+    assert refxml.same6(p[perm], z["refdat/p"]) == 0
