@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libartss_hip.so")
 ARCH = os.environ.get("ARTSS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["kernels.hip", "mg_tail.hip", "krylov_kernels.hip", "krylov.cpp", "unet_kernels.hip", "unet.cpp",
-           "domain.cpp", "operators.cpp", "vcycle.cpp", "solver.cpp", "functions.cpp", "ensemble.cpp", "settings.cpp", "xml.cpp",
+           "domain.cpp", "operators.cpp", "dyn_smagorinsky.cpp", "vcycle.cpp", "solver.cpp", "functions.cpp", "ensemble.cpp", "settings.cpp", "xml.cpp",
            "capi.cpp"]
 
 # -ffp-contract=off keeps every FP64 expression in the reference's operation

@@ -813,6 +813,32 @@ std::vector<double> Domain::gauss_spatial(const GaussSettings &g) const {
     return out;
 }
 
+// source/Cube.cpp:35-80.  The reference computes the last index of the box
+// from the domain's end coordinate (Utility::get_index(x_end, dx, X2)),
+// which gives an index at or below 1 for every box inside the domain and a
+// negative value converted to size_t otherwise, so its loop sets no cell or
+// runs past the array.  Here the box covers the cells whose centres lie in
+// it, by the rule the obstacles use (Obstacle.cpp: round((x1 - X1) / dx) + 1
+// .. round((x2 - X1) / dx)), clipped to the inner cells; cells inside
+// obstacles keep zero.
+std::vector<double> Domain::cube_source(const CubeSettings &c) const {
+    const LevelDesc &L = m_level[0];
+    std::vector<double> out(static_cast<size_t>(L.size), 0.0);
+    const double lo[3] = {m_settings.X[0], m_settings.Y[0], m_settings.Z[0]};
+    int s[3], e[3];
+    for (int a = 0; a < 3; a++) {
+        s[a] = std::max(1, static_cast<int>(std::round((c.start[a] - lo[a]) / L.h[a])) + 1);
+        e[a] = std::min(L.n[a], static_cast<int>(std::round((c.end[a] - lo[a]) / L.h[a])));
+    }
+    for (int k = s[2]; k <= e[2]; k++)
+        for (int j = s[1]; j <= e[1]; j++)
+            for (int i = s[0]; i <= e[0]; i++)
+                if (!in_obstacle(L, i, j, k))
+                    out[static_cast<size_t>(i) + static_cast<size_t>(L.N[0]) * (j + static_cast<size_t>(L.N[1]) * k)] =
+                        c.value;
+    return out;
+}
+
 void Domain::timer_clear() {
     for (auto &e : m_timer_events) {
         (void)hipEventDestroy(e.first);

@@ -108,6 +108,8 @@ class Domain {
 
     // GaussFunction::create_spatial_values (GaussFunction.cpp:52-109), host
     std::vector<double> gauss_spatial(const GaussSettings &g) const;
+    // source/Cube.cpp:35-80: `value` on the inner cells of the box; see domain.cpp
+    std::vector<double> cube_source(const CubeSettings &c) const;
 
     // Live kernel timing: while a tag is armed, every launch issued through
     // timed(tag, level, ...) is bracketed by HIP events on the domain stream.

@@ -101,6 +101,9 @@ void restrict_sweep0(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, co
 // for a level size, timed on scratch arrays; call outside stream capture
 void tune_level(hipStream_t s, const LevelDesc &L, const BCView &V);
 int tuned_kz(long long level_size);  // 0: not tuned (the size rule applies)
+// 0: autotuned choice; 1, 2 or 4: every fused level kernel uses that many
+// cells per thread along z (results are the same for every choice)
+void set_forced_kz(int kz);
 // src == nullptr (first sweeps only): the sweep starts from zero everywhere
 template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
@@ -131,6 +134,25 @@ void gate_open(hipStream_t s, IterGate *g, double tol, int max_iter);
 // reduce_sum into g->sum, then count the iteration and set done as the
 // loop `while (sqrt(sum) > tol && count < max_iter)` decides; nothing once done
 void reduce_gate(hipStream_t s, const double *partial, int n, IterGate *g);
+
+// ---- DynamicSmagorinsky (turbulence/DynamicSmagorinsky.cpp) ----
+constexpr int kDynMax = 12;
+struct DynArrays {
+    double *p[kDynMax];
+};
+struct DynConstArrays {
+    const double *p[kDynMax];
+};
+void dyn_filter(hipStream_t s, const LevelDesc &L, const DynArrays &out, const DynConstArrays &in, int count);
+void dyn_products(hipStream_t s, const LevelDesc &L, const double *uf, const double *vf, const double *wf,
+                  const DynArrays &prod);
+void dyn_strain(hipStream_t s, const LevelDesc &L, const double *uf, const double *vf, const double *wf,
+                const DynArrays &out, double *sbar);
+void dyn_coef(hipStream_t s, const LevelDesc &L, double c2, double a2, const DynConstArrays &filt,
+              const DynConstArrays &prod, const DynConstArrays &sf, double *cs);
+// the in-place average in index order as (n_x + 2 n_y + 4 n_z) wavefront launches
+void dyn_average(hipStream_t s, const LevelDesc &L, double delta_s, double nu, double *cs, const double *sbar,
+                 double *ev);
 
 // ---- IPressure (IPressure.cpp) ----
 void divergence(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *y, const double *z);

@@ -16,6 +16,8 @@
 #include "stencil.h"
 
 #include <cmath>
+#include <array>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -42,14 +44,22 @@ inline dim3 block() { return dim3(BX, BY, 1); }
 // as it is); the count is a kernel argument.
 constexpr int KZ = 4;
 constexpr long long kBatchCells = 1LL << 20;  // levels with at least this many cells batch
-// cells per thread along z chosen by tune_level for a level size (guarded by
-// kz_lock), else the size rule above
+// cells per thread along z chosen by tune_level per level shape and launch
+// extent (inner box or whole array; guarded by kz_lock), else the size rule
+// above; set_forced_kz overrides both (the values never depend on it)
+using TuneKey = std::array<long long, 4>;  // N0, N1, N2, shell_filled
+inline TuneKey tune_key(const LevelDesc &L, const BCView &V) {
+    return {L.N[0], L.N[1], L.N[2], V.shell_filled ? 1 : 0};
+}
 std::mutex kz_lock;
-std::map<long long, int> kz_tuned;
-inline int kz_of(const LevelDesc &L) {
+std::map<TuneKey, int> kz_tuned;
+std::atomic<int> kz_forced{0};
+inline int kz_of(const LevelDesc &L, const BCView &V) {
+    const int forced = kz_forced.load(std::memory_order_relaxed);
+    if (forced > 0) return forced;
     {
         std::lock_guard<std::mutex> g(kz_lock);
-        const auto it = kz_tuned.find(L.size);
+        const auto it = kz_tuned.find(tune_key(L, V));
         if (it != kz_tuned.end()) return it->second;
     }
     return L.size >= kBatchCells ? KZ : 1;
@@ -112,10 +122,12 @@ __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, lon
     if (gate != nullptr && *gate) return;
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i];
 }
-// each source into two destinations, for up to four fields at once
+// each source into two destinations, for F (1..4) fields at once
+template <int F>
 __global__ void k_copy_fan(CopyFan f, long long n) {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
-        for (int q = 0; q < f.fields; q++) {
+#pragma unroll
+        for (int q = 0; q < F; q++) {
             const double v = f.src[q][i];
             f.dst0[q][i] = v;
             f.dst1[q][i] = v;
@@ -904,8 +916,13 @@ void dense_apply(hipStream_t s, int n, const double *M, const uint32_t *cells, c
     k_dense_apply<<<(n + 3) / 4, 256, sizeof(double) * static_cast<size_t>(n), s>>>(n, M, cells, b, x);
 }
 void copy_fan(hipStream_t s, const CopyFan &f, long long n) {
-    if (f.fields < 1 || f.fields > 4) throw std::invalid_argument("copy_fan: 1 to 4 fields");
-    k_copy_fan<<<grid1d(n, 256), 256, 0, s>>>(f, n);
+    switch (f.fields) {
+    case 1: k_copy_fan<1><<<grid1d(n, 256), 256, 0, s>>>(f, n); break;
+    case 2: k_copy_fan<2><<<grid1d(n, 256), 256, 0, s>>>(f, n); break;
+    case 3: k_copy_fan<3><<<grid1d(n, 256), 256, 0, s>>>(f, n); break;
+    case 4: k_copy_fan<4><<<grid1d(n, 256), 256, 0, s>>>(f, n); break;
+    default: throw std::invalid_argument("copy_fan: 1 to 4 fields");
+    }
 }
 void fill(hipStream_t s, double *dst, double v, long long n) { k_fill<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
 void fill32(hipStream_t s, float *dst, float v, long long n) { k_fill32<<<grid1d(n, 256), 256, 0, s>>>(dst, v, n); }
@@ -1015,9 +1032,10 @@ void prolongate_add(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, dou
 // fused level kernel of that size (the values do not depend on the choice).
 // Not during stream capture: it synchronizes on events.
 void tune_level(hipStream_t s, const LevelDesc &L, const BCView &V) {
+    const TuneKey key = tune_key(L, V);
     {
         std::lock_guard<std::mutex> g(kz_lock);
-        if (kz_tuned.count(L.size)) return;
+        if (kz_tuned.count(key)) return;
     }
     const size_t n = static_cast<size_t>(L.size);
     float *buf = nullptr;
@@ -1054,16 +1072,22 @@ void tune_level(hipStream_t s, const LevelDesc &L, const BCView &V) {
     ARTSS_HIP(hipStreamSynchronize(s));
     ARTSS_HIP(hipFree(buf));
     std::lock_guard<std::mutex> g(kz_lock);
-    kz_tuned.emplace(L.size, best);
+    kz_tuned.emplace(key, best);
 }
 int tuned_kz(long long level_size) {
     std::lock_guard<std::mutex> g(kz_lock);
-    const auto it = kz_tuned.find(level_size);
-    return it == kz_tuned.end() ? 0 : it->second;
+    for (const auto &kv : kz_tuned)
+        if (kv.first[0] * kv.first[1] * kv.first[2] == level_size) return kv.second;
+    return 0;
+}
+void set_forced_kz(int kz) {
+    if (kz != 0 && kz != 1 && kz != 2 && kz != KZ)
+        throw std::invalid_argument("cells per thread along z must be 0 (autotuned), 1, 2 or 4");
+    kz_forced.store(kz);
 }
 
 void copy_bc(hipStream_t s, const LevelDesc &L, const BCView &V, double *dst, const double *src) {
-    const int kz = kz_of(L);
+    const int kz = kz_of(L, V);
     const dim3 g = zbatch(V.shell_filled ? inner_grid(L) : full_grid(L), kz);
     k_copy_bc<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, dst, src);
 }
@@ -1071,7 +1095,7 @@ template <typename T, typename TB>
 void jacobi_bc(hipStream_t s, const LevelDesc &L, const BCView &V, bool first, T *dst, const T *src, const TB *b,
                double ax, double ay, double az, double beta, double dsign, double w) {
     if (src == nullptr && !first) throw std::invalid_argument("jacobi_bc: a sweep from zero must be a first sweep");
-    const int kz = kz_of(L);
+    const int kz = kz_of(L, V);
     const dim3 g = zbatch(first && !V.shell_filled ? full_grid(L) : inner_grid(L), kz);
     const dim3 gi = with_items(g, dev::bc_items_grid(V));
     const T ax_ = static_cast<T>(ax), ay_ = static_cast<T>(ay), az_ = static_cast<T>(az), beta_ = static_cast<T>(beta),
@@ -1096,7 +1120,7 @@ template void jacobi_bc<float, float>(hipStream_t, const LevelDesc &, const BCVi
 
 template <typename T, typename TB>
 void residuum_faces(hipStream_t s, const LevelDesc &L, const BCView &V, T *out, const T *x, const TB *b) {
-    const int kz = kz_of(L);
+    const int kz = kz_of(L, V);
     const dim3 g = zbatch(inner_grid(L), kz);
     k_residuum_faces<T, TB><<<with_items(g, V.n_faces), block(), 0, s>>>(L, V, static_cast<int>(g.z), kz, out, x, b);
 }
@@ -1110,7 +1134,7 @@ template void residuum_faces<float, float>(hipStream_t, const LevelDesc &, const
 template <typename Tf, typename Tc>
 void prolongate_add_bc(hipStream_t s, const LevelDesc &Lc, const LevelDesc &Lf, const BCView &V, Tf *dst,
                        const Tf *fine, const Tc *coarse) {
-    const int kz = kz_of(Lf);
+    const int kz = kz_of(Lf, V);
     const dim3 g = zbatch(V.shell_filled ? inner_grid(Lf) : full_grid(Lf), kz);
     k_prolongate_add_bc<Tf, Tc><<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(Lc, Lf, V, static_cast<int>(g.z), kz,
                                                                                      dst, fine, coarse);
@@ -1160,6 +1184,150 @@ void projection(hipStream_t s, const LevelDesc &L, double *ou, double *ov, doubl
                 const double *iv, const double *iw, const double *p) {
     const double hx = 0.5 * (1. / L.h[0]), hy = 0.5 * (1. / L.h[1]), hz = 0.5 * (1. / L.h[2]);
     k_projection<<<inner_grid(L), block(), 0, s>>>(L, hx, hy, hz, ou, ov, ow, iu, iv, iw, p);
+}
+
+// ------------------------------------------------------------------ dynamic Smagorinsky
+// DynamicSmagorinsky.cpp:172-427.  Every kernel writes the inner fluid cells
+// of its outputs only; the other cells of the work fields stay zero.
+
+// explicit_filtering (:405-427) of `count` arrays at once: the trapezoidal
+// 3x3x3 filter summed with the x offset outermost, then y, then z
+__global__ void k_dyn_filter(LevelDesc L, DynArrays out, DynConstArrays in, int count) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    const double a[3] = {1. / 4., 1. / 2., 1. / 4.};
+    for (int q = 0; q < count; q++) {
+        const double *f = in.p[q];
+        double sum = 0;
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+#pragma unroll
+                for (int n = 0; n < 3; n++) sum = sum + a[l] * a[m] * a[n] * f[idx - 1 + l + nj * (m - 1) + nk * (n - 1)];
+        out.p[q][idx] = sum;
+    }
+}
+
+// :212-228: the six products of the filtered velocities (the reference forms
+// both its "velocity products" and its "filtered velocity products" this way)
+__global__ void k_dyn_products(LevelDesc L, const double *__restrict__ uf, const double *__restrict__ vf,
+                               const double *__restrict__ wf, DynArrays prod) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const double u = uf[idx], v = vf[idx], w = wf[idx];
+    prod.p[0][idx] = u * u;
+    prod.p[1][idx] = v * v;
+    prod.p[2][idx] = w * w;
+    prod.p[3][idx] = u * v;
+    prod.p[4][idx] = u * w;
+    prod.p[5][idx] = v * w;
+}
+
+// :256-282: strain tensor of the filtered velocity, its modulus, and the
+// products modulus x tensor (S11 S22 S33 S12 S13 S23 in out.p[0..5], the
+// products in out.p[6..11])
+__global__ void k_dyn_strain(LevelDesc L, double rdx, double rdy, double rdz, const double *__restrict__ uf,
+                             const double *__restrict__ vf, const double *__restrict__ wf, DynArrays out,
+                             double *__restrict__ sbar) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    double S[6];
+    S[0] = (uf[idx + 1] - uf[idx - 1]) * 0.5 * rdx;
+    S[1] = (vf[idx + nj] - vf[idx - nj]) * 0.5 * rdy;
+    S[2] = (wf[idx + nk] - wf[idx - nk]) * 0.5 * rdz;
+    S[3] = 0.5 * ((uf[idx + nj] - uf[idx - nj]) * 0.5 * rdy + (vf[idx + 1] - vf[idx - 1]) * 0.5 * rdx);
+    S[4] = 0.5 * ((uf[idx + nk] - uf[idx - nk]) * 0.5 * rdz + (wf[idx + 1] - wf[idx - 1]) * 0.5 * rdx);
+    S[5] = 0.5 * ((vf[idx + nk] - vf[idx - nk]) * 0.5 * rdz + (wf[idx + nj] - wf[idx - nj]) * 0.5 * rdy);
+    const double sb = sqrt(2. * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2. * (S[3] * S[3]) + 2. * (S[4] * S[4]) +
+                                 2. * (S[5] * S[5])));
+    sbar[idx] = sb;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        out.p[c][idx] = S[c];
+        out.p[6 + c][idx] = sb * S[c];
+    }
+}
+
+// :308-350: Leonard stress, the filtered strain modulus, M, and the raw
+// coefficient num / den with the reference's products where sums belong.
+// filt.p[0..5]: filtered products uu vv ww uv uw vw (uw: never filtered, zero);
+// prod.p[0..5]: the products; sf.p[0..5]: filtered strain, sf.p[6..11]:
+// filtered modulus x strain.  c2 = 2 delta_s delta_s, a2 = alpha alpha.
+__global__ void k_dyn_coef(LevelDesc L, double c2, double a2, DynConstArrays filt, DynConstArrays prod,
+                           DynConstArrays sf, double *__restrict__ cs) {
+    int i, j, k;
+    if (!inner_cell(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    double Lt[6], M[6], S[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        Lt[c] = filt.p[c][idx] - prod.p[c][idx];
+        S[c] = sf.p[c][idx];
+    }
+    const double sb = sqrt(2. * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2. * (S[3] * S[3]) + 2. * (S[4] * S[4]) +
+                                 2. * (S[5] * S[5])));
+#pragma unroll
+    for (int c = 0; c < 6; c++) M[c] = c2 * (sf.p[6 + c][idx] - a2 * sb * S[c]);
+    const double num = Lt[0] * M[0] + Lt[1] * M[1] * Lt[2] * M[2] + 2.0 * Lt[3] * M[3] + 2.0 * Lt[4] * M[4] +
+                       2.0 * Lt[5] * M[5];
+    const double den = M[0] * M[0] + M[1] * M[1] * M[2] * M[2] + 2.0 * M[3] * M[3] + 2.0 * M[4] * M[4] +
+                       2.0 * M[5] * M[5];
+    cs[idx] = num / den;
+}
+
+// :372-391 for the cells i + 2 j + 4 k = t: the in-place average over the
+// 27 neighbours reads averaged values from every neighbour earlier in index
+// order (all of them lie on smaller t) and raw values from the later ones
+// (larger t), exactly as the sequential loop in index order does; then the
+// eddy viscosity of the cell and its clamp.  One thread per (j, k).
+__global__ void k_dyn_average(LevelDesc L, int t, double delta_s, double nu, double *__restrict__ cs,
+                              const double *__restrict__ sbar, double *__restrict__ ev) {
+    const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (q >= L.n[1] * L.n[2]) return;
+    const int j = 1 + q % L.n[1], k = 1 + q / L.n[1];
+    const int i = t - 2 * j - 4 * k;
+    if (i < 1 || i > L.n[0] || in_obstacle(L, i, j, k)) return;
+    const long long idx = lin(L, i, j, k);
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    double sum = 0;
+    for (int l = 0; l < 3; l++)
+        for (int m = 0; m < 3; m++)
+            for (int n = 0; n < 3; n++) sum = sum + cs[idx - 1 + l + nj * (m - 1) + nk * (n - 1)];
+    const double c = sum / 27.0;
+    cs[idx] = c;
+    double e = 2.0 * c * delta_s * delta_s * sbar[idx];
+    if ((e + nu) < 0) e = 0;
+    ev[idx] = e;
+}
+
+void dyn_filter(hipStream_t s, const LevelDesc &L, const DynArrays &out, const DynConstArrays &in, int count) {
+    if (count < 1 || count > kDynMax) throw std::invalid_argument("dyn_filter: 1 to 12 arrays");
+    k_dyn_filter<<<inner_grid(L), block(), 0, s>>>(L, out, in, count);
+}
+void dyn_products(hipStream_t s, const LevelDesc &L, const double *uf, const double *vf, const double *wf,
+                  const DynArrays &prod) {
+    k_dyn_products<<<inner_grid(L), block(), 0, s>>>(L, uf, vf, wf, prod);
+}
+void dyn_strain(hipStream_t s, const LevelDesc &L, const double *uf, const double *vf, const double *wf,
+                const DynArrays &out, double *sbar) {
+    k_dyn_strain<<<inner_grid(L), block(), 0, s>>>(L, 1. / L.h[0], 1. / L.h[1], 1. / L.h[2], uf, vf, wf, out, sbar);
+}
+void dyn_coef(hipStream_t s, const LevelDesc &L, double c2, double a2, const DynConstArrays &filt,
+              const DynConstArrays &prod, const DynConstArrays &sf, double *cs) {
+    k_dyn_coef<<<inner_grid(L), block(), 0, s>>>(L, c2, a2, filt, prod, sf, cs);
+}
+void dyn_average(hipStream_t s, const LevelDesc &L, double delta_s, double nu, double *cs, const double *sbar,
+                 double *ev) {
+    const long long cols = static_cast<long long>(L.n[1]) * L.n[2];
+    const unsigned g = static_cast<unsigned>((cols + 255) / 256);
+    const int t_last = L.n[0] + 2 * L.n[1] + 4 * L.n[2];
+    for (int t = 1 + 2 + 4; t <= t_last; t++) k_dyn_average<<<g, 256, 0, s>>>(L, t, delta_s, nu, cs, sbar, ev);
 }
 
 void smagorinsky(hipStream_t s, const LevelDesc &L, double *ev, const double *u, const double *v, const double *w,

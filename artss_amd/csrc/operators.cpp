@@ -1,5 +1,7 @@
 #include "operators.h"
 
+#include "dyn_smagorinsky.h"
+
 #include <cmath>
 
 namespace artss {
@@ -265,6 +267,7 @@ std::unique_ptr<IDiffusion> make_diffusion(Domain &d, const DiffusionSettings &s
 
 std::unique_ptr<ITurbulence> make_turbulence(Domain &d, const Settings &s) {
     if (s.turbulence_type == "ConstSmagorinsky") return std::make_unique<ConstSmagorinsky>(d, s.cs);
+    if (s.turbulence_type == "DynamicSmagorinsky") return std::make_unique<DynamicSmagorinsky>(d);
     throw config_error("turbulence method '" + s.turbulence_type + "' is not implemented");
 }
 

@@ -121,6 +121,32 @@ BoundarySpec parse_boundary(const xml::Element *e, const std::string &ctx) {
     return b;
 }
 
+// Settings.cpp:752-769
+GaussSettings parse_gauss(const Map &v, const std::string &c) {
+    GaussSettings g;
+    g.hrr = req_real(v, "HRR", c);
+    g.cp = req_real(v, "cp", c);
+    const char *ax[3] = {"x", "y", "z"};
+    for (int a = 0; a < 3; a++) {
+        g.pos[a] = req_real(v, std::string(ax[a]) + "0", c);
+        g.sigma[a] = req_real(v, std::string("sigma_") + ax[a], c);
+    }
+    g.tau = req_real(v, "tau", c);
+    return g;
+}
+
+// Settings.cpp:771-781
+CubeSettings parse_cube(const Map &v, const std::string &c) {
+    CubeSettings cube;
+    const char *ax[3] = {"x", "y", "z"};
+    for (int a = 0; a < 3; a++) {
+        cube.start[a] = req_real(v, std::string(ax[a]) + "_start", c);
+        cube.end[a] = req_real(v, std::string(ax[a]) + "_end", c);
+    }
+    cube.value = req_real(v, "val", c);
+    return cube;
+}
+
 // Settings.cpp:613-635
 DiffusionSettings parse_diffusion(const xml::Element *head, const std::string &ctx) {
     const xml::Element *e = section(head, "diffusion", ctx);
@@ -168,6 +194,7 @@ Settings parse_settings(const std::string &xml_text) {
         s.kappa = req_real(pv, "kappa", "physical_parameters");
         s.has_kappa = true;
     }
+    if (d == "NSTempTurbConSolver" || d == "NSTempConSolver") s.gamma = req_real(pv, "gamma", "physical_parameters");
 
     // domain_parameters, Settings.cpp:126-148
     Map dv = values_of(section(root.get(), "domain_parameters", ""));
@@ -257,18 +284,37 @@ Settings parse_settings(const std::string &xml_text) {
         s.temp_random = req_bool(tsv, "random", "solver/temperature/source");
         s.temp_fct = req_str(tsv, "temp_fct", "solver/temperature/source");
         if (s.temp_fct == "Gauss") {
-            const std::string c = "solver/temperature/source/Gauss";
-            s.gauss.hrr = req_real(tsv, "HRR", c);
-            s.gauss.cp = req_real(tsv, "cp", c);
-            const char *ax[3] = {"x", "y", "z"};
-            for (int a = 0; a < 3; a++) {
-                s.gauss.pos[a] = req_real(tsv, std::string(ax[a]) + "0", c);
-                s.gauss.sigma[a] = req_real(tsv, std::string("sigma_") + ax[a], c);
-            }
-            s.gauss.tau = req_real(tsv, "tau", c);
+            s.gauss = parse_gauss(tsv, "solver/temperature/source/Gauss");
         } else if (s.temp_fct != "Zero" && s.temp_fct != "Buoyancy") {  // Buoyancy: BuoyancyMMS, no parameters
             throw config_error("temperature source function '" + s.temp_fct + "' has no implementation here.");
         }
+    }
+
+    // concentration, Settings.cpp:840-888 (with turbulence for NSTempTurbConSolver, :938, :953)
+    if (d == "NSTempConSolver" || d == "NSTempTurbConSolver") {
+        const std::string ctx = "solver/concentration";
+        const xml::Element *ce = section(solver, "concentration", "solver");
+        ConcentrationSettings &c = s.concentration;
+        s.has_concentration = true;
+        Map cav = values_of(section(ce, "advection", ctx));
+        c.advection_type = req_str(cav, "type", ctx + "/advection");
+        c.diffusion = parse_diffusion(ce, ctx);
+        if (d == "NSTempTurbConSolver") {
+            Map ctv = values_of(section(ce, "turbulence", ctx));
+            c.turbulence = req_bool(ctv, "include", ctx + "/turbulence");
+            if (c.turbulence) c.schmidt = req_real(ctv, "Sc_T", ctx + "/turbulence");
+        }
+        Map csv = values_of(section(ce, "source", ctx));
+        c.source_type = req_str(csv, "type", "source");
+        c.dir = axes(req_str(csv, "dir", "source"));
+        if (req_bool(csv, "random", "source")) throw config_error("random concentration source is not supported");
+        c.con_fct = req_str(csv, "con_fct", "source");
+        if (c.con_fct == "Gauss")
+            c.gauss = parse_gauss(csv, "source/Gauss");
+        else if (c.con_fct == "Cube")
+            c.cube = parse_cube(csv, "source/Cube");
+        else
+            throw config_error("concentration source function '" + c.con_fct + "' has no parsing implementation.");
     }
 
     // boundaries, Settings.cpp:391-402

@@ -52,9 +52,27 @@ struct GaussSettings {
     double hrr = 0, cp = 1, pos[3] = {0, 0, 0}, sigma[3] = {1, 1, 1}, tau = 1;
 };
 
+// source/Cube.cpp: a constant value on a box of cells (Settings.cpp:771-781)
+struct CubeSettings {
+    double start[3] = {0, 0, 0}, end[3] = {0, 0, 0}, value = 0;
+};
+
+// the <concentration> section of NSTempConSolver / NSTempTurbConSolver
+// (Settings.cpp:840-888)
+struct ConcentrationSettings {
+    std::string advection_type;
+    DiffusionSettings diffusion;  // parsed, unused: the solvers diffuse with the velocity's settings
+    bool turbulence = false;
+    double schmidt = 0;  // Sc_T
+    std::string source_type, con_fct;  // con_fct "Gauss" or "Cube"
+    std::vector<int> dir;
+    GaussSettings gauss;
+    CubeSettings cube;
+};
+
 struct Settings {
     std::string solver;  // solver description, e.g. "NSTempTurbSolver"
-    double t_end = 0, dt = 0, nu = 0, kappa = 0, beta = 3.34e-3, g = -9.81, rhoa = 1;
+    double t_end = 0, dt = 0, nu = 0, kappa = 0, beta = 3.34e-3, g = -9.81, rhoa = 1, gamma = 0;
     bool has_nu = false, has_kappa = false;
     double X[2] = {0, 1}, Y[2] = {0, 1}, Z[2] = {0, 1};
     int n[3] = {1, 1, 1};
@@ -84,6 +102,9 @@ struct Settings {
     std::vector<int> temp_dir;
     bool dissipation = false, temp_random = false;
     GaussSettings gauss;
+
+    bool has_concentration = false;
+    ConcentrationSettings concentration;
 
     std::vector<BoundarySpec> boundaries;
     std::vector<ObstacleSpec> obstacles;

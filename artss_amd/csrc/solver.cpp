@@ -12,10 +12,25 @@
 
 namespace artss {
 
-FieldController::FieldController(Domain &d)
+FieldController::FieldController(Domain &d, bool concentration)
     : u(d, U), v(d, V), w(d, W), u0(d, U), v0(d, V), w0(d, W), u_tmp(d, U), v_tmp(d, V), w_tmp(d, W),
       nu_t(d, NU), kappa_t(d, artss::T), p(d, P), rhs(d, P), T(d, artss::T), T0(d, artss::T), T_tmp(d, artss::T),
-      T_ambient(d, artss::T), f_x(d, U), f_y(d, V), f_z(d, W), S_T(d, artss::T) {}
+      T_ambient(d, artss::T), f_x(d, U), f_y(d, V), f_z(d, W), S_T(d, artss::T) {
+    if (concentration) {
+        // FieldController.cpp:44-53 (concentration, its copies and source are
+        // field type RHO; gamma_t, like kappa_t, carries T's boundary data)
+        C = std::make_unique<Field>(d, RHO);
+        C0 = std::make_unique<Field>(d, RHO);
+        C_tmp = std::make_unique<Field>(d, RHO);
+        S_C = std::make_unique<Field>(d, RHO);
+        gamma_t = std::make_unique<Field>(d, RHO);
+    }
+}
+
+void FieldController::couple_concentration() {
+    k::CopyFan f{{C->data}, {C0->data}, {C_tmp->data}, 1};
+    k::copy_fan(C->domain().stream(), f, C->get_size());
+}
 
 // FieldController.cpp couple_vector / couple_scalar / update_data: the copies
 // into the previous-step and temporary fields, one pass per call
@@ -35,17 +50,33 @@ void FieldController::update_data() {
                  {u_tmp.data, v_tmp.data, w_tmp.data, T_tmp.data},
                  4};
     k::copy_fan(u.domain().stream(), f, u.get_size());
+    if (C) couple_concentration();
 }
 
 std::vector<std::pair<const char *, Field *>> FieldController::all() {
-    return {{"u", &u},         {"v", &v},         {"w", &w},         {"u0", &u0},   {"v0", &v0},
-            {"w0", &w0},       {"u_tmp", &u_tmp}, {"v_tmp", &v_tmp}, {"w_tmp", &w_tmp},
-            {"nu_t", &nu_t},   {"kappa_t", &kappa_t}, {"p", &p},     {"rhs", &rhs}, {"T", &T},
-            {"T0", &T0},       {"T_tmp", &T_tmp}, {"T_ambient", &T_ambient}, {"f_x", &f_x},
-            {"f_y", &f_y},     {"f_z", &f_z},     {"S_T", &S_T}};
+    std::vector<std::pair<const char *, Field *>> out = {
+        {"u", &u},         {"v", &v},         {"w", &w},         {"u0", &u0},   {"v0", &v0},
+        {"w0", &w0},       {"u_tmp", &u_tmp}, {"v_tmp", &v_tmp}, {"w_tmp", &w_tmp},
+        {"nu_t", &nu_t},   {"kappa_t", &kappa_t}, {"p", &p},     {"rhs", &rhs}, {"T", &T},
+        {"T0", &T0},       {"T_tmp", &T_tmp}, {"T_ambient", &T_ambient}, {"f_x", &f_x},
+        {"f_y", &f_y},     {"f_z", &f_z},     {"S_T", &S_T}};
+    if (C) {
+        out.push_back({"C", C.get()});
+        out.push_back({"C0", C0.get()});
+        out.push_back({"C_tmp", C_tmp.get()});
+        out.push_back({"S_C", S_C.get()});
+        out.push_back({"gamma_t", gamma_t.get()});
+    }
+    return out;
 }
 
 Field *FieldController::by_name(const std::string &n) {
+    // Analysis.cpp:368: the RHO field is the concentration
+    if (C && (n == "C" || n == "rho")) return C.get();
+    if (C && n == "C0") return C0.get();
+    if (C && n == "C_tmp") return C_tmp.get();
+    if (C && n == "S_C") return S_C.get();
+    if (C && n == "gamma_t") return gamma_t.get();
     if (n == "u") return &u;
     if (n == "v") return &v;
     if (n == "w") return &w;
@@ -81,31 +112,34 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     const Settings &s = d.settings();
     static const char *kinds[] = {"NSSolver",        "NSTempTurbSolver",         "PressureSolver",
                                   "NSTempSolver",    "NSTurbSolver",             "DiffusionSolver",
-                                  "AdvectionDiffusionSolver", "DiffusionTurbSolver", "AdvectionSolver"};
+                                  "AdvectionDiffusionSolver", "DiffusionTurbSolver", "AdvectionSolver",
+                                  "NSTempConSolver", "NSTempTurbConSolver"};
     m_kind = -1;
-    for (int k = 0; k < 9; k++)
+    for (int k = 0; k < 11; k++)
         if (s.solver == kinds[k]) m_kind = k;
     if (m_kind < 0)
         throw config_error("solver '" + s.solver + "' is not supported by this library (NSSolver, NSTurbSolver, NSTempSolver, "
-                           "NSTempTurbSolver, PressureSolver, DiffusionSolver, DiffusionTurbSolver, "
-                           "AdvectionDiffusionSolver, AdvectionSolver)");
+                           "NSTempTurbSolver, NSTempConSolver, NSTempTurbConSolver, PressureSolver, DiffusionSolver, "
+                           "DiffusionTurbSolver, AdvectionDiffusionSolver, AdvectionSolver)");
+    const bool con = m_kind == 9 || m_kind == 10;
+    const bool temp_turb = m_kind == 1 || m_kind == 10;  // the NSTempTurb step
+    const bool temp_lam = m_kind == 3 || m_kind == 9;    // the NSTemp step
     if (m_kind == 8 && s.ic != "GaussBubble")
         throw config_error("AdvectionSolver takes its velocity from the GaussBubble initial condition "
                            "(SolverController.cpp:54-56)");
     if (s.has_temperature && s.temp_random) throw config_error("random temperature source is not supported");
-    if (s.has_temperature && s.dissipation && m_kind != 3)
-        throw config_error("dissipation is only part of NSTempSolver (NSTempSolver.cpp:181-185)");
-    if (s.has_turbulence && s.turbulence_type != "ConstSmagorinsky")
-        throw config_error("turbulence model '" + s.turbulence_type + "' is not supported by this library");
+    if (s.has_temperature && s.dissipation && m_kind != 3 && !con)
+        throw config_error("dissipation is part of NSTempSolver and the concentration solvers only "
+                           "(NSTempSolver.cpp:181-185, NSTempConSolver.cpp, NSTempTurbConSolver.cpp)");
     if (s.ic_random) throw config_error("random initial conditions are not supported");
 
-    m_fields = std::make_unique<FieldController>(d);
+    m_fields = std::make_unique<FieldController>(d, con);
     // the solver constructors (SolverSelection::set_*): NSSolver.cpp:17-36,
     // NSTurbSolver.cpp, NSTempSolver.cpp:16-49, NSTempTurbSolver.cpp,
     // DiffusionSolver.cpp, AdvectionDiffusionSolver.cpp, DiffusionTurbSolver.cpp
-    const bool velocity_adv = m_kind == 0 || m_kind == 1 || m_kind == 3 || m_kind == 4 || m_kind == 6 || m_kind == 8;
+    const bool velocity_adv = m_kind == 0 || temp_turb || temp_lam || m_kind == 4 || m_kind == 6 || m_kind == 8;
     const bool velocity_dif = m_kind != 2 && m_kind != 8;
-    const bool ns = m_kind == 0 || m_kind == 1 || m_kind == 3 || m_kind == 4;
+    const bool ns = m_kind == 0 || temp_turb || temp_lam || m_kind == 4;
     if (velocity_adv) m_adv = make_advection(d, s.advection_type);
     if (velocity_dif) m_dif = make_diffusion(d, s.diffusion);
     if (ns) {
@@ -113,7 +147,7 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
         m_add_source = s.force_fct != "Zero";
     }
     if (ns || m_kind == 2) m_pres = std::make_unique<VCycleMG>(d, s.pressure);
-    if (m_kind == 1 || m_kind == 4 || m_kind == 7) m_turb = make_turbulence(d, s);
+    if (temp_turb || m_kind == 4 || m_kind == 7) m_turb = make_turbulence(d, s);
     if (m_kind == 8) {
         // AdvectionSolver.cpp:16-34: constant transport velocity from the GaussBubble parameters
         const char *keys[3] = {"u_lin", "v_lin", "w_lin"};
@@ -123,13 +157,33 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
             m_lin[a] = std::make_unique<Field>(d, U + a, std::strtod(it->second.c_str(), nullptr));
         }
     }
-    if (m_kind == 1 || m_kind == 3) {
+    if (temp_turb || temp_lam) {
         if (s.temp_advection_type != "SemiLagrangian") throw config_error("temperature advection must be SemiLagrangian");
         m_dif_temp = make_diffusion(d, s.diffusion);  // both solvers use the velocity settings (NSTempSolver.cpp:33)
         m_src_temp = std::make_unique<ExplicitEulerSource>(d, s.temp_dir);
         if (s.temp_fct == "Gauss") m_gauss = std::make_unique<GaussFunction>(d, s.gauss);
         if (s.temp_fct == "Buoyancy") m_mms = std::make_unique<BuoyancyMMS>(d);  // SolverSelection.cpp:148-149
         m_add_temp_source = s.temp_fct != "Zero";
+    }
+    if (con) {
+        // NSTempTurbConSolver.cpp:15-58, NSTempConSolver.cpp: the reference
+        // hands the concentration's advection to adv_temp a second time and
+        // leaves adv_con unset, so its step calls through an uninitialised
+        // pointer; here the concentration is advected by the method its own
+        // section names.
+        const ConcentrationSettings &c = s.concentration;
+        m_adv_con = make_advection(d, c.advection_type);
+        m_dif_con = make_diffusion(d, s.diffusion);
+        m_src_con = std::make_unique<ExplicitEulerSource>(d, c.dir);
+        if (c.con_fct == "Gauss") {
+            m_gauss_con = std::make_unique<GaussFunction>(d, c.gauss);
+        } else {
+            m_cube_con = std::make_unique<Field>(d, RHO);
+            const std::vector<double> host = d.cube_source(c.cube);
+            upload(d, m_cube_con->data, host.data(), host.size());
+            ARTSS_HIP(hipStreamSynchronize(d.stream()));
+        }
+        m_add_con_source = true;  // con_fct is Gauss or Cube (Settings.cpp:880-887)
     }
     set_up_fields();
 }
@@ -242,7 +296,7 @@ void Solver::set_up_fields() {
     // (FieldController.cpp:41,50-52); they hold zero here, so with use_init_values="Yes" T_ambient
     // stays zero, and the Uniform force of the other solvers adds zero, as the reference's fresh
     // allocations read in practice.
-    if (m_kind == 1 || m_kind == 3) {
+    if (s.has_temperature) {
         if (s.force_fct == "Buoyancy" && !s.use_init_values) f.T_ambient.set_value(s.ambient_temperature);
         if (s.force_fct == "Uniform") {
             // Functions::uniform (Functions.cpp:820-832): the value on the inner cells
@@ -263,6 +317,7 @@ void Solver::set_up_fields() {
     d.apply_boundary(f.w.data, W, 0);
     d.apply_boundary(f.p.data, P, 0);
     d.apply_boundary(f.T.data, artss::T, 0);
+    if (f.C) d.apply_boundary(f.C->data, RHO, 0);
     d.apply_boundary(f.T_ambient.data, artss::T, 0);
     update_sources(0);
     f.update_data();
@@ -280,6 +335,10 @@ void Solver::update_sources(double t) {
     }
     if (m_gauss) m_gauss->update_source(f.S_T, t);
     if (m_mms) m_mms->update_source(f.S_T, t);
+    // NSTempTurbConSolver.cpp:358-361: the concentration's source function
+    // (GaussFunction.cpp:121-123, Cube.cpp: a copy of its constant field)
+    if (m_gauss_con) m_gauss_con->update_source(*f.S_C, t);
+    if (m_cube_con) f.S_C->copy_data(*m_cube_con);
 }
 
 void Solver::pressure_step(double t) {
@@ -383,20 +442,53 @@ void Solver::do_step_nstempturb(double t) {
     m_pres->projection(f.u, f.v, f.w, f.u_tmp, f.v_tmp, f.w_tmp, f.p, false);
     m_adv->advect(f.T, f.T0, f.u, f.v, f.w, false);
     f.couple_scalar();
+    // NSTempTurbConSolver.cpp:149-169 adds the dissipation instead of the
+    // source when it is switched on; NSTempTurbSolver has no dissipation
+    const bool dissipate = m_kind == 10 && s.dissipation;
+    const bool after = dissipate || m_add_temp_source;  // a coupling follows the diffusion's
     if (s.temp_turbulence) {
         const double rPr_T = 1. / s.prandtl;
         k::scale_to(d.stream(), f.kappa_t.data, f.nu_t.data, rPr_T, f.kappa_t.get_size());
         m_dif_temp->diffuse(f.T, f.T0, f.T_tmp, s.kappa, f.kappa_t, false);
         m_counters.diffusion[3] = m_dif_temp->last_iterations();
-        if (!m_add_temp_source) f.couple_scalar();  // else overwritten below before any read
+        if (!after) f.couple_scalar();  // else overwritten below before any read
     } else if (s.kappa != 0.) {
         m_dif_temp->diffuse(f.T, f.T0, f.T_tmp, s.kappa, false);
         m_counters.diffusion[3] = m_dif_temp->last_iterations();
-        if (!m_add_temp_source) f.couple_scalar();
+        if (!after) f.couple_scalar();
     }
-    if (m_add_temp_source) {
+    if (dissipate) {
+        ISource::dissipate(f.T, f.u, f.v, f.w, false);
+        f.couple_scalar();
+    } else if (m_add_temp_source) {
         m_src_temp->add_source(f.T, f.S_T, false);
         f.couple_scalar();
+    }
+}
+
+// the concentration step of NSTempConSolver (NSTempConSolver.cpp:204-233) and
+// NSTempTurbConSolver (NSTempTurbConSolver.cpp:241-286), after the temperature
+void Solver::do_step_concentration() {
+    Domain &d = *m_domain;
+    const Settings &s = d.settings();
+    FieldController &f = *m_fields;
+    const ConcentrationSettings &c = s.concentration;
+    m_adv_con->advect(*f.C, *f.C0, f.u, f.v, f.w, false);
+    f.couple_concentration();
+    if (m_kind == 10 && c.turbulence) {
+        const double rSc_T = 1. / c.schmidt;
+        k::scale_to(d.stream(), f.gamma_t->data, f.nu_t.data, rSc_T, f.gamma_t->get_size());
+        m_dif_con->diffuse(*f.C, *f.C0, *f.C_tmp, s.gamma, *f.gamma_t, false);
+        f.couple_concentration();
+    } else if (s.gamma != 0.) {
+        m_dif_con->diffuse(*f.C, *f.C0, *f.C_tmp, s.gamma, false);
+        f.couple_concentration();
+    }
+    // NSTempTurbConSolver adds the source only when the concentration is
+    // turbulent (:277-285); NSTempConSolver whenever there is one (:222-230)
+    if (m_kind == 10 ? c.turbulence : m_add_con_source) {
+        m_src_con->add_source(*f.C, *f.S_C, false);
+        f.couple_concentration();
     }
 }
 
@@ -503,6 +595,14 @@ void Solver::step(double t) {
         case 2: pressure_step(t); break;
         case 3: do_step_nstemp(t); break;
         case 4: do_step_nsturb(t); break;
+        case 9:
+            do_step_nstemp(t);
+            do_step_concentration();
+            break;
+        case 10:
+            do_step_nstempturb(t);
+            do_step_concentration();
+            break;
         default: do_step_transport(); break;
     }
     update_sources(t);

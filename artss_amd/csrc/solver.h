@@ -18,17 +18,21 @@
 namespace artss {
 
 struct FieldController {
-    explicit FieldController(Domain &d);
+    // concentration: also the concentration fields C, C0, C_tmp, its source
+    // S_C and eddy diffusivity gamma_t (the concentration solvers only)
+    FieldController(Domain &d, bool concentration = false);
     Field u, v, w, u0, v0, w0, u_tmp, v_tmp, w_tmp;
     Field nu_t, kappa_t;
     Field p, rhs;
     Field T, T0, T_tmp, T_ambient;
     Field f_x, f_y, f_z, S_T;
+    std::unique_ptr<Field> C, C0, C_tmp, S_C, gamma_t;
 
     // FieldController.cpp:151-186
     void couple_vector();
     void couple_scalar();
-    // FieldController.cpp:119-135 (concentration fields are not part of this path)
+    void couple_concentration();
+    // FieldController.cpp:119-135
     void update_data();
     Field *by_name(const std::string &name);
     // every field, in a fixed order, with its name (checkpoints)
@@ -137,6 +141,7 @@ class Solver {
     void do_step_nstempturb(double t);
     void do_step_nstemp(double t);
     void do_step_nsturb(double t);
+    void do_step_concentration();
     void do_step_transport();
     void pressure_step(double t);
     void update_sources(double t);
@@ -160,8 +165,17 @@ class Solver {
     std::unique_ptr<BuoyancyMMS> m_mms;
     // 0 NSSolver, 1 NSTempTurbSolver, 2 PressureSolver, 3 NSTempSolver,
     // 4 NSTurbSolver, 5 DiffusionSolver, 6 AdvectionDiffusionSolver,
-    // 7 DiffusionTurbSolver, 8 AdvectionSolver
+    // 7 DiffusionTurbSolver, 8 AdvectionSolver, 9 NSTempConSolver,
+    // 10 NSTempTurbConSolver
     int m_kind = 0;
+    // the concentration solvers (9, 10): advection, diffusion (the velocity's
+    // settings, NSTempTurbConSolver.cpp:35), source and its function
+    std::unique_ptr<IAdvection> m_adv_con;
+    std::unique_ptr<IDiffusion> m_dif_con;
+    std::unique_ptr<ExplicitEulerSource> m_src_con;
+    std::unique_ptr<GaussFunction> m_gauss_con;
+    std::unique_ptr<Field> m_cube_con;  // Cube source: its constant field
+    bool m_add_con_source = false;
     std::unique_ptr<Field> m_lin[3];  // AdvectionSolver's constant velocity
     bool m_add_source = false, m_add_temp_source = false;
     StepCounters m_counters;

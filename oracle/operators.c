@@ -404,6 +404,115 @@ void or_smagorinsky(const or_domain *D, double *ev, const double *u, const doubl
     or_apply_bc(D, ev, ev_ft, 0);
 }
 
+/* ---------------- dynamic Smagorinsky ---------------- */
+/* DynamicSmagorinsky::explicit_filtering, turbulence/DynamicSmagorinsky.cpp:405-427:
+ * trapezoidal 3x3x3 filter on the inner cells, summed x offset outermost,
+ * then y, then z. */
+static void dyn_filter(const or_domain *D, double *out, const double *in) {
+    const double a[3] = {1. / 4., 1. / 2., 1. / 4.};
+    const long Nx = (long)D->N[0][0], NxNy = (long)(D->N[0][0] * D->N[0][1]);
+    const ivec *inner = &D->inner[0];
+    for (size_t q = 0; q < inner->n; ++q) {
+        const long i = (long)inner->d[q];
+        double sum = 0;
+        for (long l = 0; l < 3; l++)
+            for (long m = 0; m < 3; m++)
+                for (long n = 0; n < 3; n++)
+                    sum = sum + a[l] * a[m] * a[n] * in[i - 1 + l + Nx * (m - 1) + NxNy * (n - 1)];
+        out[i] = sum;
+    }
+}
+
+/* DynamicSmagorinsky::calc_turbulent_viscosity, DynamicSmagorinsky.cpp:172-393,
+ * operation by operation.  The reference's work fields are never written
+ * outside the inner cells and uw_f is never written at all (its filter call
+ * filters ww twice, :235-240); here they are zero, the value a fresh
+ * allocation reads.  The coefficient average (:372-384) runs in place in
+ * inner-list order, so cells later in the list see averaged earlier ones. */
+void or_dyn_smagorinsky(const or_domain *D, double *ev, const double *u, const double *v, const double *w,
+                        double nu) {
+    const size_t n = D->size[0];
+    const long Nx = (long)D->N[0][0], NxNy = (long)(D->N[0][0] * D->N[0][1]);
+    const double dx = D->h[0][0], dy = D->h[0][1], dz = D->h[0][2];
+    const double rdx = 1. / dx, rdy = 1. / dy, rdz = 1. / dz;
+    const double alpha = 2.0;
+    const double delta_s = cbrt(dx * dy * dz);
+    enum { UF, VF, WF, UU, VV, WW, UV, UW, VW, UU_F, VV_F, WW_F, UV_F, UW_F, VW_F, S11, S22, S33, S12, S13, S23,
+           P11, P22, P33, P12, P13, P23, S11F, S22F, S33F, S12F, S13F, S23F, P11F, P22F, P33F, P12F, P13F, P23F,
+           SBAR, CS, NWORK };
+    double *f[NWORK];
+    for (int q = 0; q < NWORK; q++) f[q] = (double *)calloc(n, sizeof(double));
+    const ivec *inner = &D->inner[0];
+    dyn_filter(D, f[UF], u);
+    dyn_filter(D, f[VF], v);
+    dyn_filter(D, f[WF], w);
+    /* :212-228: both the "velocity products" and the products of the filtered
+     * velocities are formed from the filtered velocities */
+    for (size_t q = 0; q < inner->n; ++q) {
+        const size_t i = inner->d[q];
+        f[UU][i] = f[UF][i] * f[UF][i];
+        f[VV][i] = f[VF][i] * f[VF][i];
+        f[WW][i] = f[WF][i] * f[WF][i];
+        f[UV][i] = f[UF][i] * f[VF][i];
+        f[UW][i] = f[UF][i] * f[WF][i];
+        f[VW][i] = f[VF][i] * f[WF][i];
+    }
+    dyn_filter(D, f[UU_F], f[UU]);
+    dyn_filter(D, f[VV_F], f[VV]);
+    dyn_filter(D, f[WW_F], f[WW]);
+    dyn_filter(D, f[UV_F], f[UV]);
+    dyn_filter(D, f[VW_F], f[VW]);
+    /* uw_f: not filtered (stays zero) */
+    for (size_t q = 0; q < inner->n; ++q) {
+        const long i = (long)inner->d[q];
+        const double *uf = f[UF], *vf = f[VF], *wf = f[WF];
+        f[S11][i] = (uf[i + 1] - uf[i - 1]) * 0.5 * rdx;
+        f[S22][i] = (vf[i + Nx] - vf[i - Nx]) * 0.5 * rdy;
+        f[S33][i] = (wf[i + NxNy] - wf[i - NxNy]) * 0.5 * rdz;
+        f[S12][i] = 0.5 * ((uf[i + Nx] - uf[i - Nx]) * 0.5 * rdy + (vf[i + 1] - vf[i - 1]) * 0.5 * rdx);
+        f[S13][i] = 0.5 * ((uf[i + NxNy] - uf[i - NxNy]) * 0.5 * rdz + (wf[i + 1] - wf[i - 1]) * 0.5 * rdx);
+        f[S23][i] = 0.5 * ((vf[i + NxNy] - vf[i - NxNy]) * 0.5 * rdz + (wf[i + Nx] - wf[i - Nx]) * 0.5 * rdy);
+        f[SBAR][i] = sqrt(2. * (f[S11][i] * f[S11][i] + f[S22][i] * f[S22][i] + f[S33][i] * f[S33][i] +
+                                2. * (f[S12][i] * f[S12][i]) + 2. * (f[S13][i] * f[S13][i]) +
+                                2. * (f[S23][i] * f[S23][i])));
+        for (int c = 0; c < 6; c++) f[P11 + c][i] = f[SBAR][i] * f[S11 + c][i];
+    }
+    for (int c = 0; c < 6; c++) dyn_filter(D, f[S11F + c], f[S11 + c]);
+    for (int c = 0; c < 6; c++) dyn_filter(D, f[P11F + c], f[P11 + c]);
+    for (size_t q = 0; q < inner->n; ++q) {
+        const size_t i = inner->d[q];
+        /* Leonard stress, :249-254 (uf_uf == uu and so on) */
+        const double L11 = f[UU_F][i] - f[UU][i], L22 = f[VV_F][i] - f[VV][i], L33 = f[WW_F][i] - f[WW][i];
+        const double L12 = f[UV_F][i] - f[UV][i], L13 = f[UW_F][i] - f[UW][i], L23 = f[VW_F][i] - f[VW][i];
+        const double sf = sqrt(2. * (f[S11F][i] * f[S11F][i] + f[S22F][i] * f[S22F][i] + f[S33F][i] * f[S33F][i] +
+                                     2. * (f[S12F][i] * f[S12F][i]) + 2. * (f[S13F][i] * f[S13F][i]) +
+                                     2. * (f[S23F][i] * f[S23F][i])));
+        double M[6];
+        for (int c = 0; c < 6; c++)
+            M[c] = 2.0 * delta_s * delta_s * (f[P11F + c][i] - alpha * alpha * sf * f[S11F + c][i]);
+        /* :340-347, with the reference's products where sums belong */
+        const double num = L11 * M[0] + L22 * M[1] * L33 * M[2] + 2.0 * L12 * M[3] + 2.0 * L13 * M[4] +
+                           2.0 * L23 * M[5];
+        const double den = M[0] * M[0] + M[1] * M[1] * M[2] * M[2] + 2.0 * M[3] * M[3] + 2.0 * M[4] * M[4] +
+                           2.0 * M[5] * M[5];
+        f[CS][i] = num / den;
+    }
+    for (size_t q = 0; q < inner->n; ++q) {
+        const long i = (long)inner->d[q];
+        double sum = 0;
+        for (long l = 0; l < 3; l++)
+            for (long m = 0; m < 3; m++)
+                for (long nn = 0; nn < 3; nn++) sum = sum + f[CS][i - 1 + l + Nx * (m - 1) + NxNy * (nn - 1)];
+        f[CS][i] = sum / 27.0;
+        ev[i] = 2.0 * f[CS][i] * delta_s * delta_s * f[SBAR][i];
+    }
+    for (size_t q = 0; q < inner->n; ++q) {
+        const size_t i = inner->d[q];
+        if ((ev[i] + nu) < 0) ev[i] = 0;
+    }
+    for (int q = 0; q < NWORK; q++) free(f[q]);
+}
+
 /* ---------------- sources ---------------- */
 /* ExplicitEulerSource::add_source, source/ExplicitEulerSource.cpp:84-104 */
 void or_add_source(const or_domain *D, double *out, const double *s, double dt, int ft) {

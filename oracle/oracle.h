@@ -81,6 +81,10 @@ void or_projection(const or_domain *D, double *ou, double *ov, double *ow, const
                    const double *iw, const double *ip);
 void or_smagorinsky(const or_domain *D, double *ev, const double *u, const double *v, const double *w, double cs,
                     int ev_ft);
+/* DynamicSmagorinsky::calc_turbulent_viscosity (turbulence/DynamicSmagorinsky.cpp:172-393); nu the
+ * molecular viscosity; ev written on the inner cells only */
+void or_dyn_smagorinsky(const or_domain *D, double *ev, const double *u, const double *v, const double *w,
+                        double nu);
 void or_add_source(const or_domain *D, double *out, const double *s, double dt, int ft);
 void or_buoyancy(const or_domain *D, double *out, const double *T, const double *Ta, double beta, double g);
 /* GaussFunction::create_spatial_values (HRR, cp, x0,y0,z0, sx,sy,sz) */
@@ -106,7 +110,8 @@ int or_mg_pressure(or_mg *mg, double *out, const double *b, double t, double dt)
 /* ---- whole solver (solver/NSSolver.cpp, NSTempTurbSolver.cpp, TimeIntegration.cpp) ---- */
 typedef struct {
     int solver;         /* 0 NSSolver, 1 NSTempTurbSolver, 3 NSTempSolver, 4 NSTurbSolver, 5 DiffusionSolver,
-                           6 AdvectionDiffusionSolver, 7 DiffusionTurbSolver, 8 AdvectionSolver */
+                           6 AdvectionDiffusionSolver, 7 DiffusionTurbSolver, 8 AdvectionSolver,
+                           9 NSTempConSolver, 10 NSTempTurbConSolver */
     double dt, nu, kappa, beta, g;
     int diff_type;      /* 0 Jacobi, 1 CGS, 2 Explicit */
     int diff_max_iter;
@@ -123,6 +128,16 @@ typedef struct {
     or_mg_params mg;
     int dissipation;    /* NSTempSolver: temperature source dissipation */
     double u_lin[3];    /* AdvectionSolver: constant transport velocity */
+    int turb_dynamic;   /* 1: DynamicSmagorinsky instead of ConstSmagorinsky */
+    /* concentration solvers (9 NSTempConSolver, 10 NSTempTurbConSolver) */
+    double gamma;       /* molecular diffusivity of the concentration */
+    int con_turb;       /* NSTempTurbConSolver: <turbulence include="Yes"> */
+    double schmidt;     /* Sc_T */
+    int con_dir[3];
+    int con_fct;        /* 1 Gauss, 2 Cube */
+    double con_gauss[8];
+    double con_tau;
+    double con_cube[7]; /* x, y, z start, x, y, z end, value */
 } or_sim_params;
 typedef struct or_sim or_sim;
 /* u_ic ... p_ic: full arrays whose inner cells carry the initial condition (may be NULL) */

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <malloc.h>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,7 @@
 #include "source/ExplicitEulerSource.h"
 #include "source/GaussFunction.h"
 #include "turbulence/ConstSmagorinsky.h"
+#include "turbulence/DynamicSmagorinsky.h"
 #include "utility/settings/Settings.h"
 
 namespace {
@@ -432,6 +434,22 @@ int mode_op(const std::string &xml, const std::string &op, const std::string &ou
         dump_field(outdir, "smg_ev", ev);
         return 0;
     }
+    if (op == "dyn_smagorinsky") {
+        Field ev(FieldType::NU, 0.0), u(FieldType::U, 0.0), v(FieldType::V, 0.0), w(FieldType::W, 0.0);
+        for (Field *f : {&u, &v, &w}) fill_random(*f, amp, 0.0);
+        dump_field(outdir, "dyn_u", u);
+        dump_field(outdir, "dyn_v", v);
+        dump_field(outdir, "dyn_w", w);
+        DynamicSmagorinsky smg;
+        for (int call = 0; call < 2; call++) {
+            smg.calc_turbulent_viscosity(ev, u, v, w, true);
+            dump_field(outdir, "dyn_ev" + std::to_string(call), ev);
+            // a second call on changed velocities reuses the work fields
+            for (Field *f : {&u, &v, &w})
+                for (size_t i = 0; i < f->get_size(); i++) f->data[i] *= 0.5;
+        }
+        return 0;
+    }
     if (op == "sources") {
         Field out(FieldType::T, 0.0), s(FieldType::T, 0.0);
         fill_random(out, 1.0, 0.0);
@@ -484,6 +502,12 @@ int main(int argc, char **argv) {
         return 2;
     }
     const std::string mode = argv[1];
+    // Fields the reference constructs without values (Field.cpp:12-26) and
+    // never writes in places (DynamicSmagorinsky's work fields outside the
+    // inner cells, its uw_f everywhere) read whatever the allocation holds:
+    // every allocation of 4 KiB or more becomes a fresh zero-filled mapping,
+    // so those cells read zero.
+    mallopt(M_MMAP_THRESHOLD, 4096);
     try {
         if (mode == "run" && argc >= 5) {
             const std::string ic = argc > 5 && std::string(argv[5]) != "-" ? argv[5] : "";

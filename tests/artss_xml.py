@@ -82,6 +82,7 @@ def parse(path_or_text):
     cfg["kappa"] = _f(phys, "kappa", 0.0)
     cfg["beta"] = _f(phys, "beta", 3.34e-3)
     cfg["g"] = _f(phys, "g", -9.81)
+    cfg["gamma"] = _f(phys, "gamma", 0.0)
 
     dom = _values(root.find("domain_parameters"))
     cfg["domain"] = dict(
@@ -91,7 +92,7 @@ def parse(path_or_text):
     if _yes(dom, "enable_computational_domain"):
         raise ValueError("computational sub-domains are not supported")
 
-    for name in ("advection", "diffusion", "turbulence", "source", "pressure", "temperature"):
+    for name in ("advection", "diffusion", "turbulence", "source", "pressure", "temperature", "concentration"):
         el = solver_el.find(name)
         if el is None:
             continue
@@ -131,6 +132,22 @@ def parse(path_or_text):
                                         ("HRR", "cp", "x0", "y0", "z0", "sigma_x", "sigma_y", "sigma_z")]
                 t["source"]["tau"] = float(src["tau"])
             cfg["temperature"] = t
+        elif name == "concentration":
+            c = {"advection": dict(el.find("advection").attrib), "diffusion": _diffusion(el.find("diffusion"))}
+            turb = el.find("turbulence")
+            c["turbulence"] = turb is not None and turb.attrib.get("include") == "Yes"
+            if c["turbulence"]:
+                c["schmidt"] = float(_values(turb)["Sc_T"])
+            src = _values(el.find("source"))
+            c["source"] = {"type": src["type"], "con_fct": src["con_fct"], "dir": axes(src["dir"])}
+            if src["con_fct"] == "Gauss":
+                c["source"]["gauss"] = [float(src[k]) for k in
+                                        ("HRR", "cp", "x0", "y0", "z0", "sigma_x", "sigma_y", "sigma_z")]
+                c["source"]["tau"] = float(src["tau"])
+            else:
+                c["source"]["cube"] = [float(src[a + "_start"]) for a in "xyz"] + \
+                    [float(src[a + "_end"]) for a in "xyz"] + [float(src["val"])]
+            cfg["concentration"] = c
 
     cfg["boundaries"] = [_boundary(b) for b in root.find("boundaries") if b.tag == "boundary"]
     obs = root.find("obstacles")

@@ -160,3 +160,33 @@ def uniform_ic(dom, value):
     T = np.zeros(dom.size(0))
     T[dom.list(0, 0).astype(np.int64)] = value
     return T
+
+
+def concentration_plume(solver="NSTempTurbConSolver", con_fct="Gauss", turbulent=True, gamma=1e-4, n=(16, 16, 16),
+                        levels=2, steps=5, dt=0.02, obstacles=(), diffusion="Jacobi", dissipation=False, **kw):
+    """The plume with a transported concentration (NSTempConSolver or
+    NSTempTurbConSolver): a Gauss source at the floor or a Cube of constant
+    value; concentration Dirichlet 0 on the left and right, Neumann 0
+    elsewhere."""
+    x = plume(n=n, levels=levels, steps=steps, dt=dt, obstacles=obstacles, diffusion=diffusion, **kw)
+    x = x.replace('<solver description="NSTempTurbSolver">', '<solver description="%s">' % solver)
+    x = x.replace('</physical_parameters>', '<gamma> {} </gamma></physical_parameters>'.format(repr(gamma)))
+    if dissipation:
+        x = x.replace('dissipation="No"', 'dissipation="Yes"')
+    if con_fct == "Gauss":
+        src = ('<source type="ExplicitEuler" dir="y" random="No" con_fct="Gauss"><HRR> 2.5 </HRR><cp> 1. </cp>'
+               '<x0> 0.7 </x0><y0> 0. </y0><z0> 0.3 </z0><sigma_x> 0.15 </sigma_x><sigma_y> 0.1 </sigma_y>'
+               '<sigma_z> 0.2 </sigma_z><tau> 2.0 </tau></source>')
+    else:
+        src = ('<source type="ExplicitEuler" dir="y" random="No" con_fct="Cube"><x_start> 0.25 </x_start>'
+               '<y_start> 0.0 </y_start><z_start> 0.3 </z_start><x_end> 0.6 </x_end><y_end> 0.25 </y_end>'
+               '<z_end> 0.7 </z_end><val> 3.5 </val></source>')
+    turb = '<turbulence include="Yes"><Sc_T> 0.7 </Sc_T></turbulence>' if turbulent else \
+        '<turbulence include="No"></turbulence>'
+    con = ('<concentration><advection type="SemiLagrangian" field="rho"></advection>' +
+           diffusion_xml(diffusion, "rho", 100, 1e-7, 1.0) + (turb if "Turb" in solver else "") + src +
+           '</concentration>')
+    x = x.replace('<solution available="No"></solution></solver>', con + '<solution available="No"></solution></solver>')
+    x = x.replace('</boundaries>', '<boundary field="rho" patch="left,right" type="dirichlet" value="0.0" />'
+                  '<boundary field="rho" patch="front,back,bottom,top" type="neumann" value="0.0" /></boundaries>')
+    return x

@@ -44,6 +44,10 @@ CASES = {
                                                  diffusion="Jacobi", smoother="Jacobi", obstacles=OBST_ONE,
                                                  hrr=3.0, source_x=1.0),
                          ops=[("explicit", 1e-2), ("explicit_turb", 1e-2)], lists=False),
+    "ops_dyn_smagorinsky": dict(xml=lambda: _dyn(cases.plume(n=(16, 8, 8), X=(0.0, 2.0), levels=2, steps=2, dt=0.01,
+                                                            diffusion="Jacobi", obstacles=OBST_ONE, hrr=3.0,
+                                                            source_x=1.0)),
+                                ops=[("dyn_smagorinsky", 1.0)], lists=False),
     "geo_adjacent": dict(xml=lambda: _bc_mixed(OBST_ADJ), ops=[("bc", 1.0)], lists=True),
     # whole-solver runs: state after steps 1, 2 and the last one
     "run_ns_tg": dict(xml=lambda: cases.ns_taylor_green(n=16, levels=2, steps=5, dt=2e-3, reference=True),
@@ -56,6 +60,8 @@ CASES = {
                                run=("1,2", "")),
     "run_nstemp_dissipation": dict(xml=lambda: _nstemp_dissipation(), run=("1,2", "")),
     "run_nsturb_tg": dict(xml=lambda: _tg_variant("NSTurbSolver", turbulence=True), run=("1,2", "taylor_green")),
+    "run_nsturb_dyn_tg": dict(xml=lambda: _dyn(_tg_variant("NSTurbSolver", turbulence=True)),
+                              run=("1,2", "taylor_green")),
     "run_diffusion_tg": dict(xml=lambda: _tg_variant("DiffusionSolver"), run=("1,2", "taylor_green")),
     "run_diffusion_turb_tg": dict(xml=lambda: _tg_variant("DiffusionTurbSolver", turbulence=True),
                                   run=("1,2", "taylor_green")),
@@ -103,6 +109,14 @@ def _tg_variant(solver, diffusion="Jacobi", turbulence=False):
         x = x.replace('<source type="ExplicitEuler"',
                       '<turbulence type="ConstSmagorinsky"><Cs> 0.2 </Cs></turbulence><source type="ExplicitEuler"', 1)
     return x
+
+
+def _dyn(x):
+    """the same input with the dynamic Smagorinsky model"""
+    y = x.replace('<turbulence type="ConstSmagorinsky"><Cs> 0.2 </Cs></turbulence>',
+                  '<turbulence type="DynamicSmagorinsky"></turbulence>')
+    assert y != x
+    return y
 
 
 def _nstemp_dissipation():

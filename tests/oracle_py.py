@@ -45,7 +45,10 @@ class SimParams(C.Structure):
                 ("diff_tol", C.c_double), ("diff_w", C.c_double), ("cs", C.c_double), ("prandtl", C.c_double),
                 ("has_force", C.c_int), ("force_dir", C.c_int * 3), ("T_ambient", C.c_double),
                 ("has_temp_source", C.c_int), ("temp_dir", C.c_int * 3), ("gauss", C.c_double * 8),
-                ("tau", C.c_double), ("mg", MGParams), ("dissipation", C.c_int), ("u_lin", C.c_double * 3)]
+                ("tau", C.c_double), ("mg", MGParams), ("dissipation", C.c_int), ("u_lin", C.c_double * 3),
+                ("turb_dynamic", C.c_int), ("gamma", C.c_double), ("con_turb", C.c_int), ("schmidt", C.c_double),
+                ("con_dir", C.c_int * 3), ("con_fct", C.c_int), ("con_gauss", C.c_double * 8), ("con_tau", C.c_double),
+                ("con_cube", C.c_double * 7)]
 
 
 _lib = None
@@ -79,6 +82,7 @@ def lib():
         L.or_divergence.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_projection.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp, dp]
         L.or_smagorinsky.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_double, C.c_int]
+        L.or_dyn_smagorinsky.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_double]
         L.or_add_source.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_int]
         L.or_buoyancy.argtypes = [C.c_void_p, dp, dp, dp, C.c_double, C.c_double]
         L.or_gauss_spatial.argtypes = [C.c_void_p, dp, dp]
@@ -214,6 +218,9 @@ class Domain:
     def smagorinsky(self, ev, u, v, w, cs, ft=6):
         self.L.or_smagorinsky(self.h, _p(ev), _p(u), _p(v), _p(w), cs, ft)
 
+    def dyn_smagorinsky(self, ev, u, v, w, nu):
+        self.L.or_dyn_smagorinsky(self.h, _p(ev), _p(u), _p(v), _p(w), nu)
+
     def add_source(self, out, s, dt, ft):
         self.L.or_add_source(self.h, _p(out), _p(s), dt, ft)
 
@@ -254,7 +261,8 @@ def sim_params(cfg):
     sp = SimParams()
     desc = cfg["solver"]
     kinds = {"NSSolver": 0, "NSTempTurbSolver": 1, "NSTempSolver": 3, "NSTurbSolver": 4, "DiffusionSolver": 5,
-             "AdvectionDiffusionSolver": 6, "DiffusionTurbSolver": 7, "AdvectionSolver": 8}
+             "AdvectionDiffusionSolver": 6, "DiffusionTurbSolver": 7, "AdvectionSolver": 8, "NSTempConSolver": 9,
+             "NSTempTurbConSolver": 10}
     if desc not in kinds:
         raise ValueError("the oracle has no driver for " + desc)
     sp.solver = kinds[desc]
@@ -268,6 +276,7 @@ def sim_params(cfg):
     sp.diff_max_iter, sp.diff_tol, sp.diff_w = dif.get("max_iter", 0), dif.get("tol_res", 0.0), dif.get("w", 0.0)
     if "turbulence" in cfg:
         sp.cs = cfg["turbulence"]["cs"]
+        sp.turb_dynamic = 1 if cfg["turbulence"]["type"] == "DynamicSmagorinsky" else 0
     src = cfg.get("source", {"force_fct": "Zero", "dir": []})
     sp.has_force = 1 if src["force_fct"] == "Buoyancy" else 0
     for a in src["dir"]:
@@ -288,6 +297,22 @@ def sim_params(cfg):
         sp.dissipation = 1 if ts.get("dissipation") else 0
     if "pressure" in cfg:
         sp.mg = mg_params(cfg)
+    c = cfg.get("concentration")
+    if c is not None:
+        sp.gamma = cfg["gamma"]
+        sp.con_turb = 1 if c["turbulence"] else 0
+        sp.schmidt = c.get("schmidt", 0.0)
+        for a in c["source"]["dir"]:
+            sp.con_dir[a] = 1
+        if c["source"]["con_fct"] == "Gauss":
+            sp.con_fct = 1
+            for i, x in enumerate(c["source"]["gauss"]):
+                sp.con_gauss[i] = x
+            sp.con_tau = c["source"]["tau"]
+        else:
+            sp.con_fct = 2
+            for i, x in enumerate(c["source"]["cube"]):
+                sp.con_cube[i] = x
     return sp
 
 

@@ -221,3 +221,46 @@ def test_vcycle_reproduces_reference_pressure_fixture():
     mg.pressure(p, rhs, cfg["dt"], cfg["dt"])
     perm = refxml.inner_first(z["dims"])
     assert refxml.same6(p[perm], z["refdat/p"]) == 0
+
+
+@pytest.mark.parametrize("solver, turbulent, adds", [("NSTempConSolver", False, True),
+                                                     ("NSTempTurbConSolver", True, True),
+                                                     ("NSTempTurbConSolver", False, False)])
+def test_concentration_cube_source_at_rest(solver, turbulent, adds):
+    """A fluid at rest (no heat release, uniform temperature, no thermal
+    diffusivity) with a Cube
+    concentration source and no molecular diffusivity: the concentration
+    grows by dt * value per step on exactly the cells whose centres lie in
+    the box and stays zero elsewhere, velocities stay zero.
+    NSTempTurbConSolver adds the source only for a turbulent concentration
+    (NSTempTurbConSolver.cpp:277-285)."""
+    import artss_xml
+    import cases
+    import oracle_py
+
+    x = cases.concentration_plume(solver, "Cube", turbulent, gamma=0.0, hrr=0.0, kappa=0.0, n=(16, 16, 16))
+    cfg = artss_xml.parse(x)
+    dom = oracle_py.Domain(cfg)
+    sim = oracle_py.Sim(dom, cfg, None, None, None, cases.uniform_ic(dom, float(cfg["ic"]["values"]["val"])))
+    info = dom.info(0)
+    Nx, Ny, Nz = info["N"]
+    h = info["h"]
+    start, end, val = (0.25, 0.0, 0.3), (0.6, 0.25, 0.7), 3.5
+    box = np.zeros((Nz, Ny, Nx), bool)
+    for k in range(1, Nz - 1):
+        for j in range(1, Ny - 1):
+            for i in range(1, Nx - 1):
+                c = ((i - 0.5) * h[0], (j - 0.5) * h[1], (k - 0.5) * h[2])
+                box[k, j, i] = all(start[a] <= c[a] <= end[a] for a in range(3))
+    assert box.sum() == 6 * 4 * 6
+    expect = 0.0
+    for step in range(1, 5):
+        sim.step(step * cfg["dt"])
+        expect = expect + cfg["dt"] * val if adds else 0.0
+        C = sim.field("C").reshape(Nz, Ny, Nx)
+        assert np.all(C[box] == expect), step
+        inner = np.zeros_like(box)
+        inner[1:-1, 1:-1, 1:-1] = True
+        assert np.all(C[inner & ~box] == 0.0), step
+        for f in "uvw":
+            assert np.all(sim.field(f) == 0.0), (f, step)

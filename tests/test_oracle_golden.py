@@ -14,13 +14,17 @@ Every oracle function is compared bit for bit (np.array_equal on float64):
   V-cycle MG (a4, a6-a11)       test_vcycle (Jacobi and CGS smoothers, Nt == 1 and Nt > 1)
   divergence, projection (a12)  test_divergence, test_projection
   ConstSmagorinsky (a13)        test_smagorinsky
+  DynamicSmagorinsky (8(f) 3)   test_dyn_smagorinsky, test_run_drivers[run_nsturb_dyn_tg]
   sources (a17)                 test_sources (add_source scalar/vector, buoyancy, Gauss)
   solver drivers (a18, a19)     test_run (NSSolver Taylor-Green, NSTempTurb plume Jacobi/CGS/explicit),
                                 test_run_drivers (the other solvers and the reference's initial conditions)
 
-The reference's committed tests/*/*_ref.dat files are not used: they do not
-reproduce with the reference's current sources (their initial conditions
-differ), so they pin nothing about this code.
+The reference's own fixture files (tests/*/*_ref.dat of the reference) list
+the inner cells before the ghost cells; in that order the current reference
+reproduces 11 of its 16 inputs to their six digits
+(tests/golden/make_reference_inputs.py).  The oracle's V-cycle is pinned to
+tests/pressure/p_ref.dat in test_oracle_closed_forms.py; the GPU path runs
+all of those inputs in tests/test_gpu_reference_inputs.py.
 """
 import math
 import os
@@ -207,7 +211,21 @@ def test_sources(ops_jacobi):
     same(st, z["sources/gauss_st"])
 
 
-@pytest.mark.parametrize("case", ["run_nstemp_dissipation", "run_nsturb_tg", "run_diffusion_tg",
+def test_dyn_smagorinsky():
+    """two calls on one model object (its work fields persist between calls)"""
+    z, cfg = load("ops_dyn_smagorinsky")
+    dom = O.Domain(cfg)
+    u, v, w = (z["dyn_smagorinsky/dyn_%s" % c].copy() for c in "uvw")
+    ev = dom.field()
+    for call in range(2):
+        dom.dyn_smagorinsky(ev, u, v, w, cfg["nu"])
+        same(ev, z["dyn_smagorinsky/dyn_ev%d" % call])
+        u *= 0.5
+        v *= 0.5
+        w *= 0.5
+
+
+@pytest.mark.parametrize("case", ["run_nstemp_dissipation", "run_nsturb_tg", "run_nsturb_dyn_tg", "run_diffusion_tg",
                                   "run_diffusion_turb_tg", "run_advection_diffusion_tg", "run_advection_gauss",
                                   "run_diffusion_expsinusprod", "run_diffusion_hat", "run_burgers_expsinussum",
                                   "run_ns_mcdermott", "run_ns_vortex", "run_ns_drift", "run_nstemp_layers"])
