@@ -4,28 +4,46 @@
 Workload: the buoyant-plume LES (NSTempTurbSolver: SL advection, colored
 Gauss-Seidel diffusion, ConstSmagorinsky, Boussinesq buoyancy, Gaussian heat
 source, geometric multigrid pressure) on 256x128x128 cells over 2x1x1 m with
-one box obstacle, from artss_amd/configs/plume_256x128x128.xml.  A step is
-one full time step over all inner cells; the pressure equation is solved
-until ||rhs - lap p||_2 <= 1e-8 every step.
+one box obstacle, from artss_amd/configs/plume_256x128x128.xml (1000 steps of
+dt = 5e-3).  A step is one full time step over all inner cells; the pressure
+equation is solved until ||rhs - lap p||_2 <= 1e-8 every step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--pressure mixed|gcr|vcycle]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--full-steps 1000]
 
-Prints one JSON line.  `value` is cell-updates/s = inner cells x K / wall
-time of the K timed steps of the accelerated path (default: GCR with a
-mixed-precision V-cycle), fields resident in HBM throughout.  `plain_fp64`
-runs the plain FP64 path (the reference's V-cycles on their own, to the same
-residual) from rest over the same steps, with its throughput, the speedup of
-`value` over it and the largest relative difference of every field between
-the two paths.  `roofline` is the dominant kernel's algorithmic bytes per
-launch over its mean launch time (HIP events on the solver's stream around
-every launch in a timing region) against the 8 TB/s HBM3E peak.
-`cpu_baseline` runs the reference's own serial build (oracle/_ref, compiled
-from the reference sources) for a bounded sample continuing from the same
-state on this host.
+Prints one JSON line.  `value` is cell-updates/s = fluid inner cells x K /
+wall time of K timed steps (after W untimed ones, from rest) of the default
+path, fields resident in HBM: FP64 GCR preconditioned by a mixed-precision
+V-cycle (FP32 smoothing, restriction and prolongation; FP64 coarse levels,
+residuals and norms), started from the pressure extrapolated in time.
+
+`legs` times every path over the same steps W+1..W+K, each from rest:
+  mixed_extrapolated  the default path (`value`)
+  mixed_learned       the same with the learned warm start: a BF16 U-Net
+                      trained during the warm-up steps on the (input, target)
+                      pairs the solver records, then used on every timed step
+  gcr_fp64            FP64 GCR with the FP64 V-cycle (the reference's)
+  plain_fp64          the reference's V-cycles on their own, to the same residual
+  reference_schedule  the plain FP64 path with the reference's own V-cycle
+                      schedule (n_cycle per step, no residual target): the
+                      work the reference's CPU build does per step
+Each leg reports its throughput, iterations, and the largest relative field
+difference to plain_fp64 after step W+K.
+
+`full_run` runs the whole 1000-step input from rest on the default path and
+on plain_fp64: whole-run and last-100-step throughput, and every field's
+largest relative difference between the two at the last step.
+
+`roofline` is the GCR update's algorithmic bytes per launch over its mean
+launch time (HIP events on the solver's stream around every launch in a
+separate timing region) against the 8 TB/s HBM3E peak; `traffic` and
+`step` come from the rocprofv3 summaries under profiles/.  `cpu_baseline`
+runs the reference's own serial build (oracle/_ref, compiled from the
+reference sources) for a bounded sample continuing from the GPU state.
 """
 import argparse
 import json
 import os
+import platform
 import subprocess
 import sys
 import tempfile
@@ -38,26 +56,26 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s, 256x128x128 plume LES, learned+mixed vs plain FP64 MG at res 1e-8"
 HBM_PEAK_GBS = 8000.0
+FIELDS = ("u", "v", "w", "p", "T")
 
-# algorithmic bytes per level-inner cell for one launch of a kernel (each
-# distinct array element read or written once, 8-byte doubles)
+# algorithmic bytes per cell for the kernels the live timer can name (each
+# distinct array element read or written once; artss_amd/roofline.py has the
+# model of every kernel)
 BYTES_PER_CELL = {
-    "jacobi_mg": 24,       # read x, b; write x'
-    "jacobi_mg32": 16,     # level 0 of the mixed V-cycle: read x (f32), b (f64 residual); write x' (f32)
-    "cgs_mg": 16,          # one color: read x (all), b (half); write half
-    "cgs_diffuse": 20,     # one color, turbulent: + eddy viscosity (half)
-    "jacobi_diffuse": 32,  # turbulent: read x, b, ev; write x'
-    "residuum": 24,        # read x, b; write r
-    "residual": 16,        # read x, b
-    "sl_advect": 40,       # read in, u, v, w; write out
-    "restrict": 10,        # per fine cell: read it (8); per coarse cell write residual and reset error (16/8)
-    "prolongate": 17,      # out of place: read fine error (8) and coarse (8/8), write (8)
+    "jacobi_mg": 24, "jacobi_mg32": 16, "cgs_mg": 16, "cgs_diffuse": 20, "jacobi_diffuse": 32, "residuum": 24,
+    "residual": 16, "sl_advect": 40, "restrict": 10, "prolongate": 17,
     # GCR update of the mixed path, per cell of the whole array: x, r (f64) read+write,
     # z (f32) read, kept direction z_prev, w_prev (f32) read+write
     "gcr_update": 52,
 }
-# kernels that run over the whole array (ghosts included) rather than the fluid cells
 WHOLE_ARRAY = {"gcr_update"}
+
+PATHS = {"mixed_extrapolated": "GCR in FP64 preconditioned by an FP32 V-cycle with FP64 coarse levels, "
+                               "extrapolated start",
+         "mixed_learned": "the same, started from the extrapolation plus the trained U-Net's correction",
+         "gcr_fp64": "FP64 GCR preconditioned by the FP64 V-cycle, extrapolated start",
+         "plain_fp64": "FP64, V-cycles alone (the reference's iteration) to the residual target",
+         "reference_schedule": "FP64, the reference's V-cycle schedule (n_cycle per step, no residual target)"}
 
 
 def parse():
@@ -67,39 +85,50 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default=os.path.join(ROOT, "artss_amd", "configs", "plume_256x128x128.xml"))
     ap.add_argument("--tolerance", type=float, default=1e-8, help="pressure residual target per step")
-    ap.add_argument("--reference-schedule", action="store_true",
-                    help="use the reference's V-cycle schedule instead of the residual target")
+    ap.add_argument("--path", choices=("mixed_extrapolated", "gcr_fp64", "plain_fp64"), default="mixed_extrapolated",
+                    help="path of the timed region (`value`); the others are legs")
+    ap.add_argument("--legs", default="mixed_learned,gcr_fp64,plain_fp64,reference_schedule",
+                    help="paths timed beside the default one over the same steps (comma list; '' for none)")
+    ap.add_argument("--full-steps", type=int, default=1000,
+                    help="steps of the whole-run comparison of the default path with plain_fp64 (0: skip)")
     ap.add_argument("--profile-kernel", default="gcr_update@0",
                     help="kernel@level timed for the roofline (the largest HBM-bound kernel of the path)")
     ap.add_argument("--timing-steps", type=int, default=3, help="steps of the kernel timing region")
-    ap.add_argument("--plain-steps", type=int, default=5,
-                    help="timed steps of the plain FP64 V-cycle path run beside the accelerated one (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the reference CPU sample (0: skip)")
     ap.add_argument("--mg-schedule", choices=("graph", "fused", "plain"), default="graph",
                     help="V-cycle launch schedule (bit-identical results)")
-    ap.add_argument("--pressure", choices=("vcycle", "gcr", "mixed"), default="mixed",
-                    help="residual-target iteration: V-cycles alone (reference) or V-cycle-preconditioned GCR")
-    ap.add_argument("--coarse-iterations", type=int, default=0,
-                    help="mixed: cap of the preconditioner's coarse Jacobi solve (0: the XML's max_iter)")
-    ap.add_argument("--sweeps", default="1,1",
-                    help="mixed: pre,post smoothing sweeps of the FP32 levels (0: the XML's effective count)")
-    ap.add_argument("--weight", type=float, default=0.8,
-                    help="mixed: Jacobi weight of those sweeps (0: the XML's)")
+    ap.add_argument("--sweeps", default="1,1", help="pre,post smoothing sweeps of the FP32 levels")
+    ap.add_argument("--weight", type=float, default=0.8, help="Jacobi weight of those sweeps")
     ap.add_argument("--coarse", choices=("direct", "tail"), default="direct",
-                    help="mixed: exact dense solve of the coarse levels, or their single-workgroup V-cycle")
+                    help="exact dense solve of the coarse levels where they have no obstacle, or their V-cycle")
     ap.add_argument("--tail-sweeps", type=int, default=3,
-                    help="mixed: n_relax of the coarse levels' single-workgroup kernel (0: the XML's)")
+                    help="n_relax of the preconditioner's coarse levels (single-workgroup kernel; 0: the XML's)")
     ap.add_argument("--tail-cells", type=int, default=4096,
                     help="levels of at most this many cells run in one single-workgroup kernel")
-    ap.add_argument("--warm", choices=("none", "extrapolate"), default="extrapolate",
-                    help="start of each step's pressure iteration: the previous p (the reference's start) or "
-                         "p extrapolated linearly in time")
+    ap.add_argument("--train-steps", type=int, default=150, help="Adam steps of the learned leg's network")
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the ensemble measurement (uneven plume boxes; 0: skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=5, help="timed steps of the ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=50, help="timed steps of the ensemble measurement")
     ap.add_argument("--ensemble-threads", type=int, default=8, help="host threads issuing the members' steps")
     ap.add_argument("--device", type=int, default=0)
     return ap.parse_args()
+
+
+def host_cpu():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count()
+    return {"model": model, "nproc": allowed, "logical_cpus": os.cpu_count()}
 
 
 def cpu_baseline(solver, config, first_step, steps, inner):
@@ -108,57 +137,175 @@ def cpu_baseline(solver, config, first_step, steps, inner):
     if steps <= 0 or not os.path.exists(harness):
         return None
     with tempfile.TemporaryDirectory() as tmp:
-        for f in ("u", "v", "w", "p", "T"):
+        for f in FIELDS:
             solver.field(f).tofile(os.path.join(tmp, f + ".f64"))
         r = subprocess.run([harness, "state", config, tmp, str(first_step), str(steps)], capture_output=True,
                            text=True, timeout=900)
+    cpu = host_cpu()
     if r.returncode != 0:
         return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                "sample": "failed: " + r.stderr.strip()[-200:]}
+                "sample": "failed: " + r.stderr.strip()[-200:], "host": cpu}
     secs = float(r.stdout.split("seconds=")[1].split()[0])
     return {"value": inner * steps / secs, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-            "sample": "%d steps (%d..%d) of the same workload continuing from the GPU state, reference "
-                      "serial build with its own V-cycle schedule, %.1f s" % (steps, first_step,
-                                                                              first_step + steps - 1, secs)}
+            "sample": "%d steps (%d..%d) of the same workload continuing from the GPU state, reference serial "
+                      "build (one thread) with its own V-cycle schedule (n_cycle per step), %.1f s; compare "
+                      "legs.reference_schedule, the GPU doing the same work" % (
+                          steps, first_step, first_step + steps - 1, secs),
+            "host": cpu}
 
 
-PATHS = {"vcycle": "FP64, V-cycles alone (the reference's iteration)",
-         "gcr": "FP64, V-cycle-preconditioned GCR",
-         "mixed": "GCR in FP64 preconditioned by an FP32 V-cycle with FP64 coarse levels"}
-
-
-def make_solver(artss_amd, args, pressure, warm="none"):
+def make_solver(artss_amd, args, leg, net=None):
     solver = artss_amd.Solver(args.config, device=args.device)
     solver.set_mg_schedule({"plain": 0, "fused": 1, "graph": 2}[args.mg_schedule], args.tail_cells)
-    if not args.reference_schedule:
-        solver.set_pressure_mode(True, args.tolerance, 500)
-        solver.set_pressure_method({"vcycle": 0, "gcr": 1, "mixed": 2}[pressure], args.coarse_iterations)
-        if pressure != "vcycle" and warm == "extrapolate":
-            solver.set_warm_start(1)
-        if pressure == "mixed":
-            pre, post = (int(v) for v in args.sweeps.split(","))
-            solver.set_mixed_sweeps(pre, post, args.weight)
-            solver.set_coarse_solve(args.coarse == "direct")
-            solver.set_mixed_tail(args.tail_sweeps)
+    if leg == "reference_schedule":
+        return solver
+    solver.set_pressure_mode(True, args.tolerance, 500)
+    method = {"mixed_extrapolated": 2, "mixed_learned": 2, "gcr_fp64": 1, "plain_fp64": 0}[leg]
+    solver.set_pressure_method(method)
+    if method == 2:
+        pre, post = (int(v) for v in args.sweeps.split(","))
+        solver.set_mixed_sweeps(pre, post, args.weight)
+        solver.set_coarse_solve(args.coarse == "direct")
+        solver.set_mixed_tail(args.tail_sweeps)
+    if method != 0:
+        solver.set_warm_start(1, record=leg == "mixed_learned")
     return solver
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the rocprofv3 PMC summary in
-    profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
-    counter reads for a copy kernel of known size), or None."""
+def profile_value(prefix, key):
+    """`key = value` lines that the rocprofv3 summaries under profiles/ state"""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None
     for name in sorted(os.listdir(d)):
-        if not name.startswith("pmc_"):
+        if not name.startswith(prefix):
             continue
         with open(os.path.join(d, name)) as fh:
             for line in fh:
-                line = line.strip().strip("`")
-                if line.startswith("traffic[%s] = " % kernel):
-                    return int(line.split("=")[1])
+                line = line.strip().strip("`").lstrip("- ")
+                if line.startswith(key + " = "):
+                    v = line.split("=", 1)[1].strip().split()[0]
+                    try:
+                        return float(v) if "." in v or "e" in v else int(v)
+                    except ValueError:
+                        return None
     return None
+
+
+class Run:
+    """Steps one solver from rest; timed(k) times k steps between syncs."""
+
+    def __init__(self, solver, dt):
+        self.solver, self.dt, self.step = solver, dt, 1
+
+    def timed(self, k, on_step=None):
+        dom = self.solver.domain
+        cyc, dif, res = [], [], []
+        dom.sync()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            self.solver.step(self.step * self.dt)
+            c = self.solver.counters()
+            cyc.append(c["vcycles"])
+            dif.append(c["diffusion"])
+            res.append(c["residual"])
+            if on_step:
+                on_step(self.step)
+            self.step += 1
+        dom.sync()
+        return time.perf_counter() - t0, cyc, dif, res
+
+
+def rel_errors(fields, ref):
+    return {f: float(np.abs(fields[f] - ref[f]).max() / max(np.abs(ref[f]).max(), 1e-300)) for f in FIELDS}
+
+
+def learned_warmup(artss_amd, run, args):
+    """warm-up of the learned leg: record the solver's (input, target) pairs,
+    train the U-Net on them (forward, backward and Adam on the GPU), then
+    switch the start to extrapolation plus the network's correction"""
+    s = run.solver
+    dom = s.domain
+    Nx, Ny, Nz = dom.info(0)["N"]
+    box = (Nz - 2, Ny - 2, Nx - 2)
+    xs, ts = [], []
+
+    def keep(step):
+        if step < 3:
+            return
+        x = artss_amd.Buffer(dom, np.zeros(box + (2,), np.float32))
+        t = artss_amd.Buffer(dom, np.zeros(box + (1,), np.float32))
+        if s.warm_sample(x, t):
+            xs.append(x)
+            ts.append(t)
+
+    run.timed(args.warmup, keep)
+    net = artss_amd.UNet(dom, seed=5)
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    losses = []
+    for it in range(args.train_steps if xs else 0):
+        j = int(rng.integers(len(xs)))
+        loss, _ = net.gradients(xs[j], ts[j], want_grads=False)
+        losses.append(loss)
+        net.adam_step(1e-3 if it < 0.75 * args.train_steps else 3e-4)
+    dom.sync()
+    train_s = time.perf_counter() - t0
+    for b in xs + ts:
+        b._release()
+    s.set_warm_start(2, net)
+    return net, {"pairs": len(xs), "adam_steps": len(losses), "train_s": train_s,
+                 "loss_first": losses[0] if losses else None,
+                 "loss_last10": float(np.mean(losses[-10:])) if losses else None}
+
+
+def leg_run(artss_amd, args, leg, dt, inner):
+    s = make_solver(artss_amd, args, leg)
+    run = Run(s, dt)
+    extra = {}
+    net = None
+    if leg == "mixed_learned":
+        net, extra["training"] = learned_warmup(artss_amd, run, args)
+    else:
+        run.timed(args.warmup)
+    first = run.step
+    secs, cyc, dif, res = run.timed(args.steps)
+    fields = {f: s.field(f) for f in FIELDS}
+    out = {"value": inner * args.steps / secs, "unit": "cell-updates/s", "ms_per_step": secs * 1e3 / args.steps,
+           "steps": "%d..%d" % (first, run.step - 1), "path": PATHS[leg],
+           "mean_iterations": float(np.mean(cyc)),
+           "max_final_residual": None if leg == "reference_schedule" else float(np.max(res))}
+    out.update(extra)
+    if net is not None:
+        net.close()
+    s.close()
+    return out, fields
+
+
+def full_run(artss_amd, args, dt, inner):
+    """the whole input from rest: default path and plain FP64"""
+    out = {"steps": args.full_steps}
+    fields = {}
+    for leg in ("mixed_extrapolated", "plain_fp64"):
+        s = make_solver(artss_amd, args, leg)
+        run = Run(s, dt)
+        head = args.full_steps - min(100, args.full_steps)
+        t_head, c_head, _, r_head = run.timed(head) if head else (0.0, [], [], [])
+        t_tail, c_tail, _, r_tail = run.timed(args.full_steps - head)
+        fields[leg] = {f: s.field(f) for f in FIELDS}
+        total = t_head + t_tail
+        out[leg] = {"value": inner * args.full_steps / total, "ms_per_step": total * 1e3 / args.full_steps,
+                    "last_100": {"value": inner * (args.full_steps - head) / t_tail,
+                                 "ms_per_step": t_tail * 1e3 / (args.full_steps - head),
+                                 "mean_iterations": float(np.mean(c_tail))},
+                    "mean_iterations": float(np.mean(c_head + c_tail)),
+                    "max_final_residual": float(np.max(r_head + r_tail))}
+        s.close()
+    out["speedup"] = out["mixed_extrapolated"]["value"] / out["plain_fp64"]["value"]
+    out["speedup_last_100"] = (out["mixed_extrapolated"]["last_100"]["value"] /
+                               out["plain_fp64"]["last_100"]["value"])
+    out["max_rel_field_difference_at_last_step"] = rel_errors(fields["mixed_extrapolated"], fields["plain_fp64"])
+    return out
 
 
 def ensemble_run(artss_amd, args):
@@ -181,13 +328,9 @@ def ensemble_run(artss_amd, args):
         ens = artss_amd.Ensemble(members, threads=threads)
         for n in range(1, 4):
             ens.step(n)
-        for m in members:
-            m.domain.sync()
         t0 = time.perf_counter()
         for n in range(4, 4 + args.ensemble_steps):
-            ens.step(n)
-        for m in members:
-            m.domain.sync()
+            ens.step(n)  # every member's stream is synchronised when step returns
         secs = time.perf_counter() - t0
         out["cells"] = cells
         out[label] = {"value": cells * args.ensemble_steps / secs, "unit": "cell-updates/s",
@@ -197,28 +340,6 @@ def ensemble_run(artss_amd, args):
             m.close()
     out["concurrency_gain"] = out["concurrent"]["value"] / out["sequential"]["value"]
     return out
-
-
-class Run:
-    """Steps one solver from rest; timed(k) times k steps between syncs."""
-
-    def __init__(self, solver, dt):
-        self.solver, self.dt, self.step = solver, dt, 1
-
-    def timed(self, k):
-        dom = self.solver.domain
-        cyc, dif, res = [], [], []
-        dom.sync()
-        t0 = time.perf_counter()
-        for _ in range(k):
-            self.solver.step(self.step * self.dt)
-            c = self.solver.counters()
-            cyc.append(c["vcycles"])
-            dif.append(c["diffusion"])
-            res.append(c["residual"])
-            self.step += 1
-        dom.sync()
-        return time.perf_counter() - t0, cyc, dif, res
 
 
 def main():
@@ -231,7 +352,7 @@ def main():
 
     with open(args.config) as fh:
         dt = float(fh.read().split("<dt>")[1].split("</dt>")[0])
-    solver = make_solver(artss_amd, args, args.pressure, args.warm)
+    solver = make_solver(artss_amd, args, args.path)
     dom = solver.domain
     info = dom.info(0)
     inner = info["inner_cells"]
@@ -242,8 +363,7 @@ def main():
     # timed region: K steps as they run in production (V-cycles replayed from HIP graphs)
     secs, cycles, diff_it, residuals = run.timed(args.steps)
     value = inner * args.steps / secs
-    after = run.step - 1
-    fields = {f: solver.field(f) for f in ("u", "v", "w", "p", "T")} if args.plain_steps > 0 else None
+    fields = {f: solver.field(f) for f in FIELDS}
 
     # kernel timing region: HIP events around every launch of the profiled
     # kernel on the solver's stream; the V-cycle then runs its launches
@@ -252,7 +372,6 @@ def main():
     dom.profile_arm(args.profile_kernel)
     t_secs, _, _, _ = run.timed(timing_steps)
     launches, kernel_ms = dom.profile_read()
-    # cells per thread along z the autotuner picked for the fused kernels of levels 0..3
     tuned = [artss_amd.tuned_cells_per_thread(dom.info(l)["size"]) for l in range(4)]
     kname, klevel = args.profile_kernel.split("@")
     kinfo = dom.info(int(klevel))
@@ -261,57 +380,52 @@ def main():
     achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9 if launches else None
     roofline = {"bound": "hbm", "kernel": args.profile_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                "traffic": measured_traffic(args.profile_kernel),
+                "traffic": profile_value("pmc_", "traffic[%s]" % args.profile_kernel),
                 "bytes_per_launch": per_launch_bytes, "launches": launches, "mean_launch_us": mean_ms * 1e3,
                 "timing_steps": timing_steps, "timing_region_ms_per_step": t_secs * 1e3 / timing_steps,
-                "share_of_timing_region": kernel_ms / (t_secs * 1e3)}
+                "share_of_timing_region": kernel_ms / (t_secs * 1e3),
+                "rocprof_mean_launch_us": profile_value("roofline_", "mean_us[gcr_update]"),
+                "step": {"kernel_frac": profile_value("roofline_", "kernel_frac"),
+                         "step_frac": profile_value("roofline_", "step_frac"),
+                         "source": "rocprofv3 kernel trace of this command with --steps 3 --warmup 2, summarised "
+                                   "by artss_amd/roofline.py (profiles/roofline_*.md)"}}
 
     # the CPU sample continues from the state after the GPU regions (same
     # workload, later steps)
     cpu = cpu_baseline(solver, args.config, run.step, args.cpu_steps, inner)
     solver.close()
 
-    # plain FP64 path (V-cycles alone to the same residual) from rest over the
-    # same steps: its throughput, and the field error of the accelerated path
-    plain = None
-    if args.plain_steps > 0 and args.pressure != "vcycle" and not args.reference_schedule:
-        ps = make_solver(artss_amd, args, "vcycle")
-        prun = Run(ps, dt)
-        prun.timed(args.warmup)
-        k_plain = min(args.plain_steps, args.steps)
-        p_secs, p_cyc, _, p_res = prun.timed(k_plain)
-        prun.timed(after - prun.step + 1)  # to the step the accelerated fields were taken at
-        errors = {}
-        for f, a in fields.items():
-            ref = ps.field(f)
-            errors[f] = float(np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300))
-        ps.close()
-        plain = {"value": inner * k_plain / p_secs, "unit": "cell-updates/s", "steps": k_plain,
-                 "ms_per_step": p_secs * 1e3 / k_plain, "mean_vcycles": float(np.mean(p_cyc)),
-                 "max_final_residual": float(np.max(p_res)), "path": PATHS["vcycle"],
-                 "speedup_of_value": value / (inner * k_plain / p_secs),
-                 "max_rel_field_error_at_step_%d" % after: errors}
+    legs = {args.path: {"value": value, "unit": "cell-updates/s", "ms_per_step": secs * 1e3 / args.steps,
+                        "steps": "%d..%d" % (first, first + args.steps - 1),
+                        "path": PATHS[args.path], "mean_iterations": float(np.mean(cycles)),
+                        "max_final_residual": float(np.max(residuals))}}
+    leg_fields = {args.path: fields}
+    for leg in [x for x in args.legs.split(",") if x and x != args.path]:
+        legs[leg], leg_fields[leg] = leg_run(artss_amd, args, leg, dt, inner)
+    if "plain_fp64" in legs:
+        ref = leg_fields["plain_fp64"]
+        for leg, f in leg_fields.items():
+            legs[leg]["speedup_over_plain_fp64"] = legs[leg]["value"] / legs["plain_fp64"]["value"]
+            legs[leg]["max_rel_field_difference_to_plain_fp64"] = rel_errors(f, ref)
 
+    full = full_run(artss_amd, args, dt, inner) if args.full_steps > 0 else None
     ensemble = ensemble_run(artss_amd, args) if args.ensemble_members > 0 else None
 
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.pressure != "mixed" else "f64 (pressure preconditioner f32)",
+        "dtype": "f64 (pressure preconditioner f32)" if args.path == "mixed_extrapolated" else "f64",
         "data": "synthetic: plume from rest, timed steps %d..%d" % (first, first + args.steps - 1),
         "config": {"workload": "plume_256x128x128", "grid": list(info["n"]), "inner_cells": inner,
-                   "path": PATHS[args.pressure], "mg_schedule": args.mg_schedule, "tail_cells": args.tail_cells,
-                   "warm_start": ("previous p" if args.warm == "none" or args.pressure == "vcycle" else
-                                  "p + (p - p_prev), linear extrapolation in time"),
-                   "coarse_iterations": args.coarse_iterations, "sweeps": args.sweeps, "weight": args.weight,
-                   "coarse": args.coarse, "tail_sweeps": args.tail_sweeps,
-                   "tuned_cells_per_thread": tuned,
-                   "pressure": ("reference V-cycle schedule" if args.reference_schedule else
-                                "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance),
-                   "mean_vcycles": float(np.mean(cycles)), "mean_diffusion_iterations": np.mean(
-                       diff_it, axis=0).tolist(), "max_final_residual": float(np.max(residuals))},
-        "plain_fp64": plain, "ensemble": ensemble,
+                   "path": PATHS[args.path], "mg_schedule": args.mg_schedule,
+                   "tail_cells": args.tail_cells, "sweeps": args.sweeps, "weight": args.weight,
+                   "coarse": args.coarse, "tail_sweeps": args.tail_sweeps, "tuned_cells_per_thread": tuned,
+                   "pressure": "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance,
+                   "mean_vcycles": float(np.mean(cycles)),
+                   "mean_diffusion_iterations": np.mean(diff_it, axis=0).tolist(),
+                   "max_final_residual": float(np.max(residuals))},
+        "legs": legs, "full_run": full, "ensemble": ensemble,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,0 +1,198 @@
+"""The five benchmark configurations at their full sizes on the GPU.
+
+C1  64^3 Taylor-Green NSSolver, 4-level V-cycle (configs/taylor_green_64.xml):
+    100 steps against the oracle, <= 1e-10 relative per checked step and
+    <= 1e-8 after step 100.
+C2  128^3 closed-box plume with the seeded obstacle, colored Gauss-Seidel
+    diffusion, 5-level V-cycle (configs/plume_128.xml), the reference's
+    V-cycle schedule: three steps against the oracle, same bounds.
+C3  256x128x128 plume (configs/plume_256x128x128.xml) to ||rhs - lap p||_2 <=
+    1e-8 every step: the mixed-precision path (FP32 smoothing, restriction and
+    prolongation inside FP64 GCR) and FP64 GCR against the plain FP64 path
+    (V-cycles alone): every step converged, every field within 1e-6 relative
+    after 25 steps.
+C4  the same with the learned warm start (BF16 U-Net trained on the run's own
+    samples): converged every step, fields within 1e-6 of the plain FP64 path.
+C5  64 members of uneven size (scenarios.ensemble_members(64, seed=2026)),
+    mixed precision with the extrapolated start, stepped together: every
+    member ends bit for bit where it ends when run alone.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import artss_amd
+import artss_xml
+import cases
+import oracle_py as O
+from artss_amd import scenarios
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "artss_amd", "configs")
+FIELDS = ("u", "v", "w", "p", "T")
+
+
+def _xml(name):
+    with open(os.path.join(CONFIGS, name)) as fh:
+        return fh.read()
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def test_c1_taylor_green_64_vs_oracle():
+    xml = _xml("taylor_green_64.xml")
+    cfg = artss_xml.parse(xml.replace('usr_fct="TaylorGreen"', 'usr_fct="Zero"'))
+    assert cfg["domain"]["nx"] == 64 and cfg["levels"] == 4
+    odom = O.Domain(cfg)
+    u, v, w = cases.taylor_green_ic(odom)
+    osim = O.Sim(odom, cfg, u, v, w, None)
+    s = artss_amd.Solver(xml)
+    worst = 0.0
+    for step in range(1, 101):
+        t = step * cfg["dt"]
+        s.step(t)
+        osim.step(t)
+        assert s.counters()["diffusion"][:3] == osim.counters()[:3], step
+        if step % 10 == 0 or step < 3:
+            for f in "uvwp":
+                e = _rel(s.field(f), osim.field(f))
+                worst = max(worst, e)
+                assert e <= 1e-10, (f, step, e)
+    print("C1 worst relative difference over 100 steps", worst)
+    assert worst <= 1e-8
+
+
+def test_c2_closed_plume_128_vs_oracle():
+    xml = _xml("plume_128.xml")
+    cfg = artss_xml.parse(xml)
+    assert cfg["domain"]["nx"] == 128 and cfg["levels"] == 5 and len(cfg["obstacles"]) == 1
+    odom = O.Domain(cfg)
+    osim = O.Sim(odom, cfg, None, None, None, cases.uniform_ic(odom, float(cfg["ic"]["values"]["val"])))
+    s = artss_amd.Solver(xml)
+    for step in (1, 2, 3):
+        t = step * cfg["dt"]
+        s.step(t)
+        osim.step(t)
+        c = s.counters()
+        assert c["diffusion"] == osim.counters()[:4], step
+        for f in FIELDS:
+            e = _rel(s.field(f), osim.field(f))
+            assert e <= 1e-10, (f, step, e)
+        print("C2 step", step, "counters", c)
+
+
+def _plume_solver(method, warm=0, net=None, record=False):
+    s = artss_amd.Solver(_xml("plume_256x128x128.xml"))
+    s.set_mg_schedule(2, 4096)
+    s.set_pressure_mode(True, 1e-8, 500)
+    s.set_pressure_method(method)
+    if method == 2:
+        s.set_mixed_sweeps(1, 1, 0.8)
+        s.set_coarse_solve(True)
+        s.set_mixed_tail(3)
+    if warm or record:
+        s.set_warm_start(warm, net, record=record)
+    return s
+
+
+def _steps(s, first, last, log):
+    for k in range(first, last + 1):
+        s.step(k * 5e-3)  # raises NotConverged when a step misses the target
+        c = s.counters()
+        assert 0 <= c["residual"] <= 1e-8, (k, c)
+        log.append(c["vcycles"])
+
+
+@pytest.fixture(scope="module")
+def plain_256():
+    """the plain FP64 path (V-cycles alone to 1e-8) after steps 25 and 35"""
+    s = _plume_solver(0)
+    log = []
+    _steps(s, 1, 25, log)
+    at25 = {f: s.field(f) for f in FIELDS}
+    _steps(s, 26, 35, log)
+    at35 = {f: s.field(f) for f in FIELDS}
+    print("plain FP64 V-cycles per step", log)
+    s.close()
+    return at25, at35
+
+
+@pytest.mark.parametrize("method", [2, 1], ids=["mixed", "gcr_fp64"])
+def test_c3_accelerated_paths_vs_plain_fp64(plain_256, method):
+    s = _plume_solver(method, warm=1)
+    log = []
+    _steps(s, 1, 25, log)
+    errs = {f: _rel(s.field(f), plain_256[0][f]) for f in FIELDS}
+    print("C3 method", method, "iterations", log, "relative field error at step 25", errs)
+    for f in FIELDS:
+        assert errs[f] <= 1e-6, (f, errs[f])
+
+
+def test_c4_learned_mixed_vs_plain_fp64(plain_256):
+    s = _plume_solver(2, warm=1, record=True)
+    dom = s.domain
+    Nx, Ny, Nz = dom.info(0)["N"]
+    box = (Nz - 2, Ny - 2, Nx - 2)
+    xs, ts, log = [], [], []
+    for k in range(1, 26):
+        _steps(s, k, k, log)
+        x = artss_amd.Buffer(dom, np.zeros(box + (2,), np.float32))
+        t = artss_amd.Buffer(dom, np.zeros(box + (1,), np.float32))
+        if k > 5 and s.warm_sample(x, t):
+            xs.append(x)
+            ts.append(t)
+    net = artss_amd.UNet(dom, seed=11)
+    rng = np.random.default_rng(1)
+    losses = []
+    for it in range(60):
+        j = int(rng.integers(len(xs)))
+        loss, _ = net.gradients(xs[j], ts[j], want_grads=False)
+        losses.append(loss)
+        net.adam_step(1e-3)
+    s.set_warm_start(2, net)
+    _steps(s, 26, 35, log)
+    errs = {f: _rel(s.field(f), plain_256[1][f]) for f in FIELDS}
+    print("C4 iterations", log, "loss", losses[0], "->", losses[-1], "relative field error at step 35", errs)
+    assert np.mean(losses[-10:]) < losses[0]
+    for f in FIELDS:
+        assert errs[f] <= 1e-6, (f, errs[f])
+
+
+def _member(xml):
+    m = artss_amd.Solver(xml)
+    m.set_mg_schedule(2, 4096)
+    m.set_pressure_mode(True, 1e-8, 500)
+    m.set_pressure_method(2)
+    m.set_warm_start(1)
+    return m
+
+
+def test_c5_ensemble_64_members_bit_identical_to_solo():
+    xmls = scenarios.ensemble_members(64, seed=2026)
+    assert len(xmls) == 64
+    members = [_member(x) for x in xmls]
+    ens = artss_amd.Ensemble(members, threads=8)
+    steps = 3
+    for n in range(1, steps + 1):
+        ens.step(n)
+    together = [{f: m.field(f) for f in FIELDS} for m in members]
+    counts = [m.counters() for m in members]
+    ens.close()
+    for m in members:
+        m.close()
+    cells = 0
+    for i, x in enumerate(xmls):
+        m = _member(x)
+        cells += m.domain.info(0)["inner_cells"]
+        dt = float(x.split("<dt>")[1].split("</dt>")[0])
+        for n in range(1, steps + 1):
+            m.step(n * dt)
+        assert m.counters() == counts[i], i
+        for f in FIELDS:
+            assert np.array_equal(m.field(f), together[i][f]), (i, f)
+        m.close()
+    print("C5", len(xmls), "members,", cells, "inner cells, bit-identical to solo runs after", steps, "steps")
