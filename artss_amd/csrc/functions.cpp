@@ -2,6 +2,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <random>
 
 #include "settings.h"
 
@@ -66,6 +67,25 @@ Grid grid_of(const Domain &d) {
     g.dz = L.h[2];
     g.inner = d.inner_list(0);
     return g;
+}
+
+// Functions::random (Functions.cpp:738-770) with UniformRandom
+// (randomField/UniformRandom.{h,cpp}): one draw per cell of the whole array
+// in index order, an integer uniform on [-steps, steps] with steps =
+// int(range / step_size), times step_size; the inner fluid cells get it
+// added (absolute) or are scaled by 1 + noise (relative).  The draws come
+// from the standard library's mt19937 and uniform_int_distribution, as the
+// reference's do, so a custom seed reproduces the reference's noise.
+void add_noise(std::vector<double> &f, const Grid &g, const RandomSettings &r) {
+    const int steps = static_cast<int>(r.range / r.step_size);
+    std::mt19937 mt(r.custom_seed ? static_cast<std::mt19937::result_type>(r.seed) : std::random_device()());
+    std::uniform_int_distribution<int> dist(-steps, steps);
+    std::vector<double> noise(g.n);
+    for (size_t i = 0; i < g.n; i++) noise[i] = dist(mt) * r.step_size;
+    for (uint64_t idx : g.inner) {
+        if (r.absolute) f[idx] += noise[idx];
+        else f[idx] *= (1 + noise[idx]);
+    }
 }
 
 // Functions.cpp:471-514 at t = 0
@@ -278,7 +298,9 @@ InitialFields initial_fields(const Domain &d) {
         // which leaves its loop range undefined
         throw config_error("initial condition '" + f + "' is not supported");
     }
-    if (s.ic_random) throw config_error("random initial conditions are not supported");
+    // SolverController.cpp:168-180: the noise goes on T for Uniform and
+    // LayersT; the reference ignores random="Yes" for the other functions
+    if (s.ic_random && (f == "Uniform" || f == "LayersT")) add_noise(r.arrays["T"], g, s.ic_noise);
     return r;
 }
 

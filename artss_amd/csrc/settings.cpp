@@ -70,6 +70,28 @@ bool opt_bool(const Map &m, const std::string &k, bool dflt) {
     return it == m.end() ? dflt : it->second == "Yes";
 }
 
+// Settings.cpp:166-186; Functions.cpp:740-748 rejects a relative range
+// outside [0, 1]
+RandomSettings random_settings(const xml::Element *e, const std::string &ctx) {
+    const Map v = values_of(e);
+    RandomSettings r;
+    r.absolute = req_bool(v, "absolute", ctx);
+    r.custom_seed = req_bool(v, "custom_seed", ctx);
+    auto seed = v.find("seed");
+    if (r.custom_seed && seed == v.end()) throw config_error("Value seed is for " + ctx + " required.");
+    if (seed != v.end()) {
+        errno = 0;
+        char *end = nullptr;
+        r.seed = std::strtoull(seed->second.c_str(), &end, 10);
+        if (end == seed->second.c_str() || errno == ERANGE) throw config_error("Value seed is for " + ctx + " required.");
+    }
+    r.step_size = req_bool(v, "custom_steps", ctx) ? req_real(v, "step_size", ctx) : opt_real(v, "step_size", 1.0);
+    r.range = req_real(v, "range", ctx);
+    if (!r.absolute && (r.range > 1 || r.range < 0))
+        throw config_error("range for relative noise has to be between [0:1]");
+    return r;
+}
+
 std::vector<std::string> split(const std::string &s, char d) {
     std::vector<std::string> out;
     size_t p = 0;
@@ -348,6 +370,10 @@ Settings parse_settings(const std::string &xml_text) {
     Map iv = values_of(section(root.get(), "initial_conditions", ""));
     s.ic = req_str(iv, "usr_fct", "initial_conditions");
     s.ic_random = req_bool(iv, "random", "initial_conditions");
+    if (s.ic_random)
+        s.ic_noise = random_settings(section(section(root.get(), "initial_conditions", ""), "random",
+                                             "initial_conditions"),
+                                     "initial_conditions/random");
     s.ic_values = iv;
     return s;
 }

@@ -70,6 +70,15 @@ struct ConcentrationSettings {
     CubeSettings cube;
 };
 
+// noise parameters (Settings.cpp:166-186 parse_random_parameters)
+struct RandomSettings {
+    bool absolute = false;     // added (Yes) or relative, x (1 + noise) (No)
+    bool custom_seed = false;  // else a nondeterministic seed
+    unsigned long long seed = 0;
+    double step_size = 1.0;
+    double range = 0.0;
+};
+
 struct Settings {
     std::string solver;  // solver description, e.g. "NSTempTurbSolver"
     double t_end = 0, dt = 0, nu = 0, kappa = 0, beta = 3.34e-3, g = -9.81, rhoa = 1, gamma = 0;
@@ -112,6 +121,7 @@ struct Settings {
     std::string ic;  // initial_conditions usr_fct
     std::map<std::string, std::string> ic_values;
     bool ic_random = false;
+    RandomSettings ic_noise;  // with ic_random
 
     int levels = 0;  // multigrid levels of the domain (main.cpp:64-68)
 };

@@ -131,7 +131,6 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
     if (s.has_temperature && s.dissipation && m_kind != 3 && !con)
         throw config_error("dissipation is part of NSTempSolver and the concentration solvers only "
                            "(NSTempSolver.cpp:181-185, NSTempConSolver.cpp, NSTempTurbConSolver.cpp)");
-    if (s.ic_random) throw config_error("random initial conditions are not supported");
 
     m_fields = std::make_unique<FieldController>(d, con);
     // the solver constructors (SolverSelection::set_*): NSSolver.cpp:17-36,

@@ -309,6 +309,8 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
         d.timed("residuum32", level, [&] {
             if (b64) k::residuum_faces<float, double>(s, d.level(level), V, m_res0f[level], m_err1f[level], b64);
             else k::residuum_faces<float, float>(s, d.level(level), V, m_res0f[level], m_err1f[level], b32);
+            // the restriction below reads the ghost layer: fill it
+            if (restrict_reads_ghost(level)) k::fill_bc<float>(s, d.level(level), V, m_res0f[level]);
         });
         d.timed("restrict32", level, [&] {
             if (level + 1 < f && fused_first) {
@@ -670,6 +672,17 @@ bool VCycleMG::fast_level(int level) const {
     return !m_cgs && effective > 0 && m_domain.bc_idempotent(level, P);
 }
 
+// Restrict (VCycleMG.cpp:404-421) averages fine cells 2i-1 and 2i for the
+// coarse cells i = 1..n_c.  Where 2 n_c > n_f, i.e. along an axis of one
+// cell (n_c = n_f = 1: two-dimensional inputs), it reads the fine ghost
+// layer, so the fill of the fine residuum must then be applied in full.
+bool VCycleMG::restrict_reads_ghost(int level) const {
+    const LevelDesc &fine = m_domain.level(level), &coarse = m_domain.level(level + 1);
+    for (int a = 0; a < 3; a++)
+        if (2 * coarse.n[a] > fine.n[a]) return true;
+    return false;
+}
+
 // Smoothing of err1[level] whose initial iterate (with the fill applied)
 // is in tmp[level]: sweep 1 tmp -> err1 with the copy folded in, then
 // alternating, the last one into err1.
@@ -741,7 +754,8 @@ void VCycleMG::smooth_fused(int level, const double *b) {
 //  - the fill of residuum1 after Restrict (:248) is dropped: only inner cells
 //    of residuum1 are ever read;
 //  - for a simple fill, the ghost part of the fill of residuum0 (:245) is
-//    dropped (Restrict reads no ghost), Restrict and the reset of the coarse
+//    dropped where Restrict reads no ghost (every axis of the level has at
+//    least two cells; see restrict_reads_ghost), Restrict and the reset of the coarse
 //    error share one kernel, Prolongate + fill + correction (:266-270) become
 //    one kernel adding into error1 (every cell the fill of error0 writes is
 //    overwritten by the fill at the start of the next Smooth/Solve before it
@@ -766,7 +780,7 @@ void VCycleMG::vcycle_fused(Field &out, const Field &b) {
         // (all values are zero below level 0) maps to zero: fills elided
         if (fast_level(level)) smooth_fast(level, rhs);
         else smooth_fused(level, rhs);
-        if (d.bc_idempotent(level, P)) {
+        if (d.bc_idempotent(level, P) && !restrict_reads_ghost(level)) {
             d.timed("residuum", level, [&] {
                 k::residuum_faces(d.stream(), d.level(level), d.bc_view(level, P), m_res0[level]->data,
                                   m_err1[level]->data, rhs);

@@ -96,6 +96,8 @@ class VCycleMG : public IPressure {
     void smooth_fused(int level, const double *b);
     void smooth_fast(int level, const double *b);
     bool fast_level(int level) const;
+    // Restrict from `level` reads the fine ghost layer (an axis of one cell)
+    bool restrict_reads_ghost(int level) const;
     void bc(double *f, int level);
     void build_tail();
     void smooth(Field &out, Field &tmp, const Field &b, int level);

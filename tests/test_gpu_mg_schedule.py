@@ -9,8 +9,9 @@ reference by the golden and oracle tests in test_gpu_parity.py.
 Cases cover both smoothers, odd and even n_relax (the Jacobi smoother's
 buffer swaps), periodic / Neumann / Dirichlet pressure boundaries with a
 non-zero Dirichlet value, no obstacles, one obstacle and three obstacles of
-which two touch (a boundary fill that is not idempotent), and every split
-between multi-workgroup and single-workgroup levels.
+which two touch (a boundary fill that is not idempotent), two-dimensional
+inputs (one cell along z), and every split between multi-workgroup and
+single-workgroup levels.
 """
 import zlib
 
@@ -19,12 +20,18 @@ import pytest
 
 import artss_amd
 import cases
+import refxml
 
 pytestmark = pytest.mark.gpu
 
 OBST3 = [(0.25, 0.5, 0.0, 0.375, 0.5, 0.75),
          (0.5, 0.75, 0.25, 0.5, 0.25, 0.5),
          (1.25, 1.5, 0.5, 0.75, 0.25, 0.75)]
+
+
+def _ref_xml(name):
+    """one of the reference's own inputs (kept in its fixture)"""
+    return bytes(refxml.load(name)["xml"]).decode()
 
 
 def _with(xml, n_relax=None, top_p_value=None):
@@ -53,6 +60,11 @@ CASES = {
     "plume_obst3_large": lambda: cases.plume(n=(128, 64, 128), X=(0.0, 2.0), levels=4, steps=2, dt=0.01,
                                              obstacles=OBST3, hrr=3.0, source_x=1.0, open_top=True),
     "tg_periodic_relax1": lambda: _with(cases.ns_taylor_green(n=32, levels=2, steps=2), n_relax=1),
+    # two-dimensional inputs (nz = 1): the restriction reads the fine ghost
+    # layer along z, so the residuum's fill must not be dropped; 40 -> 5 -> 2
+    # along x and y (odd level sizes)
+    "cavity_2d": lambda: _ref_xml("ns_cavity"),
+    "vortex_2d_periodic": lambda: _ref_xml("ns_vortex"),
 }
 
 
