@@ -79,6 +79,7 @@ def lib():
         "artss_mg_pressure": (C.c_int, [vp, vp, vp, C.c_double, ip]),
         "artss_mg_vcycle": (C.c_int, [vp, vp, vp]),
         "artss_mg_residual": (C.c_int, [vp, vp, vp, dp]),
+        "artss_mg_residual_static": (C.c_int, [vp, vp, vp, dp]),
         "artss_mg_set_schedule": (C.c_int, [vp, C.c_int, C.c_longlong]),
         "artss_mg_tail_info": (C.c_int, [vp, ip, ip]),
         "artss_solver_create": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp)]),
@@ -352,6 +353,13 @@ class Multigrid:
     def residual(self, p, rhs):
         r = C.c_double()
         _check(self.dom.L.artss_mg_residual(self.h, p.ptr, rhs.ptr, C.byref(r)))
+        return r.value
+
+    def residual_static(self, p, rhs):
+        """residual() with its partial sums in the library's fixed device
+        scratch array: no device allocation (calls overlapping on one GPU race)"""
+        r = C.c_double()
+        _check(self.dom.L.artss_mg_residual_static(self.h, p.ptr, rhs.ptr, C.byref(r)))
         return r.value
 
     def set_schedule(self, mode=2, tail_cells=4096):

@@ -19,7 +19,7 @@ ARCH = os.environ.get("ARTSS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["kernels.hip", "mg_tail.hip", "krylov_kernels.hip", "krylov.cpp", "unet_kernels.hip", "unet.cpp",
            "domain.cpp", "operators.cpp", "dyn_smagorinsky.cpp", "vcycle.cpp", "solver.cpp", "functions.cpp", "ensemble.cpp", "settings.cpp", "xml.cpp",
-           "capi.cpp"]
+           "residual_static.hip", "capi.cpp"]
 
 # -ffp-contract=off keeps every FP64 expression in the reference's operation
 # order (no fused multiply-add), which the bit-exact parity tests rely on.

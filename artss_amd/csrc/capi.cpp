@@ -8,6 +8,7 @@
 #include <string>
 
 #include "ensemble.h"
+#include "residual_static.h"
 #include "solver.h"
 #include "unet.h"
 
@@ -461,6 +462,14 @@ int artss_mg_residual(artss_mg *mg, const double *p, const double *rhs, double *
         Domain &D = dom(mg->d);
         Field fp(D, P, 0, const_cast<double *>(p)), fb(D, P, 0, const_cast<double *>(rhs));
         *residual = mg->mg->residual(fp, fb);
+        return ARTSS_OK;
+    });
+}
+
+int artss_mg_residual_static(artss_mg *mg, const double *p, const double *rhs, double *residual) {
+    return guard([&] {
+        require(mg && p && rhs && residual, "null argument");
+        *residual = residual_static(dom(mg->d), p, rhs);
         return ARTSS_OK;
     });
 }

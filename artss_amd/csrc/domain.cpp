@@ -1,4 +1,5 @@
 #include "domain.h"
+#include "residual_static.h"
 
 #include <algorithm>
 #include <array>
@@ -135,6 +136,7 @@ Domain::Domain(const Settings &s, int device) : m_settings(s), m_device(device) 
     ARTSS_HIP(hipMalloc(&m_partial, sizeof(double) * static_cast<size_t>(m_partial_cap)));
     ARTSS_HIP(hipMalloc(&m_sum_dev, sizeof(double)));
     ARTSS_HIP(hipHostMalloc(&m_sum_host, sizeof(double), hipHostMallocDefault));
+    static_scratch_setup(device);
 }
 
 Domain::~Domain() {

@@ -121,6 +121,16 @@ int artss_mg_pressure(artss_mg *mg, double *p, const double *rhs, double t, int 
 /* one VCycleMG::VCycleMultigrid (:215) and the residual norm of :186-195 */
 int artss_mg_vcycle(artss_mg *mg, double *p, const double *rhs);
 int artss_mg_residual(artss_mg *mg, const double *p, const double *rhs, double *residual);
+/* artss_mg_residual without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The partial sums go to a fixed file-scope device array (1 MiB per GPU, held
+ * until the process ends) instead of the domain's buffers: levels whose
+ * residual kernel has more than 131,072 workgroups (64x4 threads each, e.g.
+ * beyond 512x256x256 inner cells) are rejected with ARTSS_ERR_ARGUMENT.
+ * All calls on one GPU share that array, so calls that overlap on one GPU
+ * race.  Blocking; the same value as artss_mg_residual, bit for bit. */
+int artss_mg_residual_static(artss_mg *mg, const double *p, const double *rhs, double *residual);
 /* Launch schedule of the V-cycle (no reference counterpart: the reference has
  * one).  mode 0: one launch per reference operator; mode 1: launches that
  * cannot change the result folded or dropped, and the levels whose arrays
