@@ -97,6 +97,8 @@ def lib():
         "artss_solver_set_mixed_sweeps": (C.c_int, [vp, C.c_int, C.c_int, C.c_double]),
         "artss_solver_set_coarse_solve": (C.c_int, [vp, C.c_int]),
         "artss_solver_set_mixed_tail": (C.c_int, [vp, C.c_int]),
+        "artss_solver_set_coarse_tolerance": (C.c_int, [vp, C.c_double]),
+        "artss_solver_fallbacks": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
         "artss_tuned_cells_per_thread": (C.c_int, [C.c_longlong]),
         "artss_solver_snapshot": (C.c_int, [vp, C.c_double, ip]),
         "artss_solver_restore": (C.c_int, [vp, C.c_int, dp]),
@@ -445,8 +447,8 @@ class Solver:
 
     def set_pressure_method(self, method, coarse_max_iter=0):
         """0: V-cycles on their own; 1: V-cycle-preconditioned GCR; 2: GCR with an FP32/FP64
-        mixed-precision V-cycle (residual target only); coarse_max_iter > 0 caps the
-        preconditioner's coarse solve in method 2."""
+        mixed-precision V-cycle (residual target only); coarse_max_iter > 0 sets the
+        sweeps of the GCR preconditioner's coarse solve (0: five)."""
         _check(self.L.artss_solver_set_pressure_method(self.h, int(method), int(coarse_max_iter)))
 
     def set_warm_start(self, mode, net=None, record=False):
@@ -500,6 +502,19 @@ class Solver:
     def set_mixed_tail(self, n_relax=0):
         """method 2: n_relax of the preconditioner's coarse-level kernel (0: the XML's)"""
         _check(self.L.artss_solver_set_mixed_tail(self.h, int(n_relax)))
+
+    def fallbacks(self):
+        """steps whose GCR pressure solve stalled and was finished by V-cycles"""
+        n = C.c_longlong()
+        _check(self.L.artss_solver_fallbacks(self.h, C.byref(n)))
+        return n.value
+
+    def set_coarse_tolerance(self, coarse_tol=-1.0):
+        """methods 1, 2: stopping test of the preconditioner's coarse solve: 0 the
+        XML's absolute tolerance, 0 < coarse_tol < 1 a reduction relative to its
+        start, < 0 (default) a fixed number of sweeps (the coarse_max_iter of
+        set_pressure_method, 0: five)"""
+        _check(self.L.artss_solver_set_coarse_tolerance(self.h, float(coarse_tol)))
 
     def set_mg_schedule(self, mode=2, tail_cells=4096):
         _check(self.L.artss_solver_set_mg_schedule(self.h, int(mode), int(tail_cells)))

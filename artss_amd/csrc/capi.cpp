@@ -712,6 +712,24 @@ int artss_solver_set_mixed_tail(artss_solver *s, int n_relax) {
     });
 }
 
+int artss_solver_fallbacks(artss_solver *s, long long *steps) {
+    return guard([&] {
+        require(s && steps, "null argument");
+        *steps = s->solver->fallbacks();
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_set_coarse_tolerance(artss_solver *s, double coarse_tol) {
+    return guard([&] {
+        require(s && std::isfinite(coarse_tol) && coarse_tol < 1.0, "bad argument");
+        PressureMode m = s->solver->pressure_mode();
+        m.coarse_tol = coarse_tol;
+        s->solver->set_pressure_mode(m);
+        return ARTSS_OK;
+    });
+}
+
 int artss_solver_set_coarse_solve(artss_solver *s, int direct) {
     return guard([&] {
         require(s && (direct == 0 || direct == 1), "bad argument");

@@ -21,11 +21,11 @@ namespace artss {
 class GcrPressure {
   public:
     // mixed: the preconditioner's V-cycle runs its upper levels in FP32
-    // (VCycleMG::vcycle_mixed); the iteration, residuals and norms stay FP64
-    // coarse_max_iter (mixed only, > 0): cap of the preconditioner's coarse
-    // Jacobi solve; the preconditioner need not solve its coarse problem to
-    // the reference's tolerance because GCR corrects what it misses
-    GcrPressure(Domain &d, VCycleMG &preconditioner, bool mixed = false, int coarse_max_iter = 0);
+    // (VCycleMG::vcycle_mixed); the iteration, residuals and norms stay FP64.
+    // The preconditioner's coarse solve is set on the V-cycle
+    // (VCycleMG::set_precond_coarse); it need not solve its coarse problem
+    // to the reference's tolerance because GCR corrects what it misses.
+    GcrPressure(Domain &d, VCycleMG &preconditioner, bool mixed = false);
     ~GcrPressure();
     GcrPressure(const GcrPressure &) = delete;
     GcrPressure &operator=(const GcrPressure &) = delete;
@@ -42,18 +42,27 @@ class GcrPressure {
     double initial_residual() const { return m_initial; }
     // the last solve ended with its true residual <= tol
     bool converged() const { return m_converged; }
+    // the last solve stopped early because its residual stopped falling:
+    // kStallIterations iterations without a fall to a quarter of the best
+    // residual seen (a residual the preconditioner cannot reduce)
+    bool stagnated() const { return m_stagnated; }
+    static constexpr int kStallIterations = 40;
 
   private:
-    void iteration(Field &p);          // enqueue one GCR iteration
+    void iteration(Field &p, int parity);  // enqueue one GCR iteration
     void launch_iteration(Field &p, const Field &b, int count = 1);
     double true_residual(Field &p, const Field &b);
 
     Domain &m_domain;
     VCycleMG &m_pre;
     bool m_mixed;
-    int m_coarse_max_iter;
-    Field m_r, m_z, m_zp, m_wp;
-    float *m_zp32 = nullptr, *m_wp32 = nullptr;  // kept direction of the mixed iteration
+    Field m_r, m_z;
+    // the kept direction, in two buffers that alternate between iterations
+    // (the update reads one with a stencil and writes the other); FP64 for
+    // the FP64 iteration, FP32 for the mixed one
+    Field m_zp0, m_zp1;
+    float *m_zp32[2] = {nullptr, nullptr};
+    int m_parity = 0;                  // buffer holding the kept direction
     double *m_partial = nullptr;       // 4 sums x blocks
     double *m_coef = nullptr;          // device: beta, 1/|w'|, alpha
     double *m_sum = nullptr;           // device scalar
@@ -63,7 +72,9 @@ class GcrPressure {
     int m_predicted = 0;               // iterations enqueued at once in the next solve
     double m_initial = -1.0;
     bool m_converged = false;
-    hipGraphExec_t m_graph = nullptr, m_graph2 = nullptr;  // one and two iterations
+    bool m_stagnated = false;
+    // captured iterations: [count - 1][parity of the kept direction's buffer]
+    hipGraphExec_t m_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     double *m_graph_p = nullptr;
     const double *m_graph_b = nullptr;
 };

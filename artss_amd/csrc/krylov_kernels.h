@@ -25,17 +25,18 @@ int blocks_dots(const LevelDesc &L);   // ... of lap_dots
 
 // r = b - lap(x) on inner cells (x must carry its boundary fill); 1 sum: (r, r)
 void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x, const double *b, double *partial);
-// w = lap(z) on inner cells (not stored); 4 sums: (w,w), (w,wp), (r,w), (r,wp)
-template <typename Tz, typename Tw>
-void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tw *wp, const double *r, double *partial);
+// w = lap(z), wp = lap(zp) on inner cells (not stored); 4 sums: (w,w), (w,wp), (r,w), (r,wp)
+template <typename Tz>
+void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tz *zp, const double *r, double *partial);
 // GCR coefficients from the 4 sums: coef[0] beta, coef[1] 1/|w - beta wp|, coef[2] alpha
 // (gate: nothing when closed; nullptr: always)
 void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef, const GcrGate *gate);
-// the GCR step over the whole array (w = lap(z) recomputed); 1 sum: (r, r).
-// The kept direction (zp, wp) is stored as Tw.
-template <typename Tz, typename Tw>
-void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, Tw *zp,
-                Tw *wp, double *partial, const GcrGate *gate);
+// the GCR step over the whole array (w = lap(z) and wp = lap(zp)
+// recomputed); 1 sum: (r, r).  The new kept direction goes to zp_out, a
+// buffer other than zp.
+template <typename Tz>
+void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z,
+                const Tz *zp, Tz *zp_out, double *partial, const GcrGate *gate);
 // out[0] = fixed-order sum of n partials
 void sum(hipStream_t s, const double *partial, int n, double *out);
 // the same sum into gate->rr, counting the iteration and closing the gate

@@ -3,7 +3,8 @@
 // GCR with one previous direction, right-preconditioned by a V-cycle: per
 // iteration w = A z (A: the discrete Laplacian with the linear part of the
 // boundary fill), four dot products, one coefficient kernel and one fused
-// update.  FP64 throughout; the preconditioned direction z may be FP32.
+// update.  FP64 arithmetic throughout; the preconditioned direction z and
+// the kept direction zp may be stored in FP32.
 // Dot products are per-workgroup partial sums added by one workgroup in a
 // fixed order, so runs are deterministic.
 #include <cmath>
@@ -102,15 +103,16 @@ __global__ void k_residual_vec(LevelDesc L, double *__restrict__ r, const double
     write_partials<1>(v, partial);
 }
 
-// w = lap(z) on inner cells (not stored: the update recomputes it);
-// partials (w,w), (w,wp), (r,w), (r,wp).  A thread sums kDotsZ planes
-// before the four workgroup reductions, which cost more than the loads.
+// w = lap(z) and wp = lap(zp) on inner cells (neither is stored: the update
+// recomputes both); partials (w,w), (w,wp), (r,w), (r,wp).  A thread sums
+// kDotsZ planes before the four workgroup reductions, which cost more than
+// the loads.
 constexpr int kDotsZ = 4;
 inline dim3 dots_grid(const LevelDesc &L) {
     return dim3((L.n[0] + BX - 1) / BX, (L.n[1] + BY - 1) / BY, (L.n[2] + kDotsZ - 1) / kDotsZ);
 }
-template <typename Tz, typename Tw>
-__global__ void k_lap_dots(LevelDesc L, const Tz *__restrict__ z, const Tw *__restrict__ wp,
+template <typename Tz>
+__global__ void k_lap_dots(LevelDesc L, const Tz *__restrict__ z, const Tz *__restrict__ zp,
                            const double *__restrict__ r, double *__restrict__ partial) {
     int bx, by, bz;
     xcd_block(bx, by, bz);
@@ -124,7 +126,7 @@ __global__ void k_lap_dots(LevelDesc L, const Tz *__restrict__ z, const Tw *__re
             if (in_obstacle(L, i, j, k)) continue;
             const long long idx = dev::lin(L, i, j, k);
             const double x = lap(L, z, idx);
-            const double p = wp[idx], rr = r[idx];
+            const double p = lap(L, zp, idx), rr = r[idx];
             v[0] += x * x;
             v[1] += x * p;
             v[2] += rr * x;
@@ -167,13 +169,17 @@ __global__ void k_gcr_coef(const double *__restrict__ partial, int blocks, doubl
     }
 }
 
-// w = lap(z) on inner cells, 0 elsewhere; z' = (z - beta zp)/|w'|,
-// w' = (w - beta wp)/|w'|; x += alpha z'; r -= alpha w'; zp = z', wp = w'
-// over the whole array; partial (r, r)
-template <typename Tz, typename Tw>
+// w = lap(z), wp = lap(zp) on inner cells, 0 elsewhere; z' = (z - beta zp)/|w'|,
+// w' = (w - beta wp)/|w'|; x += alpha z'; r -= alpha w'; zp_out = z' over
+// the whole array; partial (r, r).  wp is the Laplacian of the stored zp,
+// never a stored copy of the previous w': the correction of r is then the
+// Laplacian of the correction of x whatever precision zp is kept in, so
+// the recursive residual follows the true one.  zp_out is a different
+// buffer from zp (the stencil reads zp's neighbours).
+template <typename Tz>
 __global__ void k_gcr_update(LevelDesc L, const double *__restrict__ coef, double *__restrict__ x,
-                             double *__restrict__ r, const Tz *__restrict__ z, Tw *__restrict__ zp,
-                             Tw *__restrict__ wp, double *__restrict__ partial, const GcrGate *__restrict__ gate) {
+                             double *__restrict__ r, const Tz *__restrict__ z, const Tz *__restrict__ zp,
+                             Tz *__restrict__ zp_out, double *__restrict__ partial, const GcrGate *__restrict__ gate) {
     if (gate != nullptr && gate->done) return;  // uniform over the grid
     int bx, by, bz;
     xcd_block(bx, by, bz);
@@ -190,13 +196,13 @@ __global__ void k_gcr_update(LevelDesc L, const double *__restrict__ coef, doubl
             const bool inner = i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] &&
                                !in_obstacle(L, i, j, k);
             const double w = inner ? lap(L, z, idx) : 0.0;
+            const double wp = inner ? lap(L, zp, idx) : 0.0;
             const double zn = (double(z[idx]) - beta * double(zp[idx])) * inv;
-            const double wn = (w - beta * double(wp[idx])) * inv;
+            const double wn = (w - beta * wp) * inv;
             x[idx] += alpha * zn;
             const double rn = r[idx] - alpha * wn;
             r[idx] = rn;
-            zp[idx] = static_cast<Tw>(zn);
-            wp[idx] = static_cast<Tw>(wn);
+            zp_out[idx] = static_cast<Tz>(zn);
             v[0] += rn * rn;
         }
     }
@@ -346,26 +352,25 @@ int blocks_full(const LevelDesc &L) {
 void residual_vec(hipStream_t s, const LevelDesc &L, double *r, const double *x, const double *b, double *partial) {
     k_residual_vec<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, r, x, b, partial);
 }
-template <typename Tz, typename Tw>
-void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tw *wp, const double *r, double *partial) {
-    k_lap_dots<Tz, Tw><<<dots_grid(L), dim3(BX, BY), 0, s>>>(L, z, wp, r, partial);
+template <typename Tz>
+void lap_dots(hipStream_t s, const LevelDesc &L, const Tz *z, const Tz *zp, const double *r, double *partial) {
+    k_lap_dots<Tz><<<dots_grid(L), dim3(BX, BY), 0, s>>>(L, z, zp, r, partial);
 }
-template void lap_dots<double, double>(hipStream_t, const LevelDesc &, const double *, const double *,
-                                       const double *, double *);
-template void lap_dots<float, float>(hipStream_t, const LevelDesc &, const float *, const float *, const double *,
-                                     double *);
+template void lap_dots<double>(hipStream_t, const LevelDesc &, const double *, const double *, const double *,
+                               double *);
+template void lap_dots<float>(hipStream_t, const LevelDesc &, const float *, const float *, const double *, double *);
 void gcr_coef(hipStream_t s, const double *partial, int blocks, double *coef, const GcrGate *gate) {
     k_gcr_coef<<<1, 1024, 0, s>>>(partial, blocks, coef, gate);
 }
-template <typename Tz, typename Tw>
-void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z, Tw *zp,
-                Tw *wp, double *partial, const GcrGate *gate) {
-    k_gcr_update<Tz, Tw><<<update_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, wp, partial, gate);
+template <typename Tz>
+void gcr_update(hipStream_t s, const LevelDesc &L, const double *coef, double *x, double *r, const Tz *z,
+                const Tz *zp, Tz *zp_out, double *partial, const GcrGate *gate) {
+    k_gcr_update<Tz><<<update_grid(L), dim3(BX, BY), 0, s>>>(L, coef, x, r, z, zp, zp_out, partial, gate);
 }
-template void gcr_update<double, double>(hipStream_t, const LevelDesc &, const double *, double *, double *,
-                                         const double *, double *, double *, double *, const GcrGate *);
-template void gcr_update<float, float>(hipStream_t, const LevelDesc &, const double *, double *, double *,
-                                       const float *, float *, float *, double *, const GcrGate *);
+template void gcr_update<double>(hipStream_t, const LevelDesc &, const double *, double *, double *, const double *,
+                                 const double *, double *, double *, const GcrGate *);
+template void gcr_update<float>(hipStream_t, const LevelDesc &, const double *, double *, double *, const float *,
+                                const float *, float *, double *, const GcrGate *);
 void sum(hipStream_t s, const double *partial, int n, double *out) { k_sum<<<1, 1024, 0, s>>>(partial, n, out); }
 void sum_gate(hipStream_t s, const double *partial, int n, GcrGate *gate) {
     k_sum_gate<<<1, 1024, 0, s>>>(partial, n, gate);

@@ -249,18 +249,30 @@ __device__ void t_smooth_jacobi(const TailLevel &T, const TailPlan &P, double *o
     }
 }
 
+// The coarse solve's stopping test: coarse_tol == 0, the reference's
+// res > tol (absolute); coarse_tol > 0, res > coarse_tol * (the residual of
+// the starting iterate), a reduction independent of the right-hand side's
+// scale; coarse_tol < 0, no test (max_iter sweeps, no residual computed).
+__device__ double solve_target(const TailLevel &T, const TailPlan &P, double coarse_tol, const double *x,
+                               const double *b, double *red) {
+    if (coarse_tol == 0.0) return P.tol;
+    if (coarse_tol < 0.0) return -1.0;
+    return coarse_tol * t_residual(T, x, b, red);
+}
+
 // VCycleMG.cpp:725-798
 __device__ void t_solve_jacobi(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b,
-                               double *red) {
+                               double *red, double coarse_tol) {
     t_bc(T, out);
     t_copy(tmp, out, static_cast<int>(T.L.size));
     double *o = out, *s = tmp;
     int it = 0;
     double res = 10000.;
-    while (res > P.tol && it < P.max_iter) {
+    const double target = solve_target(T, P, coarse_tol, o, b, red);
+    while (res > target && it < P.max_iter) {
         t_jacobi(T, P, o, s, b);
         t_bc(T, o);
-        res = t_residual(T, o, b, red);
+        if (coarse_tol >= 0.0) res = t_residual(T, o, b, red);
         double *x = o;
         o = s;
         s = x;
@@ -281,15 +293,17 @@ __device__ void t_smooth_cgs(const TailLevel &T, const TailPlan &P, double *out,
 }
 
 // VCycleMG.cpp:617-683
-__device__ void t_solve_cgs(const TailLevel &T, const TailPlan &P, double *out, const double *b, double *red) {
+__device__ void t_solve_cgs(const TailLevel &T, const TailPlan &P, double *out, const double *b, double *red,
+                            double coarse_tol) {
     t_bc(T, out);
     int it = 0;
     double res = 10000.;
-    while (res > P.tol && it < P.max_iter) {
+    const double target = solve_target(T, P, coarse_tol, out, b, red);
+    while (res > target && it < P.max_iter) {
         t_cgs_color(T, P, 0, out, b);
         t_cgs_color(T, P, 1, out, b);
         t_bc(T, out);
-        res = t_residual(T, out, b, red);
+        if (coarse_tol >= 0.0) res = t_residual(T, out, b, red);
         it++;
     }
 }
@@ -303,17 +317,18 @@ __device__ void t_smooth(const TailLevel &T, const TailPlan &P, double *out, dou
 
 // VCycleMG::Solve :562-581
 __device__ void t_solve(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b,
-                        double *red) {
+                        double *red, double coarse_tol) {
     if (P.solve_smooths) {
         t_smooth(T, P, out, tmp, b);
         return;
     }
-    if (P.cgs) t_solve_cgs(T, P, out, b, red);
-    else t_solve_jacobi(T, P, out, tmp, b, red);
+    if (P.cgs) t_solve_cgs(T, P, out, b, red, coarse_tol);
+    else t_solve_jacobi(T, P, out, tmp, b, red, coarse_tol);
 }
 
 __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__restrict__ plan, const double *b_first,
-                                                           int max_iter, int n_relax, double w, int skip_dead) {
+                                                           int max_iter, int n_relax, double w, int skip_dead,
+                                                           double coarse_tol) {
     __shared__ TailPlan P;
     __shared__ double red[kTailThreads / 64 + 1];
     extern __shared__ double lds[];
@@ -375,7 +390,7 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const TailPlan *__rest
             t_bc(T, T.err0);
             t_add(T.err1, T.err0, static_cast<int>(T.L.size));
         }
-        if (level == Lm) t_solve(T, P, T.err1, T.tmp, rhs, red);
+        if (level == Lm) t_solve(T, P, T.err1, T.tmp, rhs, red, coarse_tol);
         else t_smooth(T, P, T.err1, T.tmp, rhs);
     }
     if (P.lv[0].lds_off[0] >= 0) t_copy(P.io_err1, P.lv[0].err1, static_cast<int>(P.lv[0].L.size));
@@ -405,8 +420,9 @@ void mg_tail_reserve_lds(int lds_bytes) {
 }
 
 void mg_tail(hipStream_t s, const TailPlan *plan, const double *b_first, int lds_bytes, int max_iter, int n_relax,
-             double w, bool skip_dead) {
-    k_mg_tail<<<1, kTailThreads, lds_bytes, s>>>(plan, b_first, max_iter, n_relax, w, skip_dead ? 1 : 0);
+             double w, bool skip_dead, double coarse_tol) {
+    k_mg_tail<<<1, kTailThreads, lds_bytes, s>>>(plan, b_first, max_iter, n_relax, w, skip_dead ? 1 : 0,
+                                                 coarse_tol);
 }
 
 }  // namespace artss

@@ -353,12 +353,13 @@ void Solver::pressure_step(double t) {
         if (!m_gcr || m_gcr_mixed != mixed || m_gcr_coarse != m_pmode.coarse_max_iter ||
             m_gcr_pre != m_pmode.mixed_pre || m_gcr_post != m_pmode.mixed_post ||
             m_gcr_weight != m_pmode.mixed_weight || m_gcr_direct != m_pmode.coarse_direct ||
-            m_gcr_tail != m_pmode.mixed_tail) {
+            m_gcr_tail != m_pmode.mixed_tail || m_gcr_coarse_tol != m_pmode.coarse_tol) {
             m_gcr.reset();
             m_pres->set_mixed_sweeps(m_pmode.mixed_pre, m_pmode.mixed_post, m_pmode.mixed_weight);
             m_pres->set_mixed_tail(m_pmode.mixed_tail);
+            m_pres->set_precond_coarse(m_pmode.coarse_tol, m_pmode.coarse_max_iter);
             if (m_gcr_direct != m_pmode.coarse_direct || !m_gcr_built) m_pres->set_direct_coarse(m_pmode.coarse_direct);
-            m_gcr = std::make_unique<GcrPressure>(*m_domain, *m_pres, mixed, m_pmode.coarse_max_iter);
+            m_gcr = std::make_unique<GcrPressure>(*m_domain, *m_pres, mixed);
             m_gcr_mixed = mixed;
             m_gcr_coarse = m_pmode.coarse_max_iter;
             m_gcr_pre = m_pmode.mixed_pre;
@@ -366,6 +367,7 @@ void Solver::pressure_step(double t) {
             m_gcr_weight = m_pmode.mixed_weight;
             m_gcr_direct = m_pmode.coarse_direct;
             m_gcr_tail = m_pmode.mixed_tail;
+            m_gcr_coarse_tol = m_pmode.coarse_tol;
             m_gcr_built = true;
         }
         const bool warm = m_warm.mode != 0 || m_warm.record;
@@ -375,18 +377,35 @@ void Solver::pressure_step(double t) {
         m_counters.residual = r;
         m_counters.initial_residual = m_gcr->initial_residual();
         m_counters.converged = m_gcr->converged();
+        if (!m_counters.converged && m_counters.vcycles < m_pmode.max_cycles) {
+            // GCR stalled: the reference's V-cycle iteration finishes the
+            // solve from where it stopped, within the same cycle budget
+            m_fallbacks++;
+            int cycles = m_counters.vcycles;
+            vcycles_to_tolerance(cycles, r);
+            m_counters.vcycles = cycles;
+            m_counters.residual = r;
+            m_counters.converged = r <= m_pmode.tolerance;
+        }
         if (warm) warm_end();
         return;
     }
-    double r = m_pres->residual(f.p, f.rhs);
     int cycles = 0;
+    double r = m_pres->residual(f.p, f.rhs);
+    vcycles_to_tolerance(cycles, r);
+    m_counters.vcycles = cycles;
+    m_counters.residual = r;
+    m_counters.converged = r <= m_pmode.tolerance;
+}
+
+// V-cycles on p until ||rhs - lap p||_2 <= tolerance or the cycle budget
+// is spent; `cycles` counts on from its value, `r` holds p's residual
+void Solver::vcycles_to_tolerance(int &cycles, double &r) {
+    FieldController &f = *m_fields;
     while (r > m_pmode.tolerance && cycles < m_pmode.max_cycles) {
         r = m_pres->vcycle_residual(f.p, f.rhs);
         cycles++;
     }
-    m_counters.vcycles = cycles;
-    m_counters.residual = r;
-    m_counters.converged = r <= m_pmode.tolerance;
 }
 
 // NSSolver.cpp:50-124

@@ -60,11 +60,17 @@ struct PressureMode {
     // iteration), 1 V-cycles as the preconditioner of GCR (krylov.h),
     // 2 as 1 with the V-cycle's upper levels in FP32
     int method = 0;
-    int coarse_max_iter = 0;  // method 2: cap of the preconditioner's coarse solve (0: the XML's)
+    // methods 1, 2: sweep cap of the preconditioner's coarse solve (0: five
+    // sweeps with a fixed count, else the XML's max_iter)
+    int coarse_max_iter = 0;
     int mixed_pre = 0, mixed_post = 0;  // method 2: smoothing sweeps above the coarse levels (0: the XML's)
     double mixed_weight = 0.0;          // method 2: their Jacobi weight (0: the XML's)
     bool coarse_direct = true;          // method 2: dense exact coarse solve where it applies (VCycleMG)
     int mixed_tail = 0;                 // method 2: n_relax of the coarse levels' kernel (0: the XML's)
+    // methods 1, 2: stopping test of the preconditioner's coarse solve: 0
+    // the XML's absolute tolerance, > 0 a reduction relative to its starting
+    // residual, < 0 (default) a fixed number of sweeps (coarse_max_iter)
+    double coarse_tol = -1.0;
 };
 
 // Where the residual-target GCR pressure iteration starts (methods 1, 2),
@@ -95,6 +101,8 @@ class Solver {
     // SolverController::solver_do_step + update_sources + FieldController::update_data
     void step(double t);
     const StepCounters &counters() const { return m_counters; }
+    // steps whose GCR pressure solve stalled and was finished by V-cycles
+    long long fallbacks() const { return m_fallbacks; }
     void set_pressure_mode(const PressureMode &m) { m_pmode = m; }
     const PressureMode &pressure_mode() const { return m_pmode; }
     VCycleMG &pressure_solver() { return *m_pres; }
@@ -144,6 +152,7 @@ class Solver {
     void do_step_concentration();
     void do_step_transport();
     void pressure_step(double t);
+    void vcycles_to_tolerance(int &cycles, double &r);
     void update_sources(double t);
     void set_up_fields();
 
@@ -159,6 +168,8 @@ class Solver {
     int m_gcr_coarse = 0, m_gcr_pre = 0, m_gcr_post = 0;
     double m_gcr_weight = 0.0;
     bool m_gcr_direct = true, m_gcr_built = false;
+    long long m_fallbacks = 0;
+    double m_gcr_coarse_tol = 0.0;
     int m_gcr_tail = 0;
     std::unique_ptr<ExplicitEulerSource> m_src_vel, m_src_temp;
     std::unique_ptr<GaussFunction> m_gauss;

@@ -283,6 +283,13 @@ void VCycleMG::set_mixed_tail(int n_relax) {
     m_mixed_tail = n_relax;
 }
 
+void VCycleMG::set_precond_coarse(double coarse_tol, int max_iter) {
+    if (max_iter < 0) throw config_error("the coarse sweep cap must not be negative");
+    drop_graphs();
+    m_pc_tol = coarse_tol;
+    m_pc_iter = max_iter;
+}
+
 void VCycleMG::set_mixed_sweeps(int pre, int post, double weight) {
     if (pre < 0 || post < 0) throw config_error("sweep counts must not be negative");
     if (!(weight >= 0.0 && weight < 2.0)) throw config_error("the Jacobi weight must lie in [0, 2)");
@@ -292,7 +299,7 @@ void VCycleMG::set_mixed_sweeps(int pre, int post, double weight) {
     m_mixed_weight = weight;
 }
 
-void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
+void VCycleMG::vcycle_mixed(const double *r) {
     if (!mixed_supported()) throw std::logic_error("mixed-precision V-cycle needs the fused fast path and the tail");
     alloc_mixed();
     Domain &d = m_domain;
@@ -337,8 +344,8 @@ void VCycleMG::vcycle_mixed(const double *r, int coarse_max_iter) {
         });
     } else {
         d.timed("mg_tail", f, [&] {
-            mg_tail(s, m_plan_dev, m_res1[f]->data, m_tail_lds_bytes, coarse_max_iter, m_mixed_tail,
-                    m_mixed_tail > 0 ? m_mixed_weight : 0.0, true);
+            mg_tail(s, m_plan_dev, m_res1[f]->data, m_tail_lds_bytes, precond_coarse_iter(), m_mixed_tail,
+                    m_mixed_tail > 0 ? m_mixed_weight : 0.0, true, m_pc_tol);
         });
     }
     for (int level = f; level > 0; --level) {
@@ -643,10 +650,20 @@ void VCycleMG::vcycle(Field &out, const Field &b) {
     else vcycle_plain(out, b);
 }
 
+// The V-cycle as GCR's preconditioner, launched eagerly (GcrPressure
+// captures it in its own graph); the coarse kernel takes the
+// preconditioner's coarse-solve settings (set_precond_coarse).
 void VCycleMG::vcycle_eager(Field &out, const Field &b) {
-    if (m_s.n_level == 0) solve(out, *m_tmp[0], b, 0);
-    else if (m_fused) vcycle_fused(out, b);
-    else vcycle_plain(out, b);
+    m_precond_call = true;
+    try {
+        if (m_s.n_level == 0) solve(out, *m_tmp[0], b, 0);
+        else if (m_fused) vcycle_fused(out, b);
+        else vcycle_plain(out, b);
+    } catch (...) {
+        m_precond_call = false;
+        throw;
+    }
+    m_precond_call = false;
 }
 
 // Boundary fill of a pressure-type field on `level`: one launch when the
@@ -798,7 +815,11 @@ void VCycleMG::vcycle_fused(Field &out, const Field &b) {
     }
     if (f < L) {
         d.timed("mg_tail", f, [&] {
-            mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes);
+            if (m_precond_call)
+                mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes,
+                        precond_coarse_iter(), 0, 0.0, false, m_pc_tol);
+            else
+                mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes);
         });
     }
     for (int level = f; level > 0; --level) {

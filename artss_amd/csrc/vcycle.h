@@ -59,8 +59,17 @@ class VCycleMG : public IPressure {
     // n_relax of the single-workgroup kernel in the mixed V-cycle (0: the
     // XML's); when set, its sweeps use the mixed Jacobi weight as well
     void set_mixed_tail(int n_relax);
-    // coarse_max_iter > 0 caps the coarse solve's Jacobi iterations
-    void vcycle_mixed(const double *r, int coarse_max_iter = 0);
+    // The coarse solve of the single-workgroup kernel when the V-cycle is a
+    // preconditioner (vcycle_mixed, vcycle_eager): coarse_tol as mg_tail's
+    // (0 the XML's absolute tolerance, > 0 a reduction relative to its
+    // start, < 0 a fixed number of sweeps), max_iter > 0 its sweep cap (0:
+    // kPrecondCoarseSweeps for a fixed count, else the XML's).  A fixed
+    // count keeps the preconditioner one linear operator; with the XML's
+    // absolute tolerance it does fewer sweeps as GCR's residual shrinks,
+    // which can stall GCR.
+    static constexpr int kPrecondCoarseSweeps = 5;
+    void set_precond_coarse(double coarse_tol, int max_iter);
+    void vcycle_mixed(const double *r);
     float *mixed_result() const { return m_err1f.empty() ? nullptr : m_err1f[0]; }
     // The coarse part of the mixed V-cycle (the levels of the single-workgroup
     // kernel) replaced by the exact solve on its first level, x = K^-1 dsign b
@@ -125,6 +134,12 @@ class VCycleMG : public IPressure {
     int m_mixed_pre = 0, m_mixed_post = 0;  // sweeps of the mixed V-cycle (0: the XML's)
     double m_mixed_weight = 0.0;            // their Jacobi weight (0: the XML's)
     int m_mixed_tail = 0;                   // n_relax of the tail kernel in the mixed V-cycle (0: the XML's)
+    double m_pc_tol = -1.0;                 // preconditioner coarse solve: stopping test
+    int m_pc_iter = 0;                      // ... and sweep cap (0: default)
+    bool m_precond_call = false;            // inside vcycle_eager
+    int precond_coarse_iter() const {
+        return m_pc_iter > 0 ? m_pc_iter : (m_pc_tol < 0.0 ? kPrecondCoarseSweeps : 0);
+    }
     void alloc_mixed();
     void free_mixed();
     void smooth_fast_f(int level, const double *b64, const float *b32, int sweeps, bool from_zero, bool first_done);

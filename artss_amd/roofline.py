@@ -141,11 +141,10 @@ def bytes_of(kernel, args, gx, gy, gz, L, turbulent=True, tail_cells=4096):
     whole = L.size(level)
     if kernel == "k_gcr_update":
         t = size_of(args[0]) if args else 8
-        return (32 + t + 4 * t) * whole, "x, r read+write, z read, kept z, w read+write (%d B), whole array" % t
+        return (32 + 3 * t) * whole, "x, r read+write, z and kept z read, new kept z written (%d B), whole array" % t
     if kernel == "k_lap_dots":
         tz = size_of(args[0]) if args else 8
-        tw = size_of(args[1]) if len(args) > 1 else 8
-        return (tz + tw + 8) * cells, "read z, kept w, r"
+        return (2 * tz + 8) * cells, "read z, kept z, r"
     if kernel == "k_jacobi_bc":
         t = size_of(args[0])
         tb = size_of(args[1])

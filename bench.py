@@ -8,13 +8,14 @@ one box obstacle, from artss_amd/configs/plume_256x128x128.xml (1000 steps of
 dt = 5e-3).  A step is one full time step over all inner cells; the pressure
 equation is solved until ||rhs - lap p||_2 <= 1e-8 every step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--full-steps 1000]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Prints one JSON line.  `value` is cell-updates/s = fluid inner cells x K /
 wall time of K timed steps (after W untimed ones, from rest) of the default
 path, fields resident in HBM: FP64 GCR preconditioned by a mixed-precision
 V-cycle (FP32 smoothing, restriction and prolongation; FP64 coarse levels,
-residuals and norms), started from the pressure extrapolated in time.
+residuals and norms), started from the pressure extrapolated in time.  The
+defaults (W = 10, K = 990) time steps 11..1000, the whole input.
 
 `legs` times every path over the same steps W+1..W+K, each from rest:
   mixed_extrapolated  the default path (`value`)
@@ -26,12 +27,9 @@ residuals and norms), started from the pressure extrapolated in time.
   reference_schedule  the plain FP64 path with the reference's own V-cycle
                       schedule (n_cycle per step, no residual target): the
                       work the reference's CPU build does per step
-Each leg reports its throughput, iterations, and the largest relative field
+Each leg reports its throughput, iterations, the steps whose GCR solve was
+finished by V-cycles (`fallbacks`), and the largest relative field
 difference to plain_fp64 after step W+K.
-
-`full_run` runs the whole 1000-step input from rest on the default path and
-on plain_fp64: whole-run and last-100-step throughput, and every field's
-largest relative difference between the two at the last step.
 
 `roofline` is the GCR update's algorithmic bytes per launch over its mean
 launch time (HIP events on the solver's stream around every launch in a
@@ -65,8 +63,8 @@ BYTES_PER_CELL = {
     "jacobi_mg": 24, "jacobi_mg32": 16, "cgs_mg": 16, "cgs_diffuse": 20, "jacobi_diffuse": 32, "residuum": 24,
     "residual": 16, "sl_advect": 40, "restrict": 10, "prolongate": 17,
     # GCR update of the mixed path, per cell of the whole array: x, r (f64) read+write,
-    # z (f32) read, kept direction z_prev, w_prev (f32) read+write
-    "gcr_update": 52,
+    # z and the kept direction (f32) read, the new kept direction (f32) written
+    "gcr_update": 44,
 }
 WHOLE_ARRAY = {"gcr_update"}
 
@@ -81,7 +79,7 @@ PATHS = {"mixed_extrapolated": "GCR in FP64 preconditioned by an FP32 V-cycle wi
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=990)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default=os.path.join(ROOT, "artss_amd", "configs", "plume_256x128x128.xml"))
     ap.add_argument("--tolerance", type=float, default=1e-8, help="pressure residual target per step")
@@ -89,8 +87,6 @@ def parse():
                     help="path of the timed region (`value`); the others are legs")
     ap.add_argument("--legs", default="mixed_learned,gcr_fp64,plain_fp64,reference_schedule",
                     help="paths timed beside the default one over the same steps (comma list; '' for none)")
-    ap.add_argument("--full-steps", type=int, default=1000,
-                    help="steps of the whole-run comparison of the default path with plain_fp64 (0: skip)")
     ap.add_argument("--profile-kernel", default="gcr_update@0",
                     help="kernel@level timed for the roofline (the largest HBM-bound kernel of the path)")
     ap.add_argument("--timing-steps", type=int, default=3, help="steps of the kernel timing region")
@@ -98,9 +94,15 @@ def parse():
     ap.add_argument("--mg-schedule", choices=("graph", "fused", "plain"), default="graph",
                     help="V-cycle launch schedule (bit-identical results)")
     ap.add_argument("--sweeps", default="1,1", help="pre,post smoothing sweeps of the FP32 levels")
-    ap.add_argument("--weight", type=float, default=0.8, help="Jacobi weight of those sweeps")
+    ap.add_argument("--weight", type=float, default=0.9, help="Jacobi weight of those sweeps")
     ap.add_argument("--coarse", choices=("direct", "tail"), default="direct",
                     help="exact dense solve of the coarse levels where they have no obstacle, or their V-cycle")
+    ap.add_argument("--coarse-sweeps", type=int, default=5,
+                    help="sweeps of the GCR preconditioner's coarse Jacobi solve (a fixed count; "
+                         "the effective count follows the reference's pointer-swap rule)")
+    ap.add_argument("--coarse-tol", type=float, default=-1.0,
+                    help="stopping test of that coarse solve: <0 fixed sweeps, 0 the XML's absolute "
+                         "tolerance, >0 relative reduction")
     ap.add_argument("--tail-sweeps", type=int, default=3,
                     help="n_relax of the preconditioner's coarse levels (single-workgroup kernel; 0: the XML's)")
     ap.add_argument("--tail-cells", type=int, default=4096,
@@ -161,7 +163,9 @@ def make_solver(artss_amd, args, leg, net=None):
         return solver
     solver.set_pressure_mode(True, args.tolerance, 500)
     method = {"mixed_extrapolated": 2, "mixed_learned": 2, "gcr_fp64": 1, "plain_fp64": 0}[leg]
-    solver.set_pressure_method(method)
+    solver.set_pressure_method(method, args.coarse_sweeps)
+    if method != 0:
+        solver.set_coarse_tolerance(args.coarse_tol)
     if method == 2:
         pre, post = (int(v) for v in args.sweeps.split(","))
         solver.set_mixed_sweeps(pre, post, args.weight)
@@ -273,39 +277,14 @@ def leg_run(artss_amd, args, leg, dt, inner):
     fields = {f: s.field(f) for f in FIELDS}
     out = {"value": inner * args.steps / secs, "unit": "cell-updates/s", "ms_per_step": secs * 1e3 / args.steps,
            "steps": "%d..%d" % (first, run.step - 1), "path": PATHS[leg],
-           "mean_iterations": float(np.mean(cyc)),
-           "max_final_residual": None if leg == "reference_schedule" else float(np.max(res))}
+           "mean_iterations": float(np.mean(cyc)), "max_iterations": int(np.max(cyc)),
+           "max_final_residual": None if leg == "reference_schedule" else float(np.max(res)),
+           "fallbacks": s.fallbacks()}
     out.update(extra)
     if net is not None:
         net.close()
     s.close()
     return out, fields
-
-
-def full_run(artss_amd, args, dt, inner):
-    """the whole input from rest: default path and plain FP64"""
-    out = {"steps": args.full_steps}
-    fields = {}
-    for leg in ("mixed_extrapolated", "plain_fp64"):
-        s = make_solver(artss_amd, args, leg)
-        run = Run(s, dt)
-        head = args.full_steps - min(100, args.full_steps)
-        t_head, c_head, _, r_head = run.timed(head) if head else (0.0, [], [], [])
-        t_tail, c_tail, _, r_tail = run.timed(args.full_steps - head)
-        fields[leg] = {f: s.field(f) for f in FIELDS}
-        total = t_head + t_tail
-        out[leg] = {"value": inner * args.full_steps / total, "ms_per_step": total * 1e3 / args.full_steps,
-                    "last_100": {"value": inner * (args.full_steps - head) / t_tail,
-                                 "ms_per_step": t_tail * 1e3 / (args.full_steps - head),
-                                 "mean_iterations": float(np.mean(c_tail))},
-                    "mean_iterations": float(np.mean(c_head + c_tail)),
-                    "max_final_residual": float(np.max(r_head + r_tail))}
-        s.close()
-    out["speedup"] = out["mixed_extrapolated"]["value"] / out["plain_fp64"]["value"]
-    out["speedup_last_100"] = (out["mixed_extrapolated"]["last_100"]["value"] /
-                               out["plain_fp64"]["last_100"]["value"])
-    out["max_rel_field_difference_at_last_step"] = rel_errors(fields["mixed_extrapolated"], fields["plain_fp64"])
-    return out
 
 
 def ensemble_run(artss_amd, args):
@@ -321,7 +300,8 @@ def ensemble_run(artss_amd, args):
             m = artss_amd.Solver(x, device=args.device)
             m.set_mg_schedule(2, args.tail_cells)
             m.set_pressure_mode(True, args.tolerance, 500)
-            m.set_pressure_method(2)
+            m.set_pressure_method(2, args.coarse_sweeps)
+            m.set_coarse_tolerance(args.coarse_tol)
             m.set_warm_start(1)
             members.append(m)
         cells = sum(m.domain.info(0)["inner_cells"] for m in members)
@@ -364,6 +344,7 @@ def main():
     secs, cycles, diff_it, residuals = run.timed(args.steps)
     value = inner * args.steps / secs
     fields = {f: solver.field(f) for f in FIELDS}
+    fallbacks = solver.fallbacks()
 
     # kernel timing region: HIP events around every launch of the profiled
     # kernel on the solver's stream; the V-cycle then runs its launches
@@ -398,7 +379,8 @@ def main():
     legs = {args.path: {"value": value, "unit": "cell-updates/s", "ms_per_step": secs * 1e3 / args.steps,
                         "steps": "%d..%d" % (first, first + args.steps - 1),
                         "path": PATHS[args.path], "mean_iterations": float(np.mean(cycles)),
-                        "max_final_residual": float(np.max(residuals))}}
+                        "max_iterations": int(np.max(cycles)), "max_final_residual": float(np.max(residuals)),
+                        "fallbacks": fallbacks}}
     leg_fields = {args.path: fields}
     for leg in [x for x in args.legs.split(",") if x and x != args.path]:
         legs[leg], leg_fields[leg] = leg_run(artss_amd, args, leg, dt, inner)
@@ -408,7 +390,6 @@ def main():
             legs[leg]["speedup_over_plain_fp64"] = legs[leg]["value"] / legs["plain_fp64"]["value"]
             legs[leg]["max_rel_field_difference_to_plain_fp64"] = rel_errors(f, ref)
 
-    full = full_run(artss_amd, args, dt, inner) if args.full_steps > 0 else None
     ensemble = ensemble_run(artss_amd, args) if args.ensemble_members > 0 else None
 
     out = {
@@ -420,12 +401,13 @@ def main():
         "config": {"workload": "plume_256x128x128", "grid": list(info["n"]), "inner_cells": inner,
                    "path": PATHS[args.path], "mg_schedule": args.mg_schedule,
                    "tail_cells": args.tail_cells, "sweeps": args.sweeps, "weight": args.weight,
-                   "coarse": args.coarse, "tail_sweeps": args.tail_sweeps, "tuned_cells_per_thread": tuned,
+                   "coarse": args.coarse, "tail_sweeps": args.tail_sweeps, "coarse_sweeps": args.coarse_sweeps,
+                   "coarse_tol": args.coarse_tol, "tuned_cells_per_thread": tuned,
                    "pressure": "iterate until ||rhs - lap p||_2 <= %g every step" % args.tolerance,
                    "mean_vcycles": float(np.mean(cycles)),
                    "mean_diffusion_iterations": np.mean(diff_it, axis=0).tolist(),
                    "max_final_residual": float(np.max(residuals))},
-        "legs": legs, "full_run": full, "ensemble": ensemble,
+        "legs": legs, "ensemble": ensemble,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

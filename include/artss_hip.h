@@ -171,9 +171,11 @@ int artss_solver_set_pressure_mode(artss_solver *s, int to_tolerance, double tol
  * (obstacles); method 2 is method 1 with the V-cycle's levels above its
  * single-workgroup coarse levels in FP32 (smoothing, residuum, restriction,
  * prolongation), the coarse levels, the GCR iteration, residuals and norms
- * in FP64; with method 2, coarse_max_iter > 0 caps the preconditioner's
- * coarse Jacobi solve (0: the XML's max_iter).  The final residual is the
- * true FP64 one in every method. */
+ * in FP64; with methods 1 and 2, coarse_max_iter > 0 sets the sweeps of the
+ * preconditioner's coarse Jacobi solve (0: five; see
+ * artss_solver_set_coarse_tolerance).  A GCR solve that stalls is finished
+ * by method 0's V-cycles (artss_solver_fallbacks).  The final residual is
+ * the true FP64 one in every method. */
 int artss_solver_set_pressure_method(artss_solver *s, int method, int coarse_max_iter);
 /* artss_mg_set_schedule for the solver's own pressure solver */
 int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cells);
@@ -199,6 +201,21 @@ int artss_solver_set_coarse_solve(artss_solver *s, int direct);
  * weight of artss_solver_set_mixed_sweeps; 0 (the default) keeps the XML's
  * n_relax and weight. */
 int artss_solver_set_mixed_tail(artss_solver *s, int n_relax);
+/* Stopping test of the coarse solve inside the GCR preconditioner of
+ * methods 1 and 2 (no reference counterpart; the reference's coarse solve
+ * stops at the XML's absolute tol_res).  coarse_tol 0: that absolute
+ * tolerance; 0 < coarse_tol < 1: stop once the coarse residual has fallen by
+ * that factor from its start, whatever the scale of the right-hand side;
+ * < 0 (the default): no test, a fixed number of sweeps (the coarse_max_iter
+ * of artss_solver_set_pressure_method, 0: five).  An absolute tolerance makes
+ * the preconditioner weaker as the GCR residual shrinks, which stalls GCR on
+ * the 256x128x128 plume from step 709 on; a fixed count keeps the
+ * preconditioner one linear operator. */
+int artss_solver_set_coarse_tolerance(artss_solver *s, double coarse_tol);
+/* Steps since creation whose GCR pressure solve (methods 1, 2) stopped
+ * falling for GCR's stall window and was finished by the reference's V-cycle
+ * iteration within the same cycle budget (no reference counterpart). */
+int artss_solver_fallbacks(artss_solver *s, long long *steps);
 
 /* No reference counterpart: the cells per thread along z (1, 2 or 4) that the
  * autotuner picked for the fused level kernels of levels with this many
