@@ -224,6 +224,14 @@ def summarize(rows, L, steps, ms_per_step=None, turbulent=True, tail_cells=4096,
         lines.append((t_all, kernel + ("<%s>" % ",".join(args) if args else ""), "%dx%dx%d" % (gx, gy, gz), len(ns),
                       idle, t_all / len(ns) / 1e3, min(ns) / 1e3, b, gbs, note))
     lines.sort(key=lambda x: -x[0])
+    # mean duration of the level-0 GCR update launches that did work (the
+    # kernel the benchmark's roofline names)
+    mean_us = {}
+    upd = [r["ns"] for r in rows if parse_name(r["name"])[0] == "k_gcr_update"]
+    if upd:
+        med = statistics.median(upd)
+        worked = [t for t in upd if t >= 0.25 * med]
+        mean_us["gcr_update"] = sum(worked) / len(worked) / 1e3
     if title:
         out.append("# " + title)
         out.append("")
@@ -252,6 +260,14 @@ def summarize(rows, L, steps, ms_per_step=None, turbulent=True, tail_cells=4096,
         frac_s = per_step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
         out.append("- per step over the measured wall time %.3f ms: %.0f GB/s = **%.3f of 8 TB/s**" % (
             ms_per_step, per_step_bytes / (ms_per_step * 1e-3) / 1e9, frac_s))
+    out.append("")
+    out.append("Figures read by bench.py:")
+    out.append("")
+    out.append("- `kernel_frac = %.4f`" % frac_k)
+    if frac_s is not None:
+        out.append("- `step_frac = %.4f`" % frac_s)
+    for name, mean in sorted(mean_us.items()):
+        out.append("- `mean_us[%s] = %.3f`" % (name, mean))
     return "\n".join(out), {"kernel_frac": frac_k, "step_frac": frac_s, "bytes_per_step": per_step_bytes,
                             "kernel_ms_per_step": per_step_kernel_ms}
 

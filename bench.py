@@ -186,7 +186,7 @@ def profile_value(prefix, key):
             continue
         with open(os.path.join(d, name)) as fh:
             for line in fh:
-                line = line.strip().strip("`").lstrip("- ")
+                line = line.strip().lstrip("- ").strip("`")
                 if line.startswith(key + " = "):
                     v = line.split("=", 1)[1].strip().split()[0]
                     try:

@@ -132,6 +132,20 @@ def test_c3_accelerated_paths_vs_plain_fp64(plain_256, method):
         assert errs[f] <= 1e-6, (f, errs[f])
 
 
+def test_c3_whole_input_converges_every_step():
+    """All 1000 steps of the input on the default path: every pressure solve
+    meets ||rhs - lap p||_2 <= 1e-8 by GCR alone (no step needs the V-cycle
+    fallback).  With the reference's absolute tolerance in the
+    preconditioner's coarse solve, both GCR paths stalled near 4e-6 from
+    step 709 on."""
+    s = _plume_solver(2, warm=1)
+    log = []
+    _steps(s, 1, 1000, log)
+    assert s.fallbacks() == 0
+    print("C3 GCR iterations: mean %.1f, max %d, steps 700..710 %s" % (np.mean(log), max(log), log[699:710]))
+    s.close()
+
+
 def test_c4_learned_mixed_vs_plain_fp64(plain_256):
     s = _plume_solver(2, warm=1, record=True)
     dom = s.domain
