@@ -73,6 +73,20 @@ void cgs_color(hipStream_t s, const LevelDesc &L, int color, double *out, const 
 // (rb*(out-in))^2 partials over inner cells, ColoredGaussSeidelDiffuse.cpp:77-82, :147-157
 void update_norm_partials(hipStream_t s, const LevelDesc &L, const double *out, const double *in, double rb,
                           double D, const double *ev, double dt, double *partial, const int *gate = nullptr);
+// coefficients of one colored Gauss-Seidel iteration: constant ax, ay, az,
+// beta = 1/rb (ev == nullptr), else per cell from D, ev, dt and 1/h
+struct CgsCoef {
+    double ax, ay, az, beta, rb, dsign, w, D, dt, rdx, rdy, rdz;
+};
+// one whole iteration (even sweep, odd sweep, fill of the result, norm
+// partials) out of place from src into dst; the fill must be idempotent
+// (Domain::bc_idempotent) and write every ghost cell (BCView::shell_filled).
+// cgs_iter_partials: the partial sums it writes.
+int cgs_iter_partials(const LevelDesc &L);
+void cgs_iter(hipStream_t s, const LevelDesc &L, const BCView &V, const CgsCoef &c, const double *ev, const double *b,
+              const double *src, double *dst, double *partial, const int *gate);
+// exchanges the contents of two arrays of n values
+void swap_arrays(hipStream_t s, double *a, double *b, long long n);
 
 // ---- multigrid (VCycleMG.cpp) ----
 void residuum(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *b);   // :346-354

@@ -16,6 +16,7 @@
 #include "stencil.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <array>
 #include <atomic>
 #include <map>
@@ -531,6 +532,159 @@ __device__ inline bool is_inner(const LevelDesc &L, long long c, int &i, int &j,
     return i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k);
 }
 
+// ---- one colored Gauss-Seidel iteration in one pass
+// ColoredGaussSeidelDiffuse.cpp:111-173 (the turbulent form; ev == nullptr
+// gives the constant coefficients of :38-98): the sweep over the cells
+// with i + j + k even, the sweep over the odd ones, the boundary fill of
+// `out` and the update norm's partial sums.  Out of place: src holds the
+// iterate with its fill, dst receives the next iterate on every cell of the
+// array -- fluid cells the update, cells the fill writes the fill, all
+// others src's value -- which is what the reference's in-place sweeps and
+// fill leave in `out`; src is then what its `in` holds.  A workgroup covers
+// a 64x4 column of cells over kCgsPlanes planes.  It evaluates the even
+// update of each plane on its tile and a one-cell rim into LDS (three
+// planes kept), then the odd update of the plane below from those values;
+// a neighbouring workgroup evaluates the same rim cells with the same
+// arithmetic.  Every update is the expression of k_cgs_color, so values
+// are bit-identical to the two colour launches.
+constexpr int kCgsPlanes = 4;  // planes per workgroup (4 and 8 measured equal, 16 and 2 slower)
+constexpr int RX = BX + 2, RY = BY + 2;
+
+__device__ __forceinline__ double cgs_value(const CgsCoef &c, const double *__restrict__ ev, long long index, double r_out,
+                                   double ox, double ox2, double oy, double oy2, double oz, double oz2, double db,
+                                   double *rb_out) {
+    double aX = c.ax, aY = c.ay, aZ = c.az, bb = c.beta, rb = c.rb;
+    if (ev != nullptr) {
+        const double e = ev[index];
+        aX = (c.D + e) * c.dt * c.rdx * c.rdx;
+        aY = (c.D + e) * c.dt * c.rdy * c.rdy;
+        aZ = (c.D + e) * c.dt * c.rdz * c.rdz;
+        rb = (1. + 2. * (aX + aY + aZ));
+        bb = 1. / rb;
+    }
+    if (rb_out) *rb_out = rb;
+    const double out_h = bb * (c.dsign * db + aX * (ox + ox2) + aY * (oy + oy2) + aZ * (oz + oz2));
+    return (1 - c.w) * r_out + c.w * out_h;
+}
+// the even (first-swept) update of fluid cell idx from src
+__device__ __forceinline__ double cgs_even(const LevelDesc &L, const CgsCoef &c, const double *__restrict__ ev,
+                                  const double *__restrict__ src, const double *__restrict__ b, long long idx) {
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    return cgs_value(c, ev, idx, src[idx], src[idx + 1], src[idx - 1], src[idx + nj], src[idx - nj], src[idx + nk],
+                     src[idx - nk], b[idx], nullptr);
+}
+__device__ __forceinline__ bool fluid_at(const LevelDesc &L, int i, int j, int k) {
+    return i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k);
+}
+// value of cell (i, j, k) after the even sweep
+__device__ __forceinline__ double cgs_after_even(const LevelDesc &L, const CgsCoef &c, const double *__restrict__ ev,
+                                        const double *__restrict__ src, const double *__restrict__ b, int i, int j,
+                                        int k) {
+    const long long idx = lin(L, i, j, k);
+    if (((i + j + k) & 1) == 0 && fluid_at(L, i, j, k)) return cgs_even(L, c, ev, src, b, idx);
+    return src[idx];
+}
+
+// plane m after the even sweep, over a workgroup's 64x4 tile and its rim
+__device__ __forceinline__ void cgs_even_plane(const LevelDesc &L, const CgsCoef &c, const double *__restrict__ ev,
+                                               const double *__restrict__ src, const double *__restrict__ b, int i0,
+                                               int j0, int m, int tid, double (*slot)[RX]) {
+    for (int q = tid; q < RX * RY; q += BX * BY) {
+        const int ii = q % RX, jj = q / RX;
+        const int i = i0 - 1 + ii, j = j0 - 1 + jj;
+        double v = 0.0;
+        if (i < L.N[0] && j < L.N[1]) v = cgs_after_even(L, c, ev, src, b, i, j, m);
+        slot[jj][ii] = v;
+    }
+}
+
+__global__ void __launch_bounds__(BX *BY) k_cgs_iter(LevelDesc L, BCView V, int main_z, int planes, CgsCoef c,
+                                                      const double *__restrict__ ev, const double *__restrict__ b,
+                                                      const double *__restrict__ src, double *__restrict__ dst,
+                                                      double *__restrict__ partial, const int *gate) {
+    if (gate != nullptr && *gate) return;  // uniform over the grid
+    if (static_cast<int>(blockIdx.z) >= main_z) {
+        // fill items: the value the iteration gives the cell each write reads
+        auto val = [&](long long q) -> double {
+            int i, j, k;
+            if (!is_inner(L, q, i, j, k)) return src[q];
+            if (((i + j + k) & 1) == 0) return cgs_even(L, c, ev, src, b, q);
+            // the six neighbours after the even sweep (one evaluation in a
+            // loop keeps the item code small)
+            double nb[6];
+#pragma unroll 1
+            for (int d = 0; d < 6; d++) {
+                const int a = d >> 1, sgn = (d & 1) ? -1 : 1;
+                const double x = cgs_after_even(L, c, ev, src, b, i + (a == 0 ? sgn : 0), j + (a == 1 ? sgn : 0),
+                                                k + (a == 2 ? sgn : 0));
+                nb[0] = d == 0 ? x : nb[0];
+                nb[1] = d == 1 ? x : nb[1];
+                nb[2] = d == 2 ? x : nb[2];
+                nb[3] = d == 3 ? x : nb[3];
+                nb[4] = d == 4 ? x : nb[4];
+                nb[5] = d == 5 ? x : nb[5];
+            }
+            return cgs_value(c, ev, q, src[q], nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], b[q], nullptr);
+        };
+        dev::bc_item<double, decltype(val), true>(L, V, item_index(main_z), dst, val);
+        return;
+    }
+    __shared__ double even[3][RY][RX];
+    int bx, by, bz;
+    main_block(main_z, bx, by, bz);
+    const int i0 = 1 + bx * BX, j0 = 1 + by * BY, k0 = 1 + bz * planes;
+    const int k1 = min(k0 + planes, L.n[2] + 1);
+    const int tx = static_cast<int>(threadIdx.x), ty = static_cast<int>(threadIdx.y), tid = ty * BX + tx;
+    // plane m after the even sweep over the tile and its rim, into slot m % 3
+    auto even_plane = [&](int m) { cgs_even_plane(L, c, ev, src, b, i0, j0, m, tid, even[m % 3]); };
+    even_plane(k0 - 1);
+    even_plane(k0);
+    double sq = 0.0;
+    const int i = i0 + tx, j = j0 + ty;
+    for (int k = k0; k < k1; k++) {
+        even_plane(k + 1);
+        __syncthreads();
+        if (i <= L.n[0] && j <= L.n[1]) {
+            const long long idx = lin(L, i, j, k);
+            const double old = src[idx];
+            if (!in_obstacle(L, i, j, k)) {
+                const double(*lo)[RX] = even[(k - 1) % 3];
+                const double(*mid)[RX] = even[k % 3];
+                const double(*hi)[RX] = even[(k + 1) % 3];
+                double v, rb;
+                if (((i + j + k) & 1) == 0) {
+                    v = mid[ty + 1][tx + 1];
+                    rb = c.rb;
+                    if (ev != nullptr) {
+                        const double e = ev[idx];
+                        rb = (1. + 2. * ((c.D + e) * c.dt * c.rdx * c.rdx + (c.D + e) * c.dt * c.rdy * c.rdy +
+                                         (c.D + e) * c.dt * c.rdz * c.rdz));
+                    }
+                } else {
+                    v = cgs_value(c, ev, idx, old, mid[ty + 1][tx + 2], mid[ty + 1][tx], mid[ty + 2][tx + 1],
+                                  mid[ty][tx + 1], hi[ty + 1][tx + 1], lo[ty + 1][tx + 1], b[idx], &rb);
+                }
+                dst[idx] = v;
+                dev::face_ghosts(L, V, dst, idx, i, j, k, v);
+                const double r = rb * (v - old);
+                sq += r * r;
+            } else if (!dev::written(V, idx)) {
+                dst[idx] = old;
+            }
+        }
+        __syncthreads();
+    }
+    write_partial(partial, sq);
+}
+
+__global__ void k_swap(double *__restrict__ a, double *__restrict__ b, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+        const double x = a[i], y = b[i];
+        a[i] = y;
+        b[i] = x;
+    }
+}
+
 // dst = src with the fill applied (copy of the whole array + fill in one launch)
 __global__ void k_copy_bc(LevelDesc L, BCView V, int main_z, int kz, double *__restrict__ dst,
                           const double *__restrict__ src) {
@@ -989,6 +1143,21 @@ void update_norm_partials(hipStream_t s, const LevelDesc &L, const double *out, 
     k_update_norm<<<inner_grid(L), block(), 0, s>>>(L, c, out, in, ev, partial, gate);
 }
 
+int cgs_planes() { return kCgsPlanes; }
+int cgs_iter_partials(const LevelDesc &L) {
+    const int planes = cgs_planes();
+    return ((L.n[0] + BX - 1) / BX) * ((L.n[1] + BY - 1) / BY) * ((L.n[2] + planes - 1) / planes);
+}
+void cgs_iter(hipStream_t s, const LevelDesc &L, const BCView &V, const CgsCoef &c, const double *ev, const double *b,
+              const double *src, double *dst, double *partial, const int *gate) {
+    const int planes = cgs_planes();
+    const dim3 g((L.n[0] + BX - 1) / BX, (L.n[1] + BY - 1) / BY, (L.n[2] + planes - 1) / planes);
+    k_cgs_iter<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), planes, c, ev, b,
+                                                                         src, dst, partial, gate);
+}
+void swap_arrays(hipStream_t s, double *a, double *b, long long n) {
+    k_swap<<<grid1d(n, 256), 256, 0, s>>>(a, b, n);
+}
 void residuum(hipStream_t s, const LevelDesc &L, double *out, const double *x, const double *b) {
     k_residuum<<<inner_grid(L), block(), 0, s>>>(L, out, x, b);
 }

@@ -45,7 +45,7 @@ __device__ inline Cell box_cell(const LevelDesc &L, int t, int n) {
 __device__ inline int box_cells(const LevelDesc &L) { return L.n[0] * L.n[1] * L.n[2]; }
 
 // DomainController::apply_boundary: obstacle faces wave by wave, then ghosts
-__device__ void t_bc(const TailLevel &T, double *f) {
+__device__ __forceinline__ void t_bc(const TailLevel &T, double *f) {
     for (int w = 0; w < T.n_waves; w++) {
         const int e0 = static_cast<int>(T.wave_start[w]), e1 = static_cast<int>(T.wave_start[w + 1]);
         for (int e = e0 + threadIdx.x; e < e1; e += kTailThreads) dev::face_write(T.faces[e], T.params, f);
@@ -58,7 +58,7 @@ __device__ void t_bc(const TailLevel &T, double *f) {
     }
 }
 
-__device__ void t_copy(double *__restrict__ dst, const double *__restrict__ src, int n) {
+__device__ __forceinline__ void t_copy(double *__restrict__ dst, const double *__restrict__ src, int n) {
     for (int base = threadIdx.x; base < n; base += kStride) {
         double v[kBatch];
 #pragma unroll
@@ -75,12 +75,12 @@ __device__ void t_copy(double *__restrict__ dst, const double *__restrict__ src,
     __syncthreads();
 }
 
-__device__ void t_fill(double *dst, double v, int n) {
+__device__ __forceinline__ void t_fill(double *dst, double v, int n) {
     for (int t = threadIdx.x; t < n; t += kTailThreads) dst[t] = v;
     __syncthreads();
 }
 
-__device__ void t_add(double *__restrict__ dst, const double *__restrict__ src, int n) {
+__device__ __forceinline__ void t_add(double *__restrict__ dst, const double *__restrict__ src, int n) {
     for (int base = threadIdx.x; base < n; base += kStride) {
         double v[kBatch];
 #pragma unroll
@@ -97,7 +97,7 @@ __device__ void t_add(double *__restrict__ dst, const double *__restrict__ src, 
     __syncthreads();
 }
 
-__device__ void t_jacobi(const TailLevel &T, const TailPlan &P, double *__restrict__ out,
+__device__ __forceinline__ void t_jacobi(const TailLevel &T, const TailPlan &P, double *__restrict__ out,
                          const double *__restrict__ in, const double *__restrict__ b) {
     const int n = box_cells(T.L);
     for (int base = threadIdx.x; base < n; base += kStride) {
@@ -116,7 +116,7 @@ __device__ void t_jacobi(const TailLevel &T, const TailPlan &P, double *__restri
 }
 
 // one color, in place: cells of a color read only cells of the other color
-__device__ void t_cgs_color(const TailLevel &T, const TailPlan &P, int color, double *out, const double *b) {
+__device__ __forceinline__ void t_cgs_color(const TailLevel &T, const TailPlan &P, int color, double *out, const double *b) {
     const int n = box_cells(T.L);
     const int nx = T.L.n[0], ny = T.L.n[1];
     for (int base = threadIdx.x; base < n; base += kStride) {
@@ -140,7 +140,7 @@ __device__ void t_cgs_color(const TailLevel &T, const TailPlan &P, int color, do
     __syncthreads();
 }
 
-__device__ void t_residuum(const TailLevel &T, double *__restrict__ out, const double *__restrict__ x,
+__device__ __forceinline__ void t_residuum(const TailLevel &T, double *__restrict__ out, const double *__restrict__ x,
                            const double *__restrict__ b) {
     const int n = box_cells(T.L);
     for (int base = threadIdx.x; base < n; base += kStride) {
@@ -159,7 +159,7 @@ __device__ void t_residuum(const TailLevel &T, double *__restrict__ out, const d
 }
 
 // ||b - lap x||_2 over the inner cells, same value in every thread
-__device__ double t_residual(const TailLevel &T, const double *__restrict__ x, const double *__restrict__ b,
+__device__ __forceinline__ double t_residual(const TailLevel &T, const double *__restrict__ x, const double *__restrict__ b,
                              double *red) {
     const int n = box_cells(T.L);
     double sq = 0.0;
@@ -188,7 +188,7 @@ __device__ double t_residual(const TailLevel &T, const double *__restrict__ x, c
     return r;
 }
 
-__device__ void t_restrict(const TailLevel &Tc, const TailLevel &Tf, double *__restrict__ out,
+__device__ __forceinline__ void t_restrict(const TailLevel &Tc, const TailLevel &Tf, double *__restrict__ out,
                            const double *__restrict__ in) {
     const int n = box_cells(Tc.L);
     const int nx = Tc.L.n[0], ny = Tc.L.n[1];
@@ -212,7 +212,7 @@ __device__ void t_restrict(const TailLevel &Tc, const TailLevel &Tf, double *__r
     __syncthreads();
 }
 
-__device__ void t_prolongate(const TailLevel &Tc, const TailLevel &Tf, double *__restrict__ out,
+__device__ __forceinline__ void t_prolongate(const TailLevel &Tc, const TailLevel &Tf, double *__restrict__ out,
                              const double *__restrict__ in) {
     const int n = box_cells(Tc.L);
     const int nx = Tc.L.n[0], ny = Tc.L.n[1];
@@ -235,7 +235,7 @@ __device__ void t_prolongate(const TailLevel &Tc, const TailLevel &Tf, double *_
 // buffer, which holds the last sweep for odd n_relax and the one before it
 // for even n_relax.  That last sweep of an even count is never read and is
 // skipped; the remaining count is odd and leaves its result in `out`.
-__device__ void t_smooth_jacobi(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b) {
+__device__ __forceinline__ void t_smooth_jacobi(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b) {
     t_bc(T, out);
     t_copy(tmp, out, static_cast<int>(T.L.size));
     const int n = (P.n_relax % 2 == 0 && P.n_relax > 0) ? P.n_relax - 1 : P.n_relax;
@@ -253,7 +253,7 @@ __device__ void t_smooth_jacobi(const TailLevel &T, const TailPlan &P, double *o
 // res > tol (absolute); coarse_tol > 0, res > coarse_tol * (the residual of
 // the starting iterate), a reduction independent of the right-hand side's
 // scale; coarse_tol < 0, no test (max_iter sweeps, no residual computed).
-__device__ double solve_target(const TailLevel &T, const TailPlan &P, double coarse_tol, const double *x,
+__device__ __forceinline__ double solve_target(const TailLevel &T, const TailPlan &P, double coarse_tol, const double *x,
                                const double *b, double *red) {
     if (coarse_tol == 0.0) return P.tol;
     if (coarse_tol < 0.0) return -1.0;
@@ -261,7 +261,7 @@ __device__ double solve_target(const TailLevel &T, const TailPlan &P, double coa
 }
 
 // VCycleMG.cpp:725-798
-__device__ void t_solve_jacobi(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b,
+__device__ __forceinline__ void t_solve_jacobi(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b,
                                double *red, double coarse_tol) {
     t_bc(T, out);
     t_copy(tmp, out, static_cast<int>(T.L.size));
@@ -283,7 +283,7 @@ __device__ void t_solve_jacobi(const TailLevel &T, const TailPlan &P, double *ou
 }
 
 // VCycleMG.cpp:583-615
-__device__ void t_smooth_cgs(const TailLevel &T, const TailPlan &P, double *out, const double *b) {
+__device__ __forceinline__ void t_smooth_cgs(const TailLevel &T, const TailPlan &P, double *out, const double *b) {
     t_bc(T, out);
     for (int i = 0; i < P.n_relax; i++) {
         t_cgs_color(T, P, 0, out, b);
@@ -293,7 +293,7 @@ __device__ void t_smooth_cgs(const TailLevel &T, const TailPlan &P, double *out,
 }
 
 // VCycleMG.cpp:617-683
-__device__ void t_solve_cgs(const TailLevel &T, const TailPlan &P, double *out, const double *b, double *red,
+__device__ __forceinline__ void t_solve_cgs(const TailLevel &T, const TailPlan &P, double *out, const double *b, double *red,
                             double coarse_tol) {
     t_bc(T, out);
     int it = 0;
@@ -309,14 +309,14 @@ __device__ void t_solve_cgs(const TailLevel &T, const TailPlan &P, double *out, 
 }
 
 // VCycleMG::Smooth :299-308
-__device__ void t_smooth(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b) {
+__device__ __forceinline__ void t_smooth(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b) {
     t_bc(T, out);
     if (P.cgs) t_smooth_cgs(T, P, out, b);
     else t_smooth_jacobi(T, P, out, tmp, b);
 }
 
 // VCycleMG::Solve :562-581
-__device__ void t_solve(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b,
+__device__ __forceinline__ void t_solve(const TailLevel &T, const TailPlan &P, double *out, double *tmp, const double *b,
                         double *red, double coarse_tol) {
     if (P.solve_smooths) {
         t_smooth(T, P, out, tmp, b);

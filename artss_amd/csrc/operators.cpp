@@ -121,8 +121,6 @@ void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b,
     const double rb = (1. + 2. * (ax + ay + az));
     const double beta = 1. / rb;
     const double *ev = EV ? EV->data : nullptr;
-    const int n_part = k::partial_count(L);
-    out.copy_data(in);
     // The loop test runs on the device (k::IterGate): as many iterations as
     // the previous call took are enqueued at once, then one at a time; the
     // kernels of iterations past the test's stop do nothing, so the fields
@@ -133,6 +131,17 @@ void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b,
         ARTSS_HIP(hipHostMalloc(&m_gate_host, sizeof(k::IterGate)));
     }
     const int *stop = &m_gate->done;
+    const int ft = out.get_type();
+    if (dm.bc_idempotent(0, ft)) {
+        const BCView V = dm.bc_view(0, ft);
+        if (V.shell_filled && k::cgs_iter_partials(L) <= dm.partial_capacity()) {
+            const k::CgsCoef c{ax, ay, az, beta, rb, 1.0, m_settings.w, D, dt, rdx, rdy, rdz};
+            run_one_pass(out, in, b, ev, V, c);
+            return;
+        }
+    }
+    const int n_part = k::partial_count(L);
+    out.copy_data(in);
     k::gate_open(s, m_gate, m_settings.tol_res, m_settings.max_iter);
     int launched = 0;
     while (true) {
@@ -156,6 +165,46 @@ void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b,
     m_iterations = m_gate_host->count;
     m_residual = m_iterations > 0 ? std::sqrt(m_gate_host->sum) : 1.;
     m_batch_hint = m_iterations;
+}
+
+// The same loop with each iteration as one out-of-place kernel
+// (k::cgs_iter: both colours, the fill and the norm partials).  The
+// reference copies in <- out before every iteration and updates out in
+// place, so after iteration q `out` holds x_q and `in` x_(q-1).  Here
+// iteration 1 reads `in` (the reference's out equals it then) and writes
+// `out`, iteration 2 reads `out` and writes `in`, and so on: after an odd
+// count both arrays hold what the reference leaves in them, after an even
+// count their contents are exchanged once at the end.
+void ColoredGaussSeidelDiffuse::run_one_pass(Field &out, const Field &in, const Field &b, const double *ev,
+                                             const BCView &V, const k::CgsCoef &c) {
+    Domain &dm = m_domain;
+    const LevelDesc &L = dm.level(0);
+    hipStream_t s = dm.stream();
+    const int *stop = &m_gate->done;
+    const int n_part = k::cgs_iter_partials(L);
+    k::gate_open(s, m_gate, m_settings.tol_res, m_settings.max_iter);
+    int launched = 0;
+    while (true) {
+        const int batch = launched < m_batch_hint ? m_batch_hint - launched : 1;
+        for (int q = 0; q < batch; q++) {
+            const bool odd = ((launched + q) & 1) == 0;  // iteration number launched + q + 1
+            const double *src = odd ? in.data : out.data;
+            double *dst = odd ? out.data : in.data;
+            dm.timed("cgs_diffuse", 0, [&] {
+                k::cgs_iter(s, L, V, c, ev, b.data, src, dst, dm.partial_buffer(), stop);
+            });
+            k::reduce_gate(s, dm.partial_buffer(), n_part, m_gate);
+        }
+        launched += batch;
+        ARTSS_HIP(hipMemcpyAsync(m_gate_host, m_gate, sizeof(k::IterGate), hipMemcpyDeviceToHost, s));
+        ARTSS_HIP(hipStreamSynchronize(s));
+        if (m_gate_host->done) break;
+    }
+    m_iterations = m_gate_host->count;
+    m_residual = m_iterations > 0 ? std::sqrt(m_gate_host->sum) : 1.;
+    m_batch_hint = m_iterations;
+    if (m_iterations == 0) out.copy_data(in);
+    else if (m_iterations % 2 == 0) k::swap_arrays(s, out.data, in.data, out.get_size());
 }
 
 ColoredGaussSeidelDiffuse::~ColoredGaussSeidelDiffuse() {

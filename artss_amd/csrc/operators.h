@@ -122,6 +122,8 @@ class ColoredGaussSeidelDiffuse : public IDiffusion {
 
   private:
     void run(Field &out, const Field &in, const Field &b, double D, const Field *EV);
+    void run_one_pass(Field &out, const Field &in, const Field &b, const double *ev, const BCView &V,
+                      const k::CgsCoef &c);
     Domain &m_domain;
     DiffusionSettings m_settings;
     k::IterGate *m_gate = nullptr;       // device loop state

@@ -112,28 +112,35 @@ __device__ __host__ inline long long ghost_count(const LevelDesc &L) {
 // Ghost cell number t (0 <= t < ghost_count) of a level: its coordinates
 // (the x faces first, then y without x edges, then z without x and y edges)
 // (32-bit arithmetic: a level's ghost count stays below 2^31)
+// (the coordinates are formed in locals and stored once: stores to the
+// reference parameters from different branches make the compiler keep them
+// in per-lane scratch memory)
 __device__ inline bool ghost_coords(const LevelDesc &L, long long t64, long long &i, long long &j, long long &k) {
     const int Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
     const int fx = Ny * Nz, fy = (Nx - 2) * Nz, fz = (Nx - 2) * (Ny - 2);
     int t = static_cast<int>(t64);
+    int a, b, c;  // i, j, k
     if (t < 2 * fx) {
         const int side = t >= fx, r = side ? t - fx : t;
-        i = side ? Nx - 1 : 0;
-        j = r % Ny;
-        k = r / Ny;
+        a = side ? Nx - 1 : 0;
+        b = r % Ny;
+        c = r / Ny;
     } else if ((t -= 2 * fx) < 2 * fy) {
         const int side = t >= fy, r = side ? t - fy : t;
-        j = side ? Ny - 1 : 0;
-        i = 1 + r % (Nx - 2);
-        k = r / (Nx - 2);
+        b = side ? Ny - 1 : 0;
+        a = 1 + r % (Nx - 2);
+        c = r / (Nx - 2);
     } else if ((t -= 2 * fy) < 2 * fz) {
         const int side = t >= fz, r = side ? t - fz : t;
-        k = side ? Nz - 1 : 0;
-        i = 1 + r % (Nx - 2);
-        j = 1 + r / (Nx - 2);
+        c = side ? Nz - 1 : 0;
+        a = 1 + r % (Nx - 2);
+        b = 1 + r / (Nx - 2);
     } else {
         return false;
     }
+    i = a;
+    j = b;
+    k = c;
     return true;
 }
 

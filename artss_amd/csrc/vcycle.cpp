@@ -344,7 +344,7 @@ void VCycleMG::vcycle_mixed(const double *r) {
         });
     } else {
         d.timed("mg_tail", f, [&] {
-            mg_tail(s, m_plan_dev, m_res1[f]->data, m_tail_lds_bytes, precond_coarse_iter(), m_mixed_tail,
+            mg_tail(s, m_plan_dev + 1, m_res1[f]->data, m_tail_lds_bytes[1], precond_coarse_iter(), m_mixed_tail,
                     m_mixed_tail > 0 ? m_mixed_weight : 0.0, true, m_pc_tol);
         });
     }
@@ -472,29 +472,39 @@ void VCycleMG::build_tail() {
         T.err0 = l < L ? m_err0[l]->data : nullptr;
         for (int a = 0; a < 5; a++) T.lds_off[a] = -1;
     }
-    // LDS placement, coarsest level first, whole levels, while they fit
+    // LDS placement, coarsest level first, whole levels, while they fit.
+    // Plan 0 serves the reference's V-cycle; plan 1 the launches that skip
+    // the coarsest level (skip_dead): there the coarsest level is never
+    // touched and the level above it needs no residuum and no prolongation
+    // target, so more of the finer levels fit in LDS.
     plan.io_err1 = m_err1[f]->data;
     const long long budget = mg_tail_lds_limit() / static_cast<long long>(sizeof(double));
-    long long used = 0;
-    for (int l = L; l >= f; l--) {
-        TailLevel &T = plan.lv[l - f];
-        const bool has[5] = {true, true, l < L, true, l < L};
-        long long need = 0;
-        for (int a = 0; a < 5; a++)
-            if (has[a]) need += d.level(l).size;
-        if (used + need > budget) break;
-        for (int a = 0; a < 5; a++)
-            if (has[a]) {
-                T.lds_off[a] = static_cast<int>(used);
-                used += d.level(l).size;
-            }
+    TailPlan plans[2] = {plan, plan};
+    for (int dead = 0; dead < 2; dead++) {
+        TailPlan &P = plans[dead];
+        long long used = 0;
+        for (int l = L; l >= f; l--) {
+            TailLevel &T = P.lv[l - f];
+            bool has[5] = {true, true, l < L, true, l < L};
+            if (dead && l == L) continue;  // never touched
+            if (dead && l == L - 1) has[2] = has[4] = false;
+            long long need = 0;
+            for (int a = 0; a < 5; a++)
+                if (has[a]) need += d.level(l).size;
+            if (used + need > budget) break;
+            for (int a = 0; a < 5; a++)
+                if (has[a]) {
+                    T.lds_off[a] = static_cast<int>(used);
+                    used += d.level(l).size;
+                }
+        }
+        // a level in LDS needs every coarser one there too (the loop stops at the first misfit)
+        P.lds_doubles = static_cast<int>(used);
+        m_tail_lds_bytes[dead] = static_cast<int>(used * sizeof(double));
     }
-    // a level in LDS needs every coarser one there too (the loop stops at the first misfit)
-    plan.lds_doubles = static_cast<int>(used);
-    m_tail_lds_bytes = static_cast<int>(used * sizeof(double));
-    mg_tail_reserve_lds(m_tail_lds_bytes);
-    if (!m_plan_dev) ARTSS_HIP(hipMalloc(&m_plan_dev, sizeof(TailPlan)));
-    ARTSS_HIP(hipMemcpy(m_plan_dev, &plan, sizeof(TailPlan), hipMemcpyHostToDevice));
+    mg_tail_reserve_lds(std::max(m_tail_lds_bytes[0], m_tail_lds_bytes[1]));
+    if (!m_plan_dev) ARTSS_HIP(hipMalloc(&m_plan_dev, 2 * sizeof(TailPlan)));
+    ARTSS_HIP(hipMemcpy(m_plan_dev, plans, 2 * sizeof(TailPlan), hipMemcpyHostToDevice));
     m_tail_first = f;
 }
 
@@ -816,10 +826,10 @@ void VCycleMG::vcycle_fused(Field &out, const Field &b) {
     if (f < L) {
         d.timed("mg_tail", f, [&] {
             if (m_precond_call)
-                mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes,
-                        precond_coarse_iter(), 0, 0.0, false, m_pc_tol);
+                mg_tail(d.stream(), m_plan_dev + 1, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes[1],
+                        precond_coarse_iter(), 0, 0.0, true, m_pc_tol);
             else
-                mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes);
+                mg_tail(d.stream(), m_plan_dev, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes[0]);
         });
     }
     for (int level = f; level > 0; --level) {

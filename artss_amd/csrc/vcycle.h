@@ -40,7 +40,7 @@ class VCycleMG : public IPressure {
     void set_schedule(bool fused, long long tail_cells);
     bool fused() const { return m_fused; }
     int tail_first() const { return m_tail_first; }
-    int tail_lds_bytes() const { return m_tail_lds_bytes; }
+    int tail_lds_bytes() const { return m_tail_lds_bytes[0]; }
     // With the fused schedule and the single-workgroup kernel in use, a
     // V-cycle has no host round trip; it is then captured once into a HIP
     // graph (per output/right-hand-side pair) and replayed.
@@ -127,8 +127,8 @@ class VCycleMG : public IPressure {
     bool m_fused = true;
     long long m_tail_cells = 4096;
     int m_tail_first = 0;          // first level of the single-workgroup kernel; n_level: none
-    TailPlan *m_plan_dev = nullptr;
-    int m_tail_lds_bytes = 0;
+    TailPlan *m_plan_dev = nullptr;     // [0] the reference's V-cycle, [1] with the coarsest level skipped
+    int m_tail_lds_bytes[2] = {0, 0};
     // FP32 level arrays of the mixed V-cycle (levels above the tail)
     std::vector<float *> m_err1f, m_tmpf, m_scrf, m_res0f, m_res1f;
     int m_mixed_pre = 0, m_mixed_post = 0;  // sweeps of the mixed V-cycle (0: the XML's)
