@@ -532,27 +532,22 @@ __device__ inline bool is_inner(const LevelDesc &L, long long c, int &i, int &j,
     return i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k);
 }
 
-// ---- one colored Gauss-Seidel iteration in one pass
+// ---- one colored Gauss-Seidel iteration in two launches
 // ColoredGaussSeidelDiffuse.cpp:111-173 (the turbulent form; ev == nullptr
-// gives the constant coefficients of :38-98): the sweep over the cells
-// with i + j + k even, the sweep over the odd ones, the boundary fill of
-// `out` and the update norm's partial sums.  Out of place: src holds the
-// iterate with its fill, dst receives the next iterate on every cell of the
-// array -- fluid cells the update, cells the fill writes the fill, all
-// others src's value -- which is what the reference's in-place sweeps and
-// fill leave in `out`; src is then what its `in` holds.  A workgroup covers
-// a 64x4 column of cells over kCgsPlanes planes.  It evaluates the even
-// update of each plane on its tile and a one-cell rim into LDS (three
-// planes kept), then the odd update of the plane below from those values;
-// a neighbouring workgroup evaluates the same rim cells with the same
-// arithmetic.  Every update is the expression of k_cgs_color, so values
-// are bit-identical to the two colour launches.
-constexpr int kCgsPlanes = 4;  // planes per workgroup (4 and 8 measured equal, 16 and 2 slower)
-constexpr int RX = BX + 2, RY = BY + 2;
-
-__device__ __forceinline__ double cgs_value(const CgsCoef &c, const double *__restrict__ ev, long long index, double r_out,
-                                   double ox, double ox2, double oy, double oy2, double oz, double oz2, double db,
-                                   double *rb_out) {
+// gives the constant coefficients of :38-98), out of place: src holds the
+// iterate with its fill (what the reference's `in` holds after its copy),
+// dst receives the next iterate.  k_cgs_even writes dst on every cell: the
+// update of the fluid cells with i + j + k even (the first colour), src's
+// value everywhere else.  k_cgs_odd then updates the odd fluid cells in
+// place in dst (their neighbours are all even, final after the first
+// launch; their own old value and the ghost and obstacle cells they read
+// come from src), applies the boundary fill of the result and adds the
+// update norm's partial sums.  Every update is
+// the expression of k_cgs_color, so the values are those of the reference's
+// in-place sweeps and fill.
+__device__ __forceinline__ double cgs_value(const CgsCoef &c, const double *__restrict__ ev, long long index,
+                                            double r_out, double ox, double ox2, double oy, double oy2, double oz,
+                                            double oz2, double db, double *rb_out) {
     double aX = c.ax, aY = c.ay, aZ = c.az, bb = c.beta, rb = c.rb;
     if (ev != nullptr) {
         const double e = ev[index];
@@ -566,113 +561,92 @@ __device__ __forceinline__ double cgs_value(const CgsCoef &c, const double *__re
     const double out_h = bb * (c.dsign * db + aX * (ox + ox2) + aY * (oy + oy2) + aZ * (oz + oz2));
     return (1 - c.w) * r_out + c.w * out_h;
 }
-// the even (first-swept) update of fluid cell idx from src
-__device__ __forceinline__ double cgs_even(const LevelDesc &L, const CgsCoef &c, const double *__restrict__ ev,
-                                  const double *__restrict__ src, const double *__restrict__ b, long long idx) {
+// the update of fluid cell idx: neighbours from nb, its own old value from
+// old (the two arrays may be the same)
+__device__ __forceinline__ double cgs_update(const LevelDesc &L, const CgsCoef &c, const double *__restrict__ ev,
+                                             const double *nb, const double *old, const double *__restrict__ b,
+                                             long long idx, double *rb_out) {
     const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
-    return cgs_value(c, ev, idx, src[idx], src[idx + 1], src[idx - 1], src[idx + nj], src[idx - nj], src[idx + nk],
-                     src[idx - nk], b[idx], nullptr);
-}
-__device__ __forceinline__ bool fluid_at(const LevelDesc &L, int i, int j, int k) {
-    return i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] && k <= L.n[2] && !in_obstacle(L, i, j, k);
-}
-// value of cell (i, j, k) after the even sweep
-__device__ __forceinline__ double cgs_after_even(const LevelDesc &L, const CgsCoef &c, const double *__restrict__ ev,
-                                        const double *__restrict__ src, const double *__restrict__ b, int i, int j,
-                                        int k) {
-    const long long idx = lin(L, i, j, k);
-    if (((i + j + k) & 1) == 0 && fluid_at(L, i, j, k)) return cgs_even(L, c, ev, src, b, idx);
-    return src[idx];
+    return cgs_value(c, ev, idx, old[idx], nb[idx + 1], nb[idx - 1], nb[idx + nj], nb[idx - nj], nb[idx + nk],
+                     nb[idx - nk], b[idx], rb_out);
 }
 
-// plane m after the even sweep, over a workgroup's 64x4 tile and its rim
-__device__ __forceinline__ void cgs_even_plane(const LevelDesc &L, const CgsCoef &c, const double *__restrict__ ev,
-                                               const double *__restrict__ src, const double *__restrict__ b, int i0,
-                                               int j0, int m, int tid, double (*slot)[RX]) {
-    for (int q = tid; q < RX * RY; q += BX * BY) {
-        const int ii = q % RX, jj = q / RX;
-        const int i = i0 - 1 + ii, j = j0 - 1 + jj;
-        double v = 0.0;
-        if (i < L.N[0] && j < L.N[1]) v = cgs_after_even(L, c, ev, src, b, i, j, m);
-        slot[jj][ii] = v;
+// the second colour's update of odd fluid cell idx: its neighbours are
+// even and final in dst, except the cells the fill writes (ghosts, obstacle
+// faces), which keep their value from before the sweeps in src because the
+// fill of dst in the same launch may already have overwritten them there;
+// every other cell is the same in both arrays
+__device__ __forceinline__ double cgs_update_odd(const LevelDesc &L, const BCView &V, const CgsCoef &c,
+                                                 const double *__restrict__ ev, const double *dst,
+                                                 const double *__restrict__ src, const double *__restrict__ b,
+                                                 long long idx, double *rb_out) {
+    const long long nj = L.N[0], nk = static_cast<long long>(L.N[0]) * L.N[1];
+    auto at = [&](long long n) { return dev::written(V, n) ? src[n] : dst[n]; };
+    return cgs_value(c, ev, idx, src[idx], at(idx + 1), at(idx - 1), at(idx + nj), at(idx - nj), at(idx + nk),
+                     at(idx - nk), b[idx], rb_out);
+}
+
+__global__ void k_cgs_even(LevelDesc L, int kz, CgsCoef c, const double *__restrict__ ev,
+                           const double *__restrict__ b, const double *__restrict__ src, double *__restrict__ dst,
+                           const int *gate) {
+    if (gate != nullptr && *gate) return;  // uniform over the grid
+    int bx, by, bz;
+    main_block(static_cast<int>(gridDim.z), bx, by, bz);
+    const int i = bx * BX + static_cast<int>(threadIdx.x);
+    const int j = by * BY + static_cast<int>(threadIdx.y);
+    if (i >= L.N[0] || j >= L.N[1]) return;
+    for (int q = 0; q < kz; q++) {
+        const int k = bz * kz + q;
+        if (k >= L.N[2]) break;
+        const long long idx = lin(L, i, j, k);
+        const bool upd = ((i + j + k) & 1) == 0 && i >= 1 && j >= 1 && k >= 1 && i <= L.n[0] && j <= L.n[1] &&
+                         k <= L.n[2] && !in_obstacle(L, i, j, k);
+        dst[idx] = upd ? cgs_update(L, c, ev, src, src, b, idx, nullptr) : src[idx];
     }
 }
 
-__global__ void __launch_bounds__(BX *BY) k_cgs_iter(LevelDesc L, BCView V, int main_z, int planes, CgsCoef c,
-                                                      const double *__restrict__ ev, const double *__restrict__ b,
-                                                      const double *__restrict__ src, double *__restrict__ dst,
-                                                      double *__restrict__ partial, const int *gate) {
+__global__ void k_cgs_odd(LevelDesc L, BCView V, int main_z, int kz, CgsCoef c, const double *__restrict__ ev,
+                          const double *__restrict__ b, const double *__restrict__ src, double *dst,
+                          double *__restrict__ partial, const int *gate) {
     if (gate != nullptr && *gate) return;  // uniform over the grid
     if (static_cast<int>(blockIdx.z) >= main_z) {
         // fill items: the value the iteration gives the cell each write reads
         auto val = [&](long long q) -> double {
             int i, j, k;
-            if (!is_inner(L, q, i, j, k)) return src[q];
-            if (((i + j + k) & 1) == 0) return cgs_even(L, c, ev, src, b, q);
-            // the six neighbours after the even sweep (one evaluation in a
-            // loop keeps the item code small)
-            double nb[6];
-#pragma unroll 1
-            for (int d = 0; d < 6; d++) {
-                const int a = d >> 1, sgn = (d & 1) ? -1 : 1;
-                const double x = cgs_after_even(L, c, ev, src, b, i + (a == 0 ? sgn : 0), j + (a == 1 ? sgn : 0),
-                                                k + (a == 2 ? sgn : 0));
-                nb[0] = d == 0 ? x : nb[0];
-                nb[1] = d == 1 ? x : nb[1];
-                nb[2] = d == 2 ? x : nb[2];
-                nb[3] = d == 3 ? x : nb[3];
-                nb[4] = d == 4 ? x : nb[4];
-                nb[5] = d == 5 ? x : nb[5];
-            }
-            return cgs_value(c, ev, q, src[q], nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], b[q], nullptr);
+            if (!is_inner(L, q, i, j, k) || ((i + j + k) & 1) == 0) return dst[q];
+            return cgs_update_odd(L, V, c, ev, dst, src, b, q, nullptr);
         };
         dev::bc_item<double, decltype(val), true>(L, V, item_index(main_z), dst, val);
         return;
     }
-    __shared__ double even[3][RY][RX];
     int bx, by, bz;
     main_block(main_z, bx, by, bz);
-    const int i0 = 1 + bx * BX, j0 = 1 + by * BY, k0 = 1 + bz * planes;
-    const int k1 = min(k0 + planes, L.n[2] + 1);
-    const int tx = static_cast<int>(threadIdx.x), ty = static_cast<int>(threadIdx.y), tid = ty * BX + tx;
-    // plane m after the even sweep over the tile and its rim, into slot m % 3
-    auto even_plane = [&](int m) { cgs_even_plane(L, c, ev, src, b, i0, j0, m, tid, even[m % 3]); };
-    even_plane(k0 - 1);
-    even_plane(k0);
+    const int i = 1 + bx * BX + static_cast<int>(threadIdx.x);
+    const int j = 1 + by * BY + static_cast<int>(threadIdx.y);
     double sq = 0.0;
-    const int i = i0 + tx, j = j0 + ty;
-    for (int k = k0; k < k1; k++) {
-        even_plane(k + 1);
-        __syncthreads();
-        if (i <= L.n[0] && j <= L.n[1]) {
+    if (i <= L.n[0] && j <= L.n[1]) {
+        for (int q = 0; q < kz; q++) {
+            const int k = 1 + bz * kz + q;
+            if (k > L.n[2]) break;
+            if (in_obstacle(L, i, j, k)) continue;
             const long long idx = lin(L, i, j, k);
-            const double old = src[idx];
-            if (!in_obstacle(L, i, j, k)) {
-                const double(*lo)[RX] = even[(k - 1) % 3];
-                const double(*mid)[RX] = even[k % 3];
-                const double(*hi)[RX] = even[(k + 1) % 3];
-                double v, rb;
-                if (((i + j + k) & 1) == 0) {
-                    v = mid[ty + 1][tx + 1];
-                    rb = c.rb;
-                    if (ev != nullptr) {
-                        const double e = ev[idx];
-                        rb = (1. + 2. * ((c.D + e) * c.dt * c.rdx * c.rdx + (c.D + e) * c.dt * c.rdy * c.rdy +
-                                         (c.D + e) * c.dt * c.rdz * c.rdz));
-                    }
-                } else {
-                    v = cgs_value(c, ev, idx, old, mid[ty + 1][tx + 2], mid[ty + 1][tx], mid[ty + 2][tx + 1],
-                                  mid[ty][tx + 1], hi[ty + 1][tx + 1], lo[ty + 1][tx + 1], b[idx], &rb);
-                }
+            double v, rb;
+            if (((i + j + k) & 1) == 1) {
+                v = cgs_update_odd(L, V, c, ev, dst, src, b, idx, &rb);
                 dst[idx] = v;
-                dev::face_ghosts(L, V, dst, idx, i, j, k, v);
-                const double r = rb * (v - old);
-                sq += r * r;
-            } else if (!dev::written(V, idx)) {
-                dst[idx] = old;
+            } else {
+                v = dst[idx];
+                rb = c.rb;
+                if (ev != nullptr) {
+                    const double e = ev[idx];
+                    rb = (1. + 2. * ((c.D + e) * c.dt * c.rdx * c.rdx + (c.D + e) * c.dt * c.rdy * c.rdy +
+                                     (c.D + e) * c.dt * c.rdz * c.rdz));
+                }
             }
+            dev::face_ghosts(L, V, dst, idx, i, j, k, v);
+            const double r = rb * (v - src[idx]);
+            sq += r * r;
         }
-        __syncthreads();
     }
     write_partial(partial, sq);
 }
@@ -1143,17 +1117,17 @@ void update_norm_partials(hipStream_t s, const LevelDesc &L, const double *out, 
     k_update_norm<<<inner_grid(L), block(), 0, s>>>(L, c, out, in, ev, partial, gate);
 }
 
-int cgs_planes() { return kCgsPlanes; }
+// cells per thread along z of the two launches
+constexpr int kCgsZ = 4;
 int cgs_iter_partials(const LevelDesc &L) {
-    const int planes = cgs_planes();
-    return ((L.n[0] + BX - 1) / BX) * ((L.n[1] + BY - 1) / BY) * ((L.n[2] + planes - 1) / planes);
+    return ((L.n[0] + BX - 1) / BX) * ((L.n[1] + BY - 1) / BY) * ((L.n[2] + kCgsZ - 1) / kCgsZ);
 }
 void cgs_iter(hipStream_t s, const LevelDesc &L, const BCView &V, const CgsCoef &c, const double *ev, const double *b,
               const double *src, double *dst, double *partial, const int *gate) {
-    const int planes = cgs_planes();
-    const dim3 g((L.n[0] + BX - 1) / BX, (L.n[1] + BY - 1) / BY, (L.n[2] + planes - 1) / planes);
-    k_cgs_iter<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), planes, c, ev, b,
-                                                                         src, dst, partial, gate);
+    k_cgs_even<<<zbatch(full_grid(L), kCgsZ), block(), 0, s>>>(L, kCgsZ, c, ev, b, src, dst, gate);
+    const dim3 g = zbatch(inner_grid(L), kCgsZ);
+    k_cgs_odd<<<with_items(g, dev::bc_items_grid(V)), block(), 0, s>>>(L, V, static_cast<int>(g.z), kCgsZ, c, ev, b,
+                                                                        src, dst, partial, gate);
 }
 void swap_arrays(hipStream_t s, double *a, double *b, long long n) {
     k_swap<<<grid1d(n, 256), 256, 0, s>>>(a, b, n);
