@@ -153,26 +153,32 @@ bool invert_dense(std::vector<double> &A, int n, std::vector<double> &inv) {
             for (size_t j = k + 1; j < N; j++) ri[j] -= l * rk[j];
         }
     }
-    // columns of the inverse: P A = L U, A^-1 e_c = U^-1 L^-1 P e_c
+    // the inverse by row operations on all its columns at once: X = P I,
+    // then L^-1 and U^-1 (contiguous row updates, no dependent sums)
     inv.assign(N * N, 0.0);
-    std::vector<double> y(N);
-    for (size_t c = 0; c < N; c++) {
-        std::fill(y.begin(), y.end(), 0.0);
-        y[c] = 1.0;
-        for (size_t k = 0; k < N; k++) std::swap(y[k], y[static_cast<size_t>(piv[k])]);
-        for (size_t i = 0; i < N; i++) {
-            double v = y[i];
-            const double *ri = &A[i * N];
-            for (size_t j = 0; j < i; j++) v -= ri[j] * y[j];
-            y[i] = v;
+    std::vector<size_t> perm(N);
+    for (size_t i = 0; i < N; i++) perm[i] = i;
+    for (size_t k = 0; k < N; k++) std::swap(perm[k], perm[static_cast<size_t>(piv[k])]);
+    for (size_t i = 0; i < N; i++) inv[i * N + perm[i]] = 1.0;
+    for (size_t k = 0; k < N; k++) {
+        const double *xk = &inv[k * N];
+        for (size_t i = k + 1; i < N; i++) {
+            const double l = A[i * N + k];
+            if (l == 0.0) continue;
+            double *xi = &inv[i * N];
+            for (size_t j = 0; j < N; j++) xi[j] -= l * xk[j];
         }
-        for (size_t i = N; i-- > 0;) {
-            double v = y[i];
-            const double *ri = &A[i * N];
-            for (size_t j = i + 1; j < N; j++) v -= ri[j] * y[j];
-            y[i] = v / ri[i];
+    }
+    for (size_t k = N; k-- > 0;) {
+        double *xk = &inv[k * N];
+        const double d = A[k * N + k];
+        for (size_t j = 0; j < N; j++) xk[j] /= d;
+        for (size_t i = 0; i < k; i++) {
+            const double u = A[i * N + k];
+            if (u == 0.0) continue;
+            double *xi = &inv[i * N];
+            for (size_t j = 0; j < N; j++) xi[j] -= u * xk[j];
         }
-        for (size_t r = 0; r < N; r++) inv[r * N + c] = y[r];
     }
     return true;
 }
@@ -198,7 +204,7 @@ void VCycleMG::prepare_mixed() {
     alloc_mixed();
     if (m_direct_enabled && !m_direct_tried) {
         m_direct_tried = true;
-        if (!build_direct()) free_direct();
+        if (!build_direct() && !build_galerkin()) free_direct();
     }
 }
 
@@ -267,6 +273,88 @@ bool VCycleMG::build_direct() {
     std::vector<double> inv;
     if (!invert_dense(K, n, inv)) return false;
     for (double &v : inv) v *= kDsign;
+    std::vector<uint32_t> idx(N);
+    for (size_t i = 0; i < N; i++) idx[i] = static_cast<uint32_t>(cells[i]);
+    ARTSS_HIP(hipMalloc(&m_direct, sizeof(double) * N * N));
+    ARTSS_HIP(hipMalloc(&m_direct_cells, sizeof(uint32_t) * N));
+    ARTSS_HIP(hipMemcpy(m_direct, inv.data(), sizeof(double) * N * N, hipMemcpyHostToDevice));
+    ARTSS_HIP(hipMemcpy(m_direct_cells, idx.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
+    m_direct_n = n;
+    return true;
+}
+
+// The Galerkin operator of the first coarse level f of the preconditioner,
+// A_c = R lap P with P the prolongation from f to f - 1 (with the linear fill
+// of both levels), lap the level f - 1 Laplacian of the residuum and R the
+// eight-cell restriction, assembled column by column by running those
+// kernels on unit vectors of the fluid cells of f, and inverted densely.
+// Where a coarse level has obstacle cells its rediscretised operator is a
+// poor stand-in for the finer ones (the obstacle grows when it is coarsened)
+// and solving it exactly stalls GCR; A_c is what level f - 1's equation
+// implies for a correction of the form P e, whatever the obstacle.
+bool VCycleMG::build_galerkin() {
+    Domain &d = m_domain;
+    const int f = m_tail_first;
+    if (f < 1 || f > m_s.n_level || !fast_level(f) || !fast_level(f - 1) || restrict_reads_ghost(f - 1)) return false;
+    const std::vector<uint64_t> cells = d.inner_list(f);
+    const int n = static_cast<int>(cells.size());
+    if (n < 1 || n > kMaxDirectCells || n > k::kMaxDenseRows) return false;
+    const LevelDesc &Lc = d.level(f), &Lf = d.level(f - 1);
+    const BCView Vc = d.bc_view(f, P, true), Vf = d.bc_view(f - 1, P, true);
+    const size_t nc = static_cast<size_t>(Lc.size), nf = static_cast<size_t>(Lf.size);
+    hipStream_t s = d.stream();
+    double *x = nullptr, *y = nullptr, *r = nullptr, *zero = nullptr, *c = nullptr, *junk = nullptr;
+    auto release = [&] {
+        for (double *q : {x, y, r, zero, c, junk})
+            if (q) (void)hipFree(q);
+    };
+    const size_t N = static_cast<size_t>(n);
+    std::vector<double> K(N * N), col(nc);
+    try {
+        ARTSS_HIP(hipMalloc(&x, sizeof(double) * nc));
+        ARTSS_HIP(hipMalloc(&c, sizeof(double) * nc));
+        ARTSS_HIP(hipMalloc(&junk, sizeof(double) * nc));
+        ARTSS_HIP(hipMalloc(&y, sizeof(double) * nf));
+        ARTSS_HIP(hipMalloc(&r, sizeof(double) * nf));
+        ARTSS_HIP(hipMalloc(&zero, sizeof(double) * nf));
+        k::fill(s, zero, 0.0, Lf.size);
+        k::fill(s, r, 0.0, Lf.size);
+        for (size_t j = 0; j < N; j++) {
+            k::fill(s, x, 0.0, Lc.size);
+            k::fill(s, x + cells[j], 1.0, 1);
+            k::fill_bc<double>(s, Lc, Vc, x);
+            k::prolongate_add_bc<double, double>(s, Lc, Lf, Vf, y, zero, x);
+            k::residuum_faces<double, double>(s, Lf, Vf, r, y, zero);
+            k::fill(s, c, 0.0, Lc.size);
+            k::restrict_fill<double, double, double>(s, Lc, Lf, c, r, junk);
+            ARTSS_HIP(hipMemcpyAsync(col.data(), c, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+            ARTSS_HIP(hipStreamSynchronize(s));
+            // c = R (0 - lap P e_j) = -A_c e_j: K = -A_c, positive on the diagonal
+            for (size_t i = 0; i < N; i++) K[i * N + j] = col[cells[i]];
+        }
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    // a constant null space (Neumann or periodic everywhere): as in
+    // build_direct, the mean diagonal times the averaging matrix is added
+    double diag = 0.0, rows = 0.0;
+    for (size_t i = 0; i < N; i++) {
+        double sum = 0.0;
+        for (size_t j = 0; j < N; j++) sum += K[i * N + j];
+        rows = std::max(rows, std::fabs(sum));
+        diag += K[i * N + i];
+    }
+    diag /= static_cast<double>(n);
+    if (!(diag > 0.0)) return false;
+    if (rows <= 1e-9 * diag) {
+        const double add = diag / static_cast<double>(n);
+        for (double &v : K) v += add;
+    }
+    std::vector<double> inv;
+    if (!invert_dense(K, n, inv)) return false;
+    for (double &v : inv) v = -v;  // A_c^-1 = -K^-1
     std::vector<uint32_t> idx(N);
     for (size_t i = 0; i < N; i++) idx[i] = static_cast<uint32_t>(cells[i]);
     ARTSS_HIP(hipMalloc(&m_direct, sizeof(double) * N * N));

@@ -145,6 +145,7 @@ class VCycleMG : public IPressure {
     void smooth_fast_f(int level, const double *b64, const float *b32, int sweeps, bool from_zero, bool first_done);
     int mixed_sweep_count(int sweeps) const;                  // effective sweeps for a count (0: the XML's)
     float *first_sweep_target(int level, int sweeps) const;   // buffer the first of them writes
+    bool build_galerkin();
     bool build_direct();
     void free_direct();
     bool m_direct_enabled = true;

@@ -88,8 +88,11 @@ def test_direct_coarse_solve_meets_target(obstacles, open_top):
     """The dense exact coarse solve (1024 cells at the 16x8x8 level; a
     constant null space in the closed box) against the single-workgroup
     V-cycle: both reach the target, the fields agree to what it allows and
-    the exact solve needs fewer iterations.  With an obstacle cell on that
-    level the exact solve does not apply and the V-cycle runs unchanged."""
+    the exact solve needs fewer iterations.  With obstacle cells on that
+    level the exact solve is of the Galerkin operator R lap P rather than
+    the rediscretised one.  In the closed box with the obstacle the
+    restricted right-hand sides are not consistent and neither coarse solve
+    reaches 1e-9; every step then reports exactly what it reached."""
     xml = cases.plume(n=(64, 32, 32), X=(0.0, 2.0), levels=4, steps=4, dt=0.01, obstacles=obstacles, hrr=3.0,
                       source_x=1.0, open_top=open_top)
     tol = 1e-9
@@ -99,18 +102,15 @@ def test_direct_coarse_solve_meets_target(obstacles, open_top):
     cyc_d = [c["vcycles"] for c in c_d]
     errs = {f: float(np.abs(f_d[f] - f_t[f]).max() / max(np.abs(f_t[f]).max(), 1e-30)) for f in f_t}
     print("tail", cyc_t, "direct", cyc_d, "residuals", [c["residual"] for c in c_d], "rel err", errs)
-    if obstacles:
-        # (the closed box with this obstacle does not reach 1e-9 by either
-        # coarse solve: its restricted right-hand sides are not consistent)
-        assert c_d == c_t
-        for f in f_t:
-            assert np.array_equal(f_d[f], f_t[f]), f
-    else:
-        for c in c_d:
-            assert 0 <= c["residual"] <= tol
-        assert sum(cyc_d) < sum(cyc_t)
-        for f in f_t:
-            assert errs[f] < 1e-5, f
+    for f in f_d:
+        assert np.all(np.isfinite(f_d[f])), f
+    if obstacles and not open_top:
+        return
+    for c in c_d:
+        assert 0 <= c["residual"] <= tol
+    assert sum(cyc_d) < sum(cyc_t)
+    for f in f_t:
+        assert errs[f] < 1e-5, f
 
 
 def test_gcr_batch_respects_iteration_cap():
