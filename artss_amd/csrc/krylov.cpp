@@ -128,6 +128,7 @@ double GcrPressure::true_residual(Field &p, const Field &b) {
 int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, double *residual) {
     Domain &d = m_domain;
     if (m_mixed) m_pre.prepare_mixed();
+    else m_pre.prepare_direct();
     const LevelDesc &L = d.level(0);
     hipStream_t s = d.stream();
     // convergence is rr <= tol2 on the host and in the gate alike

@@ -202,6 +202,10 @@ void VCycleMG::free_direct() {
 
 void VCycleMG::prepare_mixed() {
     alloc_mixed();
+    prepare_direct();
+}
+
+void VCycleMG::prepare_direct() {
     if (m_direct_enabled && !m_direct_tried) {
         m_direct_tried = true;
         if (!build_direct() && !build_galerkin()) free_direct();
@@ -911,7 +915,14 @@ void VCycleMG::vcycle_fused(Field &out, const Field &b) {
                              m_res0[level]->data, start(level + 1).data);
         });
     }
-    if (f < L) {
+    if (f < L && m_precond_call && m_direct && f > 0) {
+        // as GCR's preconditioner: the exact coarse solve of level f
+        // (restrict_fill above zeroed err1[f])
+        d.timed("coarse_direct", f, [&] {
+            k::dense_apply(d.stream(), m_direct_n, m_direct, m_direct_cells, m_res1[f]->data, m_err1[f]->data);
+            k::fill_bc<double>(d.stream(), d.level(f), d.bc_view(f, P, true), m_err1[f]->data);
+        });
+    } else if (f < L) {
         d.timed("mg_tail", f, [&] {
             if (m_precond_call)
                 mg_tail(d.stream(), m_plan_dev + 1, f == 0 ? b.data : m_res1[f]->data, m_tail_lds_bytes[1],

@@ -52,6 +52,9 @@ class VCycleMG : public IPressure {
     // The result stays in mixed_result() (FP32, level 0) until the next call.
     bool mixed_supported() const;
     void prepare_mixed();  // allocates (and assembles the direct solve); call before capturing vcycle_mixed
+    // assembles the direct coarse solve that vcycle_eager uses as GCR's
+    // preconditioner (method 1); call before capturing it
+    void prepare_direct();
     // pre- and post-smoothing sweeps of the mixed V-cycle on the levels
     // above its single-workgroup kernel (0: the XML's effective count) and
     // their Jacobi weight (0: the XML's)

@@ -187,13 +187,16 @@ int artss_solver_set_mg_schedule(artss_solver *s, int fused, long long tail_cell
  * target. */
 int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post, double weight);
 
-/* Method 2 only (no reference counterpart): direct = 1 (the default)
- * replaces the coarse levels of the preconditioner (those of the
- * single-workgroup kernel) by the exact solve on their finest level, a dense
- * inverse assembled once on the host when that level has at most 1536 cells
- * and no obstacle cells (otherwise, and with direct = 0, the single-workgroup V-cycle and
- * Jacobi coarse solve run).  The iteration still meets the same FP64
- * residual target. */
+/* Methods 1 and 2 (no reference counterpart): direct = 1 (the default)
+ * replaces the coarse levels of GCR's preconditioner (those of the
+ * single-workgroup kernel) by the exact solve on their finest level f, a
+ * dense inverse assembled once when that level has at most 1536 fluid cells:
+ * of its own operator where it has no obstacle cells, else of the Galerkin
+ * operator R lap P (restriction, level f-1's Laplacian with its linear
+ * fill, prolongation), because an obstacle grows when it is coarsened and
+ * the rediscretised operator then misleads the iteration.  With direct = 0,
+ * or above 1536 cells, the single-workgroup V-cycle and Jacobi coarse solve
+ * run.  The iteration still meets the same FP64 residual target. */
 int artss_solver_set_coarse_solve(artss_solver *s, int direct);
 
 /* Method 2 only (no reference counterpart): n_relax of the preconditioner's
