@@ -79,9 +79,9 @@ struct CgsCoef {
     double ax, ay, az, beta, rb, dsign, w, D, dt, rdx, rdy, rdz;
 };
 // one whole iteration (even sweep, odd sweep, fill of the result, norm
-// partials) out of place from src into dst, in two launches; the fill must
-// be idempotent (Domain::bc_idempotent) and write every ghost cell
-// (BCView::shell_filled).  cgs_iter_partials: the partial sums it writes.
+// partials) out of place from src into dst in one launch; the fill must be
+// idempotent (Domain::bc_idempotent).  cgs_iter_partials: the partial sums
+// it writes.
 int cgs_iter_partials(const LevelDesc &L);
 void cgs_iter(hipStream_t s, const LevelDesc &L, const BCView &V, const CgsCoef &c, const double *ev, const double *b,
               const double *src, double *dst, double *partial, const int *gate);

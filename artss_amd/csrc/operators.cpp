@@ -134,7 +134,7 @@ void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b,
     const int ft = out.get_type();
     if (dm.bc_idempotent(0, ft)) {
         const BCView V = dm.bc_view(0, ft);
-        if (V.shell_filled && k::cgs_iter_partials(L) <= dm.partial_capacity()) {
+        if (k::cgs_iter_partials(L) <= dm.partial_capacity()) {
             const k::CgsCoef c{ax, ay, az, beta, rb, 1.0, m_settings.w, D, dt, rdx, rdy, rdz};
             run_one_pass(out, in, b, ev, V, c);
             return;

@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 BX, BY = 64, 4
 
 # kernels whose launches return at once when their loop's device test has stopped
-GATED = {"k_copy", "k_cgs_color", "k_cgs_even", "k_cgs_odd", "k_update_norm", "k_reduce_gate", "k_gcr_update", "k_gcr_coef", "k_sum_gate",
+GATED = {"k_copy", "k_cgs_color", "k_cgs_iter", "k_update_norm", "k_reduce_gate", "k_gcr_update", "k_gcr_coef", "k_sum_gate",
          "k_bc_faces", "k_bc_domain"}
 
 
@@ -119,22 +119,12 @@ def bytes_of(kernel, args, gx, gy, gz, L, turbulent=True, tail_cells=4096):
         per = 8 + 8 + 8 + ev + 8
         return per * L.inner(level) // 2, "one color: read+write it, read b%s and the other color, level %d" % (
             ", nu_t" if ev else "", level)
-    if kernel == "k_cgs_even":
-        level, kind = L.level_of(gx, gy)
-        if level is None:
-            return None, "level unknown"
-        # the whole array read and written, b (and nu_t) at the first colour's cells
-        return 16 * L.size(level) + (8 + ev) * L.inner(level) // 2, \
-            "first colour: read src, write dst on every cell, read b%s on its cells, level %d" % (
-                ", nu_t" if ev else "", level)
-    if kernel == "k_cgs_odd":
-        level, kind = L.level_of(gx, gy)
-        if level is None:
-            return None, "level unknown"
-        # dst and src read on the fluid cells, b (and nu_t) read and dst written on the second colour's cells
-        return 16 * L.inner(level) + (8 + ev + 8) * L.inner(level) // 2, \
-            "second colour, fill, norm: read dst and src, read b%s and write dst on its cells, level %d" % (
-                ", nu_t" if ev else "", level)
+    if kernel == "k_cgs_iter":
+        level = 0  # the diffusion runs on the finest level only
+        # one whole iteration: src read and dst written on every cell, b (and
+        # nu_t) read on the fluid cells
+        return 16 * L.size(level) + (8 + ev) * L.inner(level), \
+            "one iteration: read src, write dst on every cell, read b%s, level %d" % (", nu_t" if ev else "", level)
     if kernel == "k_swap":
         l = L.level_of_1d(gx)
         if l is None:
