@@ -94,6 +94,7 @@ def lib():
         "artss_solver_set_pressure_method": (C.c_int, [vp, C.c_int, C.c_int]),
         "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
         "artss_solver_set_warm_start": (C.c_int, [vp, C.c_int, vp, C.c_int]),
+        "artss_solver_set_warm_start_depth": (C.c_int, [vp, C.c_int, C.c_int]),
         "artss_solver_set_mixed_sweeps": (C.c_int, [vp, C.c_int, C.c_int, C.c_double]),
         "artss_solver_set_coarse_solve": (C.c_int, [vp, C.c_int]),
         "artss_solver_set_mixed_tail": (C.c_int, [vp, C.c_int]),
@@ -451,10 +452,18 @@ class Solver:
         sweeps of the GCR preconditioner's coarse solve (0: five)."""
         _check(self.L.artss_solver_set_pressure_method(self.h, int(method), int(coarse_max_iter)))
 
-    def set_warm_start(self, mode, net=None, record=False):
+    def set_warm_start(self, mode, net=None, record=False, depth=None):
         """Start of the GCR pressure iteration: 0 the previous p, 1 linear extrapolation in
-        time, 2 extrapolation plus the correction of the warm-start UNet `net`; `record`
-        keeps each step's network input and target for warm_sample()."""
+        time, 2 extrapolation plus the correction of the warm-start UNet `net`, 3 the
+        combination of the last `depth` (default 4) increments of the solution closest to
+        the solution in the energy norm (Galerkin projection); `record` keeps each step's
+        network input and target for warm_sample()."""
+        if mode == 3:
+            if net is not None or record:
+                raise ArtssError("the projection start takes no network and records nothing")
+            _check(self.L.artss_solver_set_warm_start_depth(self.h, 3, 4 if depth is None else int(depth)))
+            self._warm_net = None
+            return
         _check(self.L.artss_solver_set_warm_start(self.h, int(mode), net.h if net is not None else None,
                                                   1 if record else 0))
         self._warm_net = net  # the solver holds a pointer to it

@@ -684,6 +684,17 @@ int artss_solver_set_warm_start(artss_solver *s, int mode, artss_unet *net, int 
     });
 }
 
+int artss_solver_set_warm_start_depth(artss_solver *s, int mode, int depth) {
+    return guard([&] {
+        require(s, "null argument");
+        WarmStart w;
+        w.mode = mode;
+        w.depth = depth;
+        s->solver->set_warm_start(w);
+        return ARTSS_OK;
+    });
+}
+
 int artss_solver_warm_sample(artss_solver *s, float *x, float *t, int *available) {
     return guard([&] {
         require(s && x && t && available, "null argument");

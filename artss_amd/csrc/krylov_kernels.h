@@ -67,5 +67,27 @@ void warm_apply(hipStream_t s, const LevelDesc &L, const double *sum, double flu
 void warm_target(hipStream_t s, const LevelDesc &L, const double *sum, double fluid_cells, const double *p_new,
                  const double *p_start, const double *p_prev, float *t);
 
+// ---- projection start (solver.cpp): p_n + sum_s alpha_s D_s over the last
+// increments D_s of the solution kept in a ring of slots, alpha from the
+// Galerkin condition (D_s, b - lap(p_n) - sum_t alpha_t W_t) = 0 with W_t the
+// image of D_t: the combination closest to the solution in the energy norm
+constexpr int kProjMax = 8;  // slots
+struct ProjRing {
+    double *d[kProjMax];  // increments, fluid cells
+    double *w[kProjMax];  // their images, fluid cells
+};
+// slot `fresh` := p - p_prev and its image; partial sums (blocks_inner each)
+// of (D_fresh, W_s) for q = s and (D_s, b - lap p) for q = kProjMax + s,
+// over the slots whose bit is set in `valid` (which includes `fresh`)
+void proj_update(hipStream_t s, const LevelDesc &L, const double *p, const double *p_prev, const double *b,
+                 const ProjRing &ring, int fresh, unsigned valid, double *partial);
+// G (kProjMax x kProjMax, by slot) gets row and column `fresh` of
+// -(D_s, W_t); alpha (by slot) solves the projection over the valid slots
+void proj_solve(hipStream_t s, const double *partial, int blocks, int fresh, unsigned valid, double *G,
+                double *alpha);
+// p += sum_s alpha_s D_s on the fluid cells
+void proj_apply(hipStream_t s, const LevelDesc &L, const ProjRing &ring, int fresh, unsigned valid,
+                const double *alpha, double *p);
+
 }  // namespace kk
 }  // namespace artss

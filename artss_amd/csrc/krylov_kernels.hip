@@ -316,6 +316,119 @@ __global__ void k_warm_target(LevelDesc L, const double *__restrict__ sum, doubl
     t[box_index(L, i, j, k)] = a;
 }
 
+// ---- projection start: the new increment D = p - p_prev and its image
+// W = lap(p) - lap(p_prev) (both arrays carry their fill, which is affine, so
+// W is the image of D under the linear part of the operator) into slot
+// `fresh` of the ring; partials of (D, W_s) and (D_s, r) for every valid
+// slot s, r = b - lap(p) the residual of p for the new right-hand side
+__global__ void k_proj_update(LevelDesc L, const double *__restrict__ p, const double *__restrict__ p_prev,
+                              const double *__restrict__ b, ProjRing ring, int fresh, unsigned valid,
+                              double *__restrict__ partial) {
+    int i, j, k;
+    double v[2 * kProjMax];
+#pragma unroll
+    for (int q = 0; q < 2 * kProjMax; q++) v[q] = 0.0;
+    if (inner_at(L, i, j, k)) {
+        const long long idx = dev::lin(L, i, j, k);
+        const double lp = dev::laplace(L, p, idx);
+        const double w = lp - dev::laplace(L, p_prev, idx);
+        const double r = b[idx] - lp;
+        const double d = p[idx] - p_prev[idx];
+        ring.d[fresh][idx] = d;
+        ring.w[fresh][idx] = w;
+#pragma unroll
+        for (int q = 0; q < kProjMax; q++) {
+            if (!((valid >> q) & 1u)) continue;
+            v[q] = d * (q == fresh ? w : ring.w[q][idx]);
+            v[kProjMax + q] = (q == fresh ? d : ring.d[q][idx]) * r;
+        }
+    }
+    write_partials<2 * kProjMax>(v, partial);
+}
+// the sums, row and column `fresh` of H = -(D_s, W_t) (slots; the operator
+// is symmetric and negative definite, so H is symmetric and positive
+// definite on independent increments), and the coefficients alpha (slots)
+// of the Galerkin condition (D_s, r - sum_t alpha_t W_t) = 0 over the valid
+// slots, newest first, by Cholesky: the combination closest to the solution
+// in the energy norm.  A slot whose increment is within 1e-5 (relative, in
+// that norm) of the span of the newer ones gets no coefficient.
+__global__ void k_proj_solve(const double *__restrict__ partial, int blocks, int fresh, unsigned valid,
+                             double *__restrict__ G, double *__restrict__ alpha) {
+    __shared__ double part[2 * kProjMax][16];
+    double acc[2 * kProjMax];
+#pragma unroll
+    for (int q = 0; q < 2 * kProjMax; q++) acc[q] = 0.0;
+    for (int t = threadIdx.x; t < blocks; t += 1024)
+#pragma unroll
+        for (int q = 0; q < 2 * kProjMax; q++) acc[q] += partial[q * blocks + t];
+#pragma unroll
+    for (int q = 0; q < 2 * kProjMax; q++) {
+        double x = acc[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((threadIdx.x & 63) == 0) part[q][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    // (the one thread's work arrays live in LDS: indexed at run time, private
+    // arrays would go to per-lane scratch memory)
+    __shared__ double g[kProjMax], y[kProjMax], Lc[kProjMax][kProjMax];
+    __shared__ int order[kProjMax];
+    __shared__ bool keep[kProjMax];
+    for (int q = 0; q < 2 * kProjMax; q++) {
+        double x = 0.0;
+        for (int w = 0; w < 16; w++) x += part[q][w];
+        if (q < kProjMax) {
+            G[fresh * kProjMax + q] = -x;
+            G[q * kProjMax + fresh] = -x;
+        } else {
+            g[q - kProjMax] = -x;
+        }
+    }
+    // valid slots by age, newest first
+    int m = 0;
+    for (int a = 0; a < kProjMax; a++) {
+        const int q = (fresh - a + kProjMax) % kProjMax;
+        if ((valid >> q) & 1u) order[m++] = q;
+    }
+    for (int a = 0; a < m; a++) {
+        double dsum = G[order[a] * kProjMax + order[a]];
+        for (int c = 0; c < a; c++) dsum -= Lc[a][c] * Lc[a][c];
+        keep[a] = dsum > 1e-10 * G[order[a] * kProjMax + order[a]] && dsum > 0.0;
+        const double piv = keep[a] ? sqrt(dsum) : 0.0;
+        Lc[a][a] = piv;
+        for (int e = a + 1; e < m; e++) {
+            double x = G[order[e] * kProjMax + order[a]];
+            for (int c = 0; c < a; c++) x -= Lc[e][c] * Lc[a][c];
+            Lc[e][a] = keep[a] ? x / piv : 0.0;
+        }
+    }
+    for (int a = 0; a < m; a++) {
+        double x = g[order[a]];
+        for (int c = 0; c < a; c++) x -= Lc[a][c] * y[c];
+        y[a] = keep[a] ? x / Lc[a][a] : 0.0;
+    }
+    for (int q = 0; q < kProjMax; q++) alpha[q] = 0.0;
+    for (int a = m - 1; a >= 0; a--) {
+        double x = y[a];
+        for (int c = a + 1; c < m; c++) x -= Lc[c][a] * alpha[order[c]];
+        alpha[order[a]] = keep[a] ? x / Lc[a][a] : 0.0;
+    }
+}
+// p += sum over the valid slots (newest first) of alpha_s D_s
+__global__ void k_proj_apply(LevelDesc L, ProjRing ring, int fresh, unsigned valid, const double *__restrict__ alpha,
+                             double *__restrict__ p) {
+    int i, j, k;
+    if (!inner_at(L, i, j, k)) return;
+    const long long idx = dev::lin(L, i, j, k);
+    double x = p[idx];
+#pragma unroll
+    for (int a = 0; a < kProjMax; a++) {
+        const int q = (fresh - a + kProjMax) % kProjMax;
+        if ((valid >> q) & 1u) x += alpha[q] * ring.d[q][idx];
+    }
+    p[idx] = x;
+}
+
 }  // namespace
 
 void warm_sum(hipStream_t s, const LevelDesc &L, const double *p, const double *p_prev, double *partial,
@@ -326,6 +439,18 @@ void warm_sum(hipStream_t s, const LevelDesc &L, const double *p, const double *
 void warm_pack(hipStream_t s, const LevelDesc &L, const double *p, const double *b, const double *p_prev,
                const double *sum, double fluid_cells, double c, float *x) {
     k_warm_pack<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, p, b, p_prev, sum, fluid_cells, c, x);
+}
+void proj_update(hipStream_t s, const LevelDesc &L, const double *p, const double *p_prev, const double *b,
+                 const ProjRing &ring, int fresh, unsigned valid, double *partial) {
+    k_proj_update<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, p, p_prev, b, ring, fresh, valid, partial);
+}
+void proj_solve(hipStream_t s, const double *partial, int blocks, int fresh, unsigned valid, double *G,
+                double *alpha) {
+    k_proj_solve<<<1, 1024, 0, s>>>(partial, blocks, fresh, valid, G, alpha);
+}
+void proj_apply(hipStream_t s, const LevelDesc &L, const ProjRing &ring, int fresh, unsigned valid,
+                const double *alpha, double *p) {
+    k_proj_apply<<<inner_grid(L), dim3(BX, BY), 0, s>>>(L, ring, fresh, valid, alpha, p);
 }
 void warm_apply(hipStream_t s, const LevelDesc &L, const double *sum, double fluid_cells, const float *y,
                 const double *p_prev, double *p) {

@@ -194,12 +194,20 @@ Solver::~Solver() {
             for (double *q : snap->data) (void)hipFree(q);
     for (void *q : {static_cast<void *>(m_p_prev), static_cast<void *>(m_p_start),
                     static_cast<void *>(m_warm_partial), static_cast<void *>(m_warm_sum),
-                    static_cast<void *>(m_warm_x), static_cast<void *>(m_warm_y), static_cast<void *>(m_warm_t)})
+                    static_cast<void *>(m_warm_x), static_cast<void *>(m_warm_y), static_cast<void *>(m_warm_t),
+                    static_cast<void *>(m_proj_G), static_cast<void *>(m_proj_alpha),
+                    static_cast<void *>(m_proj_partial)})
         if (q) (void)hipFree(q);
+    for (int q = 0; q < kk::kProjMax; q++) {
+        if (m_proj.d[q]) (void)hipFree(m_proj.d[q]);
+        if (m_proj.w[q]) (void)hipFree(m_proj.w[q]);
+    }
 }
 
 void Solver::set_warm_start(const WarmStart &w) {
-    if (w.mode < 0 || w.mode > 2) throw config_error("warm start mode must be 0, 1 or 2");
+    if (w.mode < 0 || w.mode > 3) throw config_error("warm start mode must be 0, 1, 2 or 3");
+    if (w.mode == 3 && (w.depth < 1 || w.depth > kk::kProjMax))
+        throw config_error("the projection start keeps 1 to " + std::to_string(kk::kProjMax) + " increments");
     if (w.mode == 2 && w.net == nullptr) throw config_error("the learned warm start needs a network");
     const LevelDesc &L = m_domain->level(0);
     const bool net = w.mode == 2 || w.record;
@@ -212,6 +220,21 @@ void Solver::set_warm_start(const WarmStart &w) {
         ARTSS_HIP(hipMalloc(&m_p_start, sizeof(double) * n));
         ARTSS_HIP(hipMalloc(&m_warm_partial, sizeof(double) * static_cast<size_t>(kk::blocks_inner(L))));
         ARTSS_HIP(hipMalloc(&m_warm_sum, sizeof(double)));
+    }
+    if (w.mode == 3) {
+        for (int q = 0; q < w.depth; q++)
+            if (!m_proj.d[q]) {
+                ARTSS_HIP(hipMalloc(&m_proj.d[q], sizeof(double) * n));
+                ARTSS_HIP(hipMalloc(&m_proj.w[q], sizeof(double) * n));
+            }
+        if (!m_proj_G) {
+            ARTSS_HIP(hipMalloc(&m_proj_G, sizeof(double) * kk::kProjMax * kk::kProjMax));
+            ARTSS_HIP(hipMalloc(&m_proj_alpha, sizeof(double) * kk::kProjMax));
+            ARTSS_HIP(hipMalloc(&m_proj_partial,
+                                sizeof(double) * 2 * kk::kProjMax * static_cast<size_t>(kk::blocks_inner(L))));
+        }
+        // the increments restart from the previous solution
+        proj_reset();
     }
     if (net && !m_warm_x) {
         ARTSS_HIP(hipMalloc(&m_warm_x, sizeof(float) * 2 * v));
@@ -258,6 +281,12 @@ void Solver::warm_begin() {
         kk::warm_apply(s, L, m_warm_sum, fluid, m_warm_y, m_p_prev, f.p.data);
     } else if (m_warm.mode == 1) {
         kk::warm_apply(s, L, m_warm_sum, fluid, nullptr, m_p_prev, f.p.data);
+    } else if (m_warm.mode == 3) {
+        m_proj_fresh = (m_proj_fresh + 1) % m_warm.depth;
+        m_proj_valid |= 1u << m_proj_fresh;
+        kk::proj_update(s, L, f.p.data, m_p_prev, f.rhs.data, m_proj, m_proj_fresh, m_proj_valid, m_proj_partial);
+        kk::proj_solve(s, m_proj_partial, kk::blocks_inner(L), m_proj_fresh, m_proj_valid, m_proj_G, m_proj_alpha);
+        kk::proj_apply(s, L, m_proj, m_proj_fresh, m_proj_valid, m_proj_alpha, f.p.data);
     }
 }
 
@@ -680,6 +709,9 @@ double Solver::restore(int slot) {
         m_warm_history = 0;  // no previous solution in the snapshot: the next solve starts plainly
     }
     m_warm_sample = false;
+    // the projection start keeps no increments across a restore: its next
+    // start uses the one increment the snapshot holds
+    proj_reset();
     ARTSS_HIP(hipStreamSynchronize(d.stream()));
     return snap.t;
 }
@@ -776,6 +808,7 @@ double Solver::load(const std::string &path) {
         ARTSS_HIP(hipStreamSynchronize(d.stream()));
     }
     m_warm_history = have_prev ? static_cast<int>(header[5]) : 0;
+    proj_reset();
     m_warm_sample = false;
     return times[0];
 }

@@ -81,6 +81,11 @@ struct PressureMode {
 //      to the extrapolation, r = rhs - lap(p_n + d) the residual of the
 //      extrapolated start, s = rms(d) over the fluid cells,
 //      c = 1 / (2/dx^2 + 2/dy^2 + 2/dz^2)
+//   3  p_n + sum_s alpha_s d_s over the last `depth` increments d_s of the
+//      solution (d_1 = d), the combination closest to the solution in the
+//      energy norm (Galerkin projection onto the previous steps' increments;
+//      alpha = (1, 0, ...) is the extrapolation, so its energy-norm error is
+//      never larger)
 // The iteration then runs to the same residual target from any start.
 // `record` keeps the last step's network input and target, (p_{n+1} - p_n -
 // d) / s, for training.
@@ -88,6 +93,7 @@ struct WarmStart {
     int mode = 0;
     UNet *net = nullptr;  // mode 2, not owned
     bool record = false;
+    int depth = 4;  // mode 3: increments kept (1 .. kk::kProjMax)
 };
 
 class Solver {
@@ -196,6 +202,17 @@ class Solver {
     bool m_warm_sample = false;
     double *m_p_prev = nullptr, *m_p_start = nullptr;
     double *m_warm_partial = nullptr, *m_warm_sum = nullptr;
+    // mode 3: increments and their images in a ring of slots, the Gram matrix
+    // of the images by slot, the coefficients, partial sums; the slot of the
+    // newest increment and the slots that hold one
+    kk::ProjRing m_proj{};
+    double *m_proj_G = nullptr, *m_proj_alpha = nullptr, *m_proj_partial = nullptr;
+    int m_proj_fresh = -1;
+    unsigned m_proj_valid = 0;
+    void proj_reset() {
+        m_proj_fresh = -1;
+        m_proj_valid = 0;
+    }
     float *m_warm_x = nullptr, *m_warm_y = nullptr, *m_warm_t = nullptr;
 };
 

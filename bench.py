@@ -13,16 +13,21 @@ equation is solved until ||rhs - lap p||_2 <= 1e-8 every step.
 Prints one JSON line.  `value` is cell-updates/s = fluid inner cells x K /
 wall time of K timed steps (after W untimed ones, from rest) of the default
 path, fields resident in HBM: FP64 GCR preconditioned by a mixed-precision
-V-cycle (FP32 smoothing, restriction and prolongation; FP64 coarse levels,
-residuals and norms), started from the pressure extrapolated in time.  The
-defaults (W = 10, K = 990) time steps 11..1000, the whole input.
+V-cycle (FP32 smoothing, restriction and prolongation; an exact FP64 coarse
+solve; FP64 residuals and norms), started from the pressure extrapolated in
+time (--start projection starts from the combination of the last --depth
+increments of the pressure closest to the solution in the energy norm).  The defaults (W = 10, K = 990) time steps 11..1000, the whole input.
 
 `legs` times every path over the same steps W+1..W+K, each from rest:
-  mixed_extrapolated  the default path (`value`)
+  mixed               the default path (`value`)
+  mixed_projection    the same from the projection start: fewer iterations,
+                      but the residual target then leaves larger smooth
+                      pressure errors (compare its field difference)
   mixed_learned       the same with the learned warm start: a BF16 U-Net
                       trained during the warm-up steps on the (input, target)
                       pairs the solver records, then used on every timed step
-  gcr_fp64            FP64 GCR with the FP64 V-cycle (the reference's)
+  gcr_fp64            FP64 GCR with the FP64 V-cycle (the reference's),
+                      the default path's start
   plain_fp64          the reference's V-cycles on their own, to the same residual
   reference_schedule  the plain FP64 path with the reference's own V-cycle
                       schedule (n_cycle per step, no residual target): the
@@ -68,10 +73,11 @@ BYTES_PER_CELL = {
 }
 WHOLE_ARRAY = {"gcr_update"}
 
-PATHS = {"mixed_extrapolated": "GCR in FP64 preconditioned by an FP32 V-cycle with FP64 coarse levels, "
-                               "extrapolated start",
+PATHS = {"mixed": "GCR in FP64 preconditioned by an FP32 V-cycle with an exact FP64 coarse solve, %s start",
+         "mixed_projection": "GCR in FP64 preconditioned by an FP32 V-cycle with an exact FP64 coarse solve, "
+                             "projection start",
          "mixed_learned": "the same, started from the extrapolation plus the trained U-Net's correction",
-         "gcr_fp64": "FP64 GCR preconditioned by the FP64 V-cycle, extrapolated start",
+         "gcr_fp64": "FP64 GCR preconditioned by the FP64 V-cycle, %s start",
          "plain_fp64": "FP64, V-cycles alone (the reference's iteration) to the residual target",
          "reference_schedule": "FP64, the reference's V-cycle schedule (n_cycle per step, no residual target)"}
 
@@ -83,9 +89,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default=os.path.join(ROOT, "artss_amd", "configs", "plume_256x128x128.xml"))
     ap.add_argument("--tolerance", type=float, default=1e-8, help="pressure residual target per step")
-    ap.add_argument("--path", choices=("mixed_extrapolated", "gcr_fp64", "plain_fp64"), default="mixed_extrapolated",
+    ap.add_argument("--path", choices=("mixed", "gcr_fp64", "plain_fp64"), default="mixed",
                     help="path of the timed region (`value`); the others are legs")
-    ap.add_argument("--legs", default="mixed_learned,gcr_fp64,plain_fp64,reference_schedule",
+    ap.add_argument("--legs", default="mixed_projection,mixed_learned,gcr_fp64,plain_fp64,reference_schedule",
                     help="paths timed beside the default one over the same steps (comma list; '' for none)")
     ap.add_argument("--profile-kernel", default="gcr_update@0",
                     help="kernel@level timed for the roofline (the largest HBM-bound kernel of the path)")
@@ -107,6 +113,10 @@ def parse():
                     help="n_relax of the preconditioner's coarse levels (single-workgroup kernel; 0: the XML's)")
     ap.add_argument("--tail-cells", type=int, default=4096,
                     help="levels of at most this many cells run in one single-workgroup kernel")
+    ap.add_argument("--start", choices=("extrapolation", "projection"), default="extrapolation",
+                    help="start of the GCR iteration: linear extrapolation in time, or the combination of the "
+                         "last --depth increments closest to the solution in the energy norm")
+    ap.add_argument("--depth", type=int, default=4, help="increments kept by the projection start")
     ap.add_argument("--train-steps", type=int, default=150, help="Adam steps of the learned leg's network")
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the ensemble measurement (uneven plume boxes; 0: skip)")
@@ -156,13 +166,18 @@ def cpu_baseline(solver, config, first_step, steps, inner):
             "host": cpu}
 
 
+def path_text(args, leg):
+    text = PATHS[leg]
+    return text % args.start if "%s" in text else text
+
+
 def make_solver(artss_amd, args, leg, net=None):
     solver = artss_amd.Solver(args.config, device=args.device)
     solver.set_mg_schedule({"plain": 0, "fused": 1, "graph": 2}[args.mg_schedule], args.tail_cells)
     if leg == "reference_schedule":
         return solver
     solver.set_pressure_mode(True, args.tolerance, 500)
-    method = {"mixed_extrapolated": 2, "mixed_learned": 2, "gcr_fp64": 1, "plain_fp64": 0}[leg]
+    method = {"mixed": 2, "mixed_projection": 2, "mixed_learned": 2, "gcr_fp64": 1, "plain_fp64": 0}[leg]
     solver.set_pressure_method(method, args.coarse_sweeps)
     if method != 0:
         solver.set_coarse_tolerance(args.coarse_tol)
@@ -172,7 +187,10 @@ def make_solver(artss_amd, args, leg, net=None):
         solver.set_coarse_solve(args.coarse == "direct")
         solver.set_mixed_tail(args.tail_sweeps)
     if method != 0:
-        solver.set_warm_start(1, record=leg == "mixed_learned")
+        if leg == "mixed_projection" or (args.start == "projection" and leg in ("mixed", "gcr_fp64")):
+            solver.set_warm_start(3, depth=args.depth)
+        else:
+            solver.set_warm_start(1, record=leg == "mixed_learned")
     return solver
 
 
@@ -276,7 +294,7 @@ def leg_run(artss_amd, args, leg, dt, inner):
     secs, cyc, dif, res = run.timed(args.steps)
     fields = {f: s.field(f) for f in FIELDS}
     out = {"value": inner * args.steps / secs, "unit": "cell-updates/s", "ms_per_step": secs * 1e3 / args.steps,
-           "steps": "%d..%d" % (first, run.step - 1), "path": PATHS[leg],
+           "steps": "%d..%d" % (first, run.step - 1), "path": path_text(args, leg),
            "mean_iterations": float(np.mean(cyc)), "max_iterations": int(np.max(cyc)),
            "max_final_residual": None if leg == "reference_schedule" else float(np.max(res)),
            "fallbacks": s.fallbacks()}
@@ -302,7 +320,10 @@ def ensemble_run(artss_amd, args):
             m.set_pressure_mode(True, args.tolerance, 500)
             m.set_pressure_method(2, args.coarse_sweeps)
             m.set_coarse_tolerance(args.coarse_tol)
-            m.set_warm_start(1)
+            if args.start == "projection":
+                m.set_warm_start(3, depth=args.depth)
+            else:
+                m.set_warm_start(1)
             members.append(m)
         cells = sum(m.domain.info(0)["inner_cells"] for m in members)
         ens = artss_amd.Ensemble(members, threads=threads)
@@ -378,7 +399,7 @@ def main():
 
     legs = {args.path: {"value": value, "unit": "cell-updates/s", "ms_per_step": secs * 1e3 / args.steps,
                         "steps": "%d..%d" % (first, first + args.steps - 1),
-                        "path": PATHS[args.path], "mean_iterations": float(np.mean(cycles)),
+                        "path": path_text(args, args.path), "mean_iterations": float(np.mean(cycles)),
                         "max_iterations": int(np.max(cycles)), "max_final_residual": float(np.max(residuals)),
                         "fallbacks": fallbacks}}
     leg_fields = {args.path: fields}
@@ -396,10 +417,11 @@ def main():
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (pressure preconditioner f32)" if args.path == "mixed_extrapolated" else "f64",
+        "dtype": "f64 (pressure preconditioner f32)" if args.path == "mixed" else "f64",
         "data": "synthetic: plume from rest, timed steps %d..%d" % (first, first + args.steps - 1),
         "config": {"workload": "plume_256x128x128", "grid": list(info["n"]), "inner_cells": inner,
-                   "path": PATHS[args.path], "mg_schedule": args.mg_schedule,
+                   "path": path_text(args, args.path), "mg_schedule": args.mg_schedule,
+                   "start": args.start, "depth": args.depth if args.start == "projection" else None,
                    "tail_cells": args.tail_cells, "sweeps": args.sweeps, "weight": args.weight,
                    "coarse": args.coarse, "tail_sweeps": args.tail_sweeps, "coarse_sweeps": args.coarse_sweeps,
                    "coarse_tol": args.coarse_tol, "tuned_cells_per_thread": tuned,

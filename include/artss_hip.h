@@ -236,6 +236,13 @@ int artss_tuned_cells_per_thread(long long level_size);
  * keeps each step's network input and its target (p_{n+1} - p_n - d) / s.
  * Mode 2 and `record` need inner box sides that are multiples of 8. */
 int artss_solver_set_warm_start(artss_solver *s, int mode, artss_unet *net, int record);
+/* The same for the starts that need no network, with `depth` for mode 3:
+ * p_n + sum_s alpha_s d_s over the last `depth` (1 .. 8) increments d_s of
+ * the solution, the combination closest to the solution in the energy norm
+ * (Galerkin projection onto the previous steps' increments; its error in
+ * that norm is never larger than that of mode 1, alpha = (1, 0, ...)).  No
+ * reference counterpart.  A restore or a load clears the kept increments. */
+int artss_solver_set_warm_start_depth(artss_solver *s, int mode, int depth);
 /* copies the last recorded sample to device buffers x [V][2] and t [V] (V
  * inner box cells, x fastest); *available = 0 when none is recorded yet */
 int artss_solver_warm_sample(artss_solver *s, float *x, float *t, int *available);

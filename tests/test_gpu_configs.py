@@ -94,7 +94,9 @@ def _plume_solver(method, warm=0, net=None, record=False):
         s.set_mixed_sweeps(1, 1, 0.8)
         s.set_coarse_solve(True)
         s.set_mixed_tail(3)
-    if warm or record:
+    if warm == 3:
+        s.set_warm_start(3)
+    elif warm or record:
         s.set_warm_start(warm, net, record=record)
     return s
 
@@ -121,9 +123,9 @@ def plain_256():
     return at25, at35
 
 
-@pytest.mark.parametrize("method", [2, 1], ids=["mixed", "gcr_fp64"])
-def test_c3_accelerated_paths_vs_plain_fp64(plain_256, method):
-    s = _plume_solver(method, warm=1)
+@pytest.mark.parametrize("method,warm", [(2, 1), (1, 1)], ids=["mixed", "gcr_fp64"])
+def test_c3_accelerated_paths_vs_plain_fp64(plain_256, method, warm):
+    s = _plume_solver(method, warm=warm)
     log = []
     _steps(s, 1, 25, log)
     errs = {f: _rel(s.field(f), plain_256[0][f]) for f in FIELDS}
@@ -132,17 +134,19 @@ def test_c3_accelerated_paths_vs_plain_fp64(plain_256, method):
         assert errs[f] <= 1e-6, (f, errs[f])
 
 
-def test_c3_whole_input_converges_every_step():
-    """All 1000 steps of the input on the default path: every pressure solve
+@pytest.mark.parametrize("warm", [1, 3], ids=["extrapolation", "projection"])
+def test_c3_whole_input_converges_every_step(warm):
+    """All 1000 steps of the input on the mixed path: every pressure solve
     meets ||rhs - lap p||_2 <= 1e-8 by GCR alone (no step needs the V-cycle
     fallback).  With the reference's absolute tolerance in the
     preconditioner's coarse solve, both GCR paths stalled near 4e-6 from
     step 709 on."""
-    s = _plume_solver(2, warm=1)
+    s = _plume_solver(2, warm=warm)
     log = []
     _steps(s, 1, 1000, log)
     assert s.fallbacks() == 0
-    print("C3 GCR iterations: mean %.1f, max %d, steps 700..710 %s" % (np.mean(log), max(log), log[699:710]))
+    print("C3 start %d GCR iterations: mean %.2f, max %d, steps 700..710 %s" % (warm, np.mean(log), max(log),
+                                                                                log[699:710]))
     s.close()
 
 
