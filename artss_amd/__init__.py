@@ -95,6 +95,10 @@ def lib():
         "artss_solver_write_dat": (C.c_int, [vp, C.c_char_p, ip]),
         "artss_solver_set_warm_start": (C.c_int, [vp, C.c_int, vp, C.c_int]),
         "artss_solver_set_warm_start_depth": (C.c_int, [vp, C.c_int, C.c_int]),
+        "artss_solver_set_step_graph": (C.c_int, [vp, C.c_int, C.c_int]),
+        "artss_solver_step_graph_stats": (C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                                    C.POINTER(C.c_longlong), C.POINTER(C.c_int),
+                                                    C.POINTER(C.c_char_p)]),
         "artss_solver_set_mixed_sweeps": (C.c_int, [vp, C.c_int, C.c_int, C.c_double]),
         "artss_solver_set_coarse_solve": (C.c_int, [vp, C.c_int]),
         "artss_solver_set_mixed_tail": (C.c_int, [vp, C.c_int]),
@@ -451,6 +455,19 @@ class Solver:
         mixed-precision V-cycle (residual target only); coarse_max_iter > 0 sets the
         sweeps of the GCR preconditioner's coarse solve (0: five)."""
         _check(self.L.artss_solver_set_pressure_method(self.h, int(method), int(coarse_max_iter)))
+
+    def set_step_graph(self, on=True, margin=2):
+        """Capture the whole step in one HIP graph after an eager step that records each
+        residual-target loop's iterations (budget: that count + margin); a replay that would
+        need more is rolled back and the step runs eagerly.  Fields are bit-identical."""
+        _check(self.L.artss_solver_set_step_graph(self.h, 1 if on else 0, int(margin)))
+
+    def step_graph_stats(self):
+        r, b, c, a, n = C.c_longlong(), C.c_longlong(), C.c_longlong(), C.c_int(), C.c_char_p()
+        _check(self.L.artss_solver_step_graph_stats(self.h, C.byref(r), C.byref(b), C.byref(c), C.byref(a),
+                                                    C.byref(n)))
+        return dict(replays=r.value, rollbacks=b.value, captures=c.value, active=bool(a.value),
+                    note=(n.value or b"").decode())
 
     def set_warm_start(self, mode, net=None, record=False, depth=None):
         """Start of the GCR pressure iteration: 0 the previous p, 1 linear extrapolation in

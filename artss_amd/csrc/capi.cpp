@@ -684,6 +684,28 @@ int artss_solver_set_warm_start(artss_solver *s, int mode, artss_unet *net, int 
     });
 }
 
+int artss_solver_set_step_graph(artss_solver *s, int on, int margin) {
+    return guard([&] {
+        require(s && margin >= -1000 && margin <= 1000, "bad argument");
+        s->solver->set_step_graph(on != 0, margin);
+        return ARTSS_OK;
+    });
+}
+
+int artss_solver_step_graph_stats(artss_solver *s, long long *replays, long long *rollbacks, long long *captures,
+                                  int *active, const char **note) {
+    return guard([&] {
+        require(s && replays && rollbacks && captures && active && note, "null argument");
+        const Solver::GraphStats &g = s->solver->graph_stats();
+        *replays = g.replays;
+        *rollbacks = g.rollbacks;
+        *captures = g.captures;
+        *active = g.active ? 1 : 0;
+        *note = g.note.c_str();
+        return ARTSS_OK;
+    });
+}
+
 int artss_solver_set_warm_start_depth(artss_solver *s, int mode, int depth) {
     return guard([&] {
         require(s, "null argument");
@@ -766,6 +788,7 @@ int artss_solver_set_mixed_sweeps(artss_solver *s, int pre, int post, double wei
 int artss_solver_set_mg_schedule(artss_solver *s, int mode, long long tail_cells) {
     return guard([&] {
         require(s && tail_cells >= 0 && mode >= 0 && mode <= 2, "bad argument");
+        s->solver->invalidate_step_graph();
         s->solver->pressure_solver().set_schedule(mode != 0, tail_cells);
         s->solver->pressure_solver().set_graphs(mode == 2);
         finish(s->solver->domain());

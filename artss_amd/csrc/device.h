@@ -70,6 +70,16 @@ struct DepGhost {
 
 // Everything a kernel needs to produce the values of an idempotent boundary
 // fill itself (faces of one wave, ghosts, ghosts depending on faces).
+// Outcome of one residual-target loop of a captured step, written on the
+// device: the iterations that ran, whether the loop ended as its eager form
+// would (ok), and the final residual
+struct LoopLog {
+    double value;
+    int count;
+    int ok;
+    double start;  // the residual at the loop's start, where it has one
+};
+
 struct BCView {
     const FaceEntry *faces;
     long long n_faces;

@@ -35,6 +35,30 @@ struct ObstacleLevel {
     std::vector<uint64_t> face[6];  // Obstacle::m_boundary per patch, sorted
 };
 
+// Step capture (Solver::set_step_graph).  While `mode` is kRecord, every
+// residual-target loop of a step appends the iterations it took to `counts`
+// (a loop that cannot be captured, or that ended other than by meeting its
+// test, sets `failed`); while it is kCapture, loop number `next` enqueues
+// budgets[next] iterations behind its device gate, writes its outcome to
+// log[next] and reads nothing back to the host.
+struct StepCapture {
+    enum { kOff = 0, kRecord = 1, kCapture = 2 };
+    static constexpr int kMaxLoops = 16;
+    int mode = kOff;
+    int next = 0;
+    bool failed = false;
+    std::vector<int> counts, budgets;
+    std::vector<int> kinds;  // per loop: kDiffusion or kPressure
+    enum { kDiffusion = 0, kPressure = 1 };
+    LoopLog *log = nullptr;  // device, kMaxLoops entries, owned by the solver
+};
+
+// thrown by a loop asked to capture a form it has no capturable variant of
+class capture_unsupported : public std::runtime_error {
+  public:
+    using std::runtime_error::runtime_error;
+};
+
 class Domain {
   public:
     // device < 0: host-only (geometry and index lists; no stream, no device memory)
@@ -118,6 +142,7 @@ class Domain {
     using EventPairs = std::vector<std::pair<hipEvent_t, hipEvent_t>>;
     void timer_arm(const std::string &tag);
     const std::string &timer_tag() const { return m_timer_tag; }
+    StepCapture &capture() { return m_capture; }
     template <typename F>
     void timed(const char *kernel, int level, F &&launch) {
         if (m_timer_tag.empty() || m_timer_tag != std::string(kernel) + "@" + std::to_string(level)) {
@@ -170,6 +195,7 @@ class Domain {
     FaceParam *m_params_dev = nullptr;
 
     std::string m_timer_tag;
+    StepCapture m_capture;
     EventPairs m_timer_events;
     long long m_timer_count = 0;
     double m_timer_ms = 0.0;

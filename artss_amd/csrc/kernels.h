@@ -31,6 +31,9 @@ void fill32(hipStream_t s, float *dst, float v, long long n);
 void scale(hipStream_t s, double *dst, double v, long long n);           // dst *= v
 void add(hipStream_t s, double *dst, const double *src, long long n);    // dst += src
 void scale_to(hipStream_t s, double *dst, const double *src, double v, long long n);  // dst = src * v
+void scale_to_dev(hipStream_t s, double *dst, const double *src, const double *v, long long n);  // dst = src * *v
+// dst[q] := src[q] for q < count arrays of n values; the pointer arrays are in device memory
+void copy_many(hipStream_t s, double *const *dst, const double *const *src, int count, long long n);
 // copy on the domain list (inner + ghost shell, obstacle cells untouched), JacobiDiffuse.cpp:85-99
 void copy_domain(hipStream_t s, const LevelDesc &L, double *dst, const double *src);
 
@@ -145,6 +148,12 @@ struct IterGate {
     int done;      // 1: the loop would have stopped
 };
 void gate_open(hipStream_t s, IterGate *g, double tol, int max_iter);
+// log := the gate's count, done and sqrt(sum)
+void log_gate(hipStream_t s, const IterGate *g, LoopLog *log);
+// the end of an out-of-place loop of g->count iterations alternating between
+// out (odd iterations write it) and in: after none, out := in; after an even
+// count the arrays' contents are exchanged, so out holds the last iterate
+void alternate_finish(hipStream_t s, const IterGate *g, double *out, double *in, long long n);
 // reduce_sum into g->sum, then count the iteration and set done as the
 // loop `while (sqrt(sum) > tol && count < max_iter)` decides; nothing once done
 void reduce_gate(hipStream_t s, const double *partial, int n, IterGate *g);

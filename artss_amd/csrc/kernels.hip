@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <algorithm>
 #include <array>
 #include <atomic>
 #include <map>
@@ -150,6 +151,16 @@ __global__ void k_add(double *__restrict__ d, const double *__restrict__ s, long
 }
 __global__ void k_scale_to(double *__restrict__ d, const double *__restrict__ s, double v, long long n) {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i] * v;
+}
+__global__ void k_copy_many(double *const *__restrict__ dst, const double *const *__restrict__ src, long long n) {
+    double *d = dst[blockIdx.y];
+    const double *s = src[blockIdx.y];
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i];
+}
+__global__ void k_scale_to_dev(double *__restrict__ d, const double *__restrict__ s, const double *__restrict__ v,
+                               long long n) {
+    const double f = *v;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) d[i] = s[i] * f;
 }
 
 // copy over inner + ghost shell, leaving obstacle cells alone
@@ -1118,6 +1129,24 @@ __global__ void k_reduce_gate(const double *__restrict__ partial, int n, IterGat
         g->done = (sqrt(r) > g->tol && c < g->max_iter) ? 0 : 1;
     }
 }
+__global__ void k_log_gate(const IterGate *__restrict__ g, LoopLog *__restrict__ log) {
+    if (threadIdx.x == 0) {
+        log->value = sqrt(g->sum);
+        log->count = g->count;
+        log->ok = g->done;
+        log->start = 0.0;
+    }
+}
+__global__ void k_alternate_finish(const IterGate *__restrict__ g, double *__restrict__ out,
+                                   double *__restrict__ in, long long n) {
+    const int c = g->count;
+    if (c > 0 && (c & 1)) return;  // uniform over the grid
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+        const double x = out[i], y = in[i];
+        out[i] = y;
+        if (c != 0) in[i] = x;
+    }
+}
 __global__ void k_gate_open_iter(IterGate *__restrict__ g, double tol, int max_iter) {
     if (threadIdx.x == 0) {
         g->sum = 1.0;
@@ -1246,6 +1275,14 @@ void add(hipStream_t s, double *dst, const double *src, long long n) {
 }
 void scale_to(hipStream_t s, double *dst, const double *src, double v, long long n) {
     k_scale_to<<<grid1d(n, 256), 256, 0, s>>>(dst, src, v, n);
+}
+void copy_many(hipStream_t s, double *const *dst, const double *const *src, int count, long long n) {
+    if (count <= 0) return;
+    const unsigned g = std::min(grid1d(n, 256), 2048u);
+    k_copy_many<<<dim3(g, static_cast<unsigned>(count)), 256, 0, s>>>(dst, src, n);
+}
+void scale_to_dev(hipStream_t s, double *dst, const double *src, const double *v, long long n) {
+    k_scale_to_dev<<<grid1d(n, 256), 256, 0, s>>>(dst, src, v, n);
 }
 void copy_domain(hipStream_t s, const LevelDesc &L, double *dst, const double *src) {
     k_copy_domain<<<full_grid(L), block(), 0, s>>>(L, dst, src);
@@ -1500,6 +1537,10 @@ void reduce_gate(hipStream_t s, const double *partial, int n, IterGate *g) {
     k_reduce_gate<<<1, 1024, 0, s>>>(partial, n, g);
 }
 void gate_open(hipStream_t s, IterGate *g, double tol, int max_iter) { k_gate_open_iter<<<1, 64, 0, s>>>(g, tol, max_iter); }
+void log_gate(hipStream_t s, const IterGate *g, LoopLog *log) { k_log_gate<<<1, 64, 0, s>>>(g, log); }
+void alternate_finish(hipStream_t s, const IterGate *g, double *out, double *in, long long n) {
+    k_alternate_finish<<<grid1d(n, 256), 256, 0, s>>>(g, out, in, n);
+}
 void reduce_sum(hipStream_t s, const double *partial, int n, double *out) {
     k_reduce_sum<<<1, 1024, 0, s>>>(partial, n, out);
 }

@@ -133,6 +133,26 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
     hipStream_t s = d.stream();
     // convergence is rr <= tol2 on the host and in the gate alike
     const double tol2 = tol * tol;
+    StepCapture &cap = d.capture();
+    if (cap.mode == StepCapture::kCapture) {
+        // a fixed budget of iterations behind the gate, then the true
+        // residual; the outcome stays on the device
+        const int slot = cap.next++;
+        if (slot >= StepCapture::kMaxLoops || slot >= static_cast<int>(cap.budgets.size()))
+            throw capture_unsupported("more residual-target loops than the recorded step had");
+        d.apply_boundary(p.data, P, 0);
+        kk::residual_vec(s, L, m_r.data, p.data, b.data, m_partial);
+        if (m_mixed) k::fill32(s, m_zp32[0], 0.0f, L.size);
+        else m_zp0.set_value(0.0);
+        kk::gate_start(s, m_partial, kk::blocks_inner(L), m_gate, tol2);
+        const int budget = std::min(cap.budgets[static_cast<size_t>(slot)], max_iter);
+        for (int q = 0; q < budget; q++) iteration(p, q & 1);
+        d.apply_boundary(p.data, P, 0);
+        k::residual_partials(s, L, p.data, b.data, d.partial_buffer());
+        kk::gate_confirm(s, d.partial_buffer(), k::partial_count(L), m_gate, tol, cap.log + slot);
+        *residual = 0.0;
+        return 0;
+    }
     auto restart = [&]() {
         d.apply_boundary(p.data, P, 0);
         kk::residual_vec(s, L, m_r.data, p.data, b.data, m_partial);
@@ -157,6 +177,7 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
     // the summation order; at least one iteration follows, so every restart
     // costs an iteration and the loop ends at max_iter
     bool restarted = false;
+    int restarts = 0;
     while (true) {
         if (rr <= tol2 && !restarted) {
             const double t = true_residual(p, b);
@@ -164,6 +185,12 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
                 *residual = t;
                 m_converged = true;
                 m_predicted = std::max(0, it - 1);
+                if (cap.mode == StepCapture::kRecord) {
+                    // a solve that restarted cannot be replayed as one budget
+                    if (restarts > 0) cap.failed = true;
+                    cap.counts.push_back(it);
+                    cap.kinds.push_back(StepCapture::kPressure);
+                }
                 return it;
             }
             // the recursive residual drifted below the true one: restart
@@ -171,6 +198,7 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
             rr = restart();
             it_base = it;
             restarted = true;
+            restarts++;
             continue;
         }
         if (it >= max_iter) break;
@@ -194,6 +222,7 @@ int GcrPressure::solve(Field &p, const Field &b, double tol, int max_iter, doubl
     *residual = true_residual(p, b);
     m_converged = *residual <= tol;
     if (m_converged) m_predicted = std::max(0, it - 1);
+    if (cap.mode == StepCapture::kRecord) cap.failed = true;  // ended by the iteration limit or a stall
     return it;
 }
 

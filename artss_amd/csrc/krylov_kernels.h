@@ -17,6 +17,7 @@ struct GcrGate {
     double tol2;  // tolerance squared
     int count;    // iterations that ran since gate_open
     int done;     // 1 once rr <= tol2
+    double rr0;   // (r, r) at gate_start
 };
 
 int blocks_inner(const LevelDesc &L);  // partial slots per sum of the inner-grid kernels
@@ -44,6 +45,11 @@ void sum(hipStream_t s, const double *partial, int n, double *out);
 void sum_gate(hipStream_t s, const double *partial, int n, GcrGate *gate);
 // rr = 0, count = 0, done = 0, tol2 = tol2
 void gate_open(hipStream_t s, GcrGate *gate, double tol2);
+// rr = the fixed-order sum of n partials (as sum), count = 0, done = rr <= tol2
+void gate_start(hipStream_t s, const double *partial, int n, GcrGate *gate, double tol2);
+// log: the gate's count, t = sqrt of the sum of n partials (in the order of
+// k::reduce_sum), ok = the gate closed and t <= tol
+void gate_confirm(hipStream_t s, const double *partial, int n, const GcrGate *gate, double tol, LoopLog *log);
 
 }  // namespace kk
 }  // namespace artss

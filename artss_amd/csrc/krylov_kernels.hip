@@ -243,9 +243,52 @@ __global__ void k_sum_gate(const double *__restrict__ partial, int n, GcrGate *_
     }
 }
 
+// the start of a captured solve: (r, r) summed as k_sum does, the gate
+// opened and closed at once when it already meets the tolerance
+__global__ void k_gate_start(const double *__restrict__ partial, int n, GcrGate *__restrict__ gate, double tol2) {
+    __shared__ double s[16];
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += partial[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < 16; w++) r += s[w];
+        gate->rr = r;
+        gate->rr0 = r;
+        gate->tol2 = tol2;
+        gate->count = 0;
+        gate->done = r <= tol2 ? 1 : 0;
+    }
+}
+// the end of a captured solve: the true residual's partials (the
+// multigrid's residual_partials) summed as k::reduce_sum does; the solve
+// ended as its eager form would when the gate closed and sqrt of that sum
+// meets tol
+__global__ void k_gate_confirm(const double *__restrict__ partial, int n, const GcrGate *__restrict__ gate,
+                               double tol, LoopLog *__restrict__ log) {
+    __shared__ double s[16];
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += partial[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < 16; w++) r += s[w];
+        const double t = sqrt(r);
+        log->value = t;
+        log->count = gate->count;
+        log->ok = gate->done && t <= tol ? 1 : 0;
+        log->start = sqrt(gate->rr0);
+    }
+}
+
 __global__ void k_gate_open(GcrGate *__restrict__ gate, double tol2) {
     if (threadIdx.x == 0) {
         gate->rr = 0.0;
+        gate->rr0 = 0.0;
         gate->tol2 = tol2;
         gate->count = 0;
         gate->done = 0;
@@ -501,6 +544,12 @@ void sum_gate(hipStream_t s, const double *partial, int n, GcrGate *gate) {
     k_sum_gate<<<1, 1024, 0, s>>>(partial, n, gate);
 }
 void gate_open(hipStream_t s, GcrGate *gate, double tol2) { k_gate_open<<<1, 64, 0, s>>>(gate, tol2); }
+void gate_start(hipStream_t s, const double *partial, int n, GcrGate *gate, double tol2) {
+    k_gate_start<<<1, 1024, 0, s>>>(partial, n, gate, tol2);
+}
+void gate_confirm(hipStream_t s, const double *partial, int n, const GcrGate *gate, double tol, LoopLog *log) {
+    k_gate_confirm<<<1, 1024, 0, s>>>(partial, n, gate, tol, log);
+}
 
 }  // namespace kk
 }  // namespace artss

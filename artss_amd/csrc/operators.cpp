@@ -64,6 +64,9 @@ void JacobiDiffuse::run(Field &out, const Field &in, const Field &b, double D, c
     const double ax = D * dt * rdx * rdx, ay = D * dt * rdy * rdy, az = D * dt * rdz * rdz;
     const double rb = (1. + 2. * (ax + ay + az));
     const int n_part = k::partial_count(L);
+    StepCapture &cap = dm.capture();
+    if (cap.mode == StepCapture::kCapture) throw capture_unsupported("the Jacobi diffusion loop");
+    if (cap.mode == StepCapture::kRecord) cap.failed = true;
     int it = 0;
     double res = 1.;
     while (res > m_settings.tol_res && it < m_settings.max_iter) {
@@ -140,6 +143,10 @@ void ColoredGaussSeidelDiffuse::run(Field &out, const Field &in, const Field &b,
             return;
         }
     }
+    StepCapture &cap = dm.capture();
+    if (cap.mode == StepCapture::kCapture)
+        throw capture_unsupported("the colored Gauss-Seidel loop with a fill that is not idempotent");
+    if (cap.mode == StepCapture::kRecord) cap.failed = true;
     const int n_part = k::partial_count(L);
     out.copy_data(in);
     k::gate_open(s, m_gate, m_settings.tol_res, m_settings.max_iter);
@@ -183,6 +190,23 @@ void ColoredGaussSeidelDiffuse::run_one_pass(Field &out, const Field &in, const 
     const int *stop = &m_gate->done;
     const int n_part = k::cgs_iter_partials(L);
     k::gate_open(s, m_gate, m_settings.tol_res, m_settings.max_iter);
+    StepCapture &cap = dm.capture();
+    if (cap.mode == StepCapture::kCapture) {
+        // a fixed budget behind the gate; the arrays' final exchange and the
+        // outcome are settled on the device
+        const int slot = cap.next++;
+        if (slot >= StepCapture::kMaxLoops || slot >= static_cast<int>(cap.budgets.size()))
+            throw capture_unsupported("more residual-target loops than the recorded step had");
+        for (int q = 0; q < cap.budgets[static_cast<size_t>(slot)]; q++) {
+            const bool odd = (q & 1) == 0;
+            k::cgs_iter(s, L, V, c, ev, b.data, odd ? in.data : out.data, odd ? out.data : in.data,
+                        dm.partial_buffer(), stop);
+            k::reduce_gate(s, dm.partial_buffer(), n_part, m_gate);
+        }
+        k::alternate_finish(s, m_gate, out.data, in.data, out.get_size());
+        k::log_gate(s, m_gate, cap.log + slot);
+        return;
+    }
     int launched = 0;
     while (true) {
         const int batch = launched < m_batch_hint ? m_batch_hint - launched : 1;
@@ -205,6 +229,10 @@ void ColoredGaussSeidelDiffuse::run_one_pass(Field &out, const Field &in, const 
     m_batch_hint = m_iterations;
     if (m_iterations == 0) out.copy_data(in);
     else if (m_iterations % 2 == 0) k::swap_arrays(s, out.data, in.data, out.get_size());
+    if (cap.mode == StepCapture::kRecord) {
+        cap.counts.push_back(m_iterations);
+        cap.kinds.push_back(StepCapture::kDiffusion);
+    }
 }
 
 ColoredGaussSeidelDiffuse::~ColoredGaussSeidelDiffuse() {
@@ -264,12 +292,29 @@ GaussFunction::GaussFunction(Domain &d, const GaussSettings &g)
     ARTSS_HIP(hipMemcpyAsync(m_spatial.data, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice,
                              d.stream()));
     ARTSS_HIP(hipStreamSynchronize(d.stream()));
+    ARTSS_HIP(hipHostMalloc(&m_tv_host, sizeof(double)));
+    ARTSS_HIP(hipMalloc(&m_tv_dev, sizeof(double)));
+    *m_tv_host = 0.0;
 }
+
+GaussFunction::~GaussFunction() {
+    (void)hipStreamSynchronize(m_domain.stream());
+    (void)hipHostFree(m_tv_host);
+    (void)hipFree(m_tv_dev);
+}
+
+void GaussFunction::set_time(double t_cur) { *m_tv_host = std::tanh(t_cur / m_settings.tau); }
 
 // GaussFunction.cpp:27-40, 121-123: out = spatial * tanh(t / tau)
 void GaussFunction::update_source(Field &out, double t_cur) {
+    hipStream_t s = m_domain.stream();
+    if (m_domain.capture().mode == StepCapture::kCapture) {
+        ARTSS_HIP(hipMemcpyAsync(m_tv_dev, m_tv_host, sizeof(double), hipMemcpyHostToDevice, s));
+        k::scale_to_dev(s, out.data, m_spatial.data, m_tv_dev, out.get_size());
+        return;
+    }
     const double tv = std::tanh(t_cur / m_settings.tau);
-    k::scale_to(m_domain.stream(), out.data, m_spatial.data, tv, out.get_size());
+    k::scale_to(s, out.data, m_spatial.data, tv, out.get_size());
 }
 
 BuoyancyMMS::BuoyancyMMS(Domain &d) : m_domain(d), m_source(d, RHO) {
@@ -299,6 +344,8 @@ BuoyancyMMS::BuoyancyMMS(Domain &d) : m_domain(d), m_source(d, RHO) {
 
 // BuoyancyMMS.cpp:58-61: out = source * exp(-t) over the whole array
 void BuoyancyMMS::update_source(Field &out, double t_cur) {
+    if (m_domain.capture().mode == StepCapture::kCapture)
+        throw capture_unsupported("the manufactured buoyancy source (its factor exp(-t) is a kernel argument)");
     k::scale_to(m_domain.stream(), out.data, m_source.data, std::exp(-t_cur), out.get_size());
 }
 

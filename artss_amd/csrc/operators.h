@@ -162,13 +162,21 @@ class ExplicitEulerSource : public ISource {
 class GaussFunction {
   public:
     GaussFunction(Domain &d, const GaussSettings &g);
+    ~GaussFunction();
+    GaussFunction(const GaussFunction &) = delete;
+    GaussFunction &operator=(const GaussFunction &) = delete;
     void update_source(Field &out, double t_cur);
     const Field &spatial() const { return m_spatial; }
+    // a captured step reads the time factor tanh(t / tau) from device memory,
+    // copied there from pinned memory that set_time writes before each replay
+    void set_time(double t_cur);
 
   private:
     Domain &m_domain;
     GaussSettings m_settings;
     Field m_spatial;
+    double *m_tv_host = nullptr;  // pinned
+    double *m_tv_dev = nullptr;
 };
 
 // source/BuoyancyMMS.cpp:11-69 (temp_fct="Buoyancy", SolverSelection.cpp:148-149):
@@ -178,6 +186,7 @@ class GaussFunction {
 class BuoyancyMMS {
   public:
     explicit BuoyancyMMS(Domain &d);
+    // not capturable (its factor exp(-t) is a kernel argument)
     void update_source(Field &out, double t_cur);
 
   private:

@@ -1,5 +1,8 @@
 #include "solver.h"
 
+#include <mutex>
+#include <shared_mutex>
+
 #include <fstream>
 #include <utility>
 
@@ -189,6 +192,12 @@ Solver::Solver(std::unique_ptr<Domain> domain) : m_domain(std::move(domain)) {
 
 Solver::~Solver() {
     if (m_domain) (void)hipStreamSynchronize(m_domain->stream());
+    if (m_step_exec) (void)hipGraphExecDestroy(m_step_exec);
+    for (double *q : m_rollback) (void)hipFree(q);
+    for (void *q : {static_cast<void *>(m_loop_log), static_cast<void *>(m_state_dev),
+                    static_cast<void *>(m_rollback_dev)})
+        if (q) (void)hipFree(q);
+    if (m_loop_log_host) (void)hipHostFree(m_loop_log_host);
     for (auto &snap : m_snapshots)
         if (snap)
             for (double *q : snap->data) (void)hipFree(q);
@@ -205,6 +214,7 @@ Solver::~Solver() {
 }
 
 void Solver::set_warm_start(const WarmStart &w) {
+    drop_step_graph();
     if (w.mode < 0 || w.mode > 3) throw config_error("warm start mode must be 0, 1, 2 or 3");
     if (w.mode == 3 && (w.depth < 1 || w.depth > kk::kProjMax))
         throw config_error("the projection start keeps 1 to " + std::to_string(kk::kProjMax) + " increments");
@@ -267,6 +277,8 @@ void Solver::warm_begin() {
     FieldController &f = *m_fields;
     const LevelDesc &L = d.level(0);
     hipStream_t s = d.stream();
+    if (d.capture().mode == StepCapture::kCapture && (m_warm.mode >= 2 || m_warm.record))
+        throw capture_unsupported("the learned and projection starts and recording");
     k::copy(s, m_p_start, f.p.data, L.size);
     if (m_warm_history == 0) return;
     const double fluid = static_cast<double>(d.inner_cells(0));
@@ -300,8 +312,14 @@ void Solver::warm_end() {
                         m_p_start, m_p_prev, m_warm_t);
         m_warm_sample = true;
     }
-    std::swap(m_p_prev, m_p_start);
-    m_warm_history++;
+    if (m_graph_on) {
+        // a captured step keeps its buffers: copy instead of exchanging
+        k::copy(d.stream(), m_p_prev, m_p_start, L.size);
+    } else {
+        std::swap(m_p_prev, m_p_start);
+    }
+    // a captured step counts its solve when a replay succeeds
+    if (d.capture().mode != StepCapture::kCapture) m_warm_history++;
 }
 
 // SolverController.cpp:26-43, 95-205
@@ -372,6 +390,9 @@ void Solver::update_sources(double t) {
 void Solver::pressure_step(double t) {
     FieldController &f = *m_fields;
     m_counters.converged = true;
+    const bool capturing = m_domain->capture().mode == StepCapture::kCapture;
+    if (capturing && !(m_pmode.to_tolerance && (m_pmode.method == 1 || m_pmode.method == 2)))
+        throw capture_unsupported("a pressure solve other than the residual-target GCR");
     if (!m_pmode.to_tolerance) {
         m_pres->pressure(f.p, f.rhs, t, false);
         m_counters.vcycles = m_pres->last_cycles();
@@ -406,7 +427,7 @@ void Solver::pressure_step(double t) {
         m_counters.residual = r;
         m_counters.initial_residual = m_gcr->initial_residual();
         m_counters.converged = m_gcr->converged();
-        if (!m_counters.converged && m_counters.vcycles < m_pmode.max_cycles) {
+        if (!capturing && !m_counters.converged && m_counters.vcycles < m_pmode.max_cycles) {
             // GCR stalled: the reference's V-cycle iteration finishes the
             // solve from where it stopped, within the same cycle budget
             m_fallbacks++;
@@ -634,7 +655,168 @@ void Solver::do_step_transport() {
     }
 }
 
+// A stream capture fails when another thread meanwhile makes a call that
+// synchronises with the whole device (a synchronous copy from pageable host
+// memory, a free): eager steps, which make such calls when they set things
+// up, hold this lock shared, a capture holds it exclusively.  Replays make
+// no such call and take no lock.
+static std::shared_mutex g_capture_lock;
+
 void Solver::step(double t) {
+    Domain &d = *m_domain;
+    if (!m_graph_on || !m_graph_stats.note.empty() || !d.timer_tag().empty()) {
+        std::shared_lock<std::shared_mutex> g(g_capture_lock);
+        step_eager(t);
+        return;
+    }
+    if (m_step_exec) {
+        if (step_replay(t)) return;
+        drop_step_graph();  // rolled back: the step runs eagerly and is captured again
+    }
+    StepCapture &cap = d.capture();
+    cap = StepCapture{};
+    cap.mode = StepCapture::kRecord;
+    try {
+        std::shared_lock<std::shared_mutex> g(g_capture_lock);
+        step_eager(t);
+    } catch (...) {
+        cap.mode = StepCapture::kOff;
+        throw;
+    }
+    cap.mode = StepCapture::kOff;
+    if (cap.failed || cap.counts.empty() || cap.counts.size() > static_cast<size_t>(StepCapture::kMaxLoops))
+        return;  // capture after a step whose loops all ended by their tests
+    std::unique_lock<std::shared_mutex> g(g_capture_lock);
+    step_capture();
+}
+
+void Solver::set_step_graph(bool on, int margin) {
+    drop_step_graph();
+    m_graph_on = on;
+    m_graph_margin = margin;
+    m_graph_stats = GraphStats{};
+    if (!on || m_loop_log) return;
+    Domain &d = *m_domain;
+    ARTSS_HIP(hipMalloc(&m_loop_log, sizeof(LoopLog) * StepCapture::kMaxLoops));
+    ARTSS_HIP(hipHostMalloc(&m_loop_log_host, sizeof(LoopLog) * StepCapture::kMaxLoops));
+    const size_t n = static_cast<size_t>(d.level(0).size);
+    auto fields = m_fields->all();
+    std::vector<double *> state;
+    for (auto &kv : fields) state.push_back(kv.second->data);
+    // the warm start's previous solution (kept in place while a step is captured)
+    if (!m_p_prev) {
+        const LevelDesc &L = d.level(0);
+        ARTSS_HIP(hipMalloc(&m_p_prev, sizeof(double) * n));
+        ARTSS_HIP(hipMalloc(&m_p_start, sizeof(double) * n));
+        ARTSS_HIP(hipMalloc(&m_warm_partial, sizeof(double) * static_cast<size_t>(kk::blocks_inner(L))));
+        ARTSS_HIP(hipMalloc(&m_warm_sum, sizeof(double)));
+    }
+    state.push_back(m_p_prev);
+    for (size_t q = 0; q < state.size(); q++) {
+        double *c = nullptr;
+        ARTSS_HIP(hipMalloc(&c, sizeof(double) * n));
+        m_rollback.push_back(c);
+    }
+    m_state_count = static_cast<int>(state.size());
+    ARTSS_HIP(hipMalloc(&m_state_dev, sizeof(double *) * state.size()));
+    ARTSS_HIP(hipMalloc(&m_rollback_dev, sizeof(double *) * state.size()));
+    ARTSS_HIP(hipMemcpy(m_state_dev, state.data(), sizeof(double *) * state.size(), hipMemcpyHostToDevice));
+    ARTSS_HIP(hipMemcpy(m_rollback_dev, m_rollback.data(), sizeof(double *) * state.size(), hipMemcpyHostToDevice));
+}
+
+void Solver::drop_step_graph() {
+    if (m_step_exec) {
+        (void)hipStreamSynchronize(m_domain->stream());
+        (void)hipGraphExecDestroy(m_step_exec);
+        m_step_exec = nullptr;
+    }
+    m_graph_stats.active = false;
+}
+
+// records the step launched by the next replays: the fields copied aside,
+// then the step with each residual-target loop on a budget
+void Solver::step_capture() {
+    Domain &d = *m_domain;
+    StepCapture &cap = d.capture();
+    std::vector<int> budgets = cap.counts;
+    for (int &b : budgets) b = std::max(0, b + m_graph_margin);
+    m_loop_kind = cap.kinds;
+    const size_t n = static_cast<size_t>(d.level(0).size);
+    cap.mode = StepCapture::kCapture;
+    cap.next = 0;
+    cap.budgets = budgets;
+    cap.log = m_loop_log;
+    hipGraph_t graph = nullptr;
+    // the pass that records the graph runs the step's host code: the
+    // counters of the step just taken stay
+    const StepCounters taken = m_counters;
+    ARTSS_HIP(hipStreamBeginCapture(d.stream(), hipStreamCaptureModeRelaxed));
+    try {
+        k::copy_many(d.stream(), m_rollback_dev, m_state_dev, m_state_count, static_cast<long long>(n));
+        step_eager(0.0);  // time enters the captured sources through their device factors
+        m_counters = taken;
+    } catch (const std::exception &e) {
+        m_counters = taken;
+        (void)hipStreamEndCapture(d.stream(), &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        cap.mode = StepCapture::kOff;
+        m_graph_stats.note = std::string("not captured: ") + e.what();
+        return;
+    }
+    cap.mode = StepCapture::kOff;
+    ARTSS_HIP(hipStreamEndCapture(d.stream(), &graph));
+    if (cap.next != static_cast<int>(budgets.size())) {
+        (void)hipGraphDestroy(graph);
+        m_graph_stats.note = "not captured: the captured step has other loops than the recorded one";
+        return;
+    }
+    const hipError_t e = hipGraphInstantiate(&m_step_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    ARTSS_HIP(e);
+    m_graph_stats.captures++;
+    m_graph_stats.active = true;
+}
+
+// one replay; false (with the fields rolled back) when a loop would have
+// needed more than its budget
+bool Solver::step_replay(double t) {
+    Domain &d = *m_domain;
+    hipStream_t s = d.stream();
+    if (m_gauss) m_gauss->set_time(t);
+    if (m_gauss_con) m_gauss_con->set_time(t);
+    const size_t loops = m_loop_kind.size();
+    ARTSS_HIP(hipGraphLaunch(m_step_exec, s));
+    ARTSS_HIP(hipMemcpyAsync(m_loop_log_host, m_loop_log, sizeof(LoopLog) * loops, hipMemcpyDeviceToHost, s));
+    ARTSS_HIP(hipStreamSynchronize(s));
+    bool ok = true;
+    for (size_t q = 0; q < loops; q++) ok = ok && m_loop_log_host[q].ok != 0;
+    if (!ok) {
+        const size_t n = static_cast<size_t>(d.level(0).size);
+        k::copy_many(s, m_state_dev, m_rollback_dev, m_state_count, static_cast<long long>(n));
+        ARTSS_HIP(hipStreamSynchronize(s));
+        m_graph_stats.rollbacks++;
+        return false;
+    }
+    m_counters = StepCounters{};
+    int dif = 0;
+    for (size_t q = 0; q < loops; q++) {
+        const LoopLog &l = m_loop_log_host[q];
+        if (m_loop_kind[q] == StepCapture::kPressure) {
+            m_counters.vcycles = l.count;
+            m_counters.residual = l.value;
+            m_counters.initial_residual = l.start;
+            m_counters.converged = true;
+        } else if (dif < 4) {
+            m_counters.diffusion[dif++] = l.count;
+        }
+    }
+    if (m_warm.mode != 0) m_warm_history++;
+    m_graph_stats.replays++;
+    return true;
+}
+
+void Solver::step_eager(double t) {
     m_counters = StepCounters{};
     switch (m_kind) {
         case 0: do_step_ns(t); break;
@@ -710,8 +892,10 @@ double Solver::restore(int slot) {
     }
     m_warm_sample = false;
     // the projection start keeps no increments across a restore: its next
-    // start uses the one increment the snapshot holds
+    // start uses the one increment the snapshot holds; a captured step is
+    // recorded again
     proj_reset();
+    drop_step_graph();
     ARTSS_HIP(hipStreamSynchronize(d.stream()));
     return snap.t;
 }
@@ -809,6 +993,7 @@ double Solver::load(const std::string &path) {
     }
     m_warm_history = have_prev ? static_cast<int>(header[5]) : 0;
     proj_reset();
+    drop_step_graph();
     m_warm_sample = false;
     return times[0];
 }

@@ -106,10 +106,36 @@ class Solver {
 
     // SolverController::solver_do_step + update_sources + FieldController::update_data
     void step(double t);
+
+    // ---- step graph (no reference counterpart).  After one eager step that
+    // records how many iterations each residual-target loop took, the whole
+    // step is captured in one HIP graph, with that count + margin iterations
+    // per loop behind the loop's device gate, and replayed: the host launches
+    // it and reads one outcome per loop.  The graph first copies the fields
+    // aside; a replay in which a loop would have needed more than its budget
+    // (or restarted, or stalled) is rolled back from that copy and the step
+    // runs eagerly, after which the graph is captured again.  The fields are
+    // those of the eager step bit for bit.  It needs the residual-target GCR
+    // pressure solve (methods 1, 2), warm start 0 or 1 without recording, and
+    // loops with a capturable form (colored Gauss-Seidel with an idempotent
+    // fill); other configurations step eagerly and graph_stats().note says
+    // why.  While a kernel timer is armed steps run eagerly.
+    void set_step_graph(bool on, int margin);
+    struct GraphStats {
+        long long replays = 0, rollbacks = 0, captures = 0;
+        bool active = false;  // a captured graph is in use
+        std::string note;     // why the step is not captured, if it is not
+    };
+    const GraphStats &graph_stats() const { return m_graph_stats; }
+    // a change of the pressure solver's launch schedule makes a captured step stale
+    void invalidate_step_graph() { drop_step_graph(); }
     const StepCounters &counters() const { return m_counters; }
     // steps whose GCR pressure solve stalled and was finished by V-cycles
     long long fallbacks() const { return m_fallbacks; }
-    void set_pressure_mode(const PressureMode &m) { m_pmode = m; }
+    void set_pressure_mode(const PressureMode &m) {
+        m_pmode = m;
+        drop_step_graph();
+    }
     const PressureMode &pressure_mode() const { return m_pmode; }
     VCycleMG &pressure_solver() { return *m_pres; }
 
@@ -151,6 +177,22 @@ class Solver {
     std::vector<std::unique_ptr<Snapshot>> m_snapshots;
     void warm_begin();
     void warm_end();
+    void step_eager(double t);
+    bool step_replay(double t);
+    void step_capture();
+    void drop_step_graph();
+    bool m_graph_on = false;
+    int m_graph_margin = 2;
+    hipGraphExec_t m_step_exec = nullptr;
+    LoopLog *m_loop_log = nullptr;       // device
+    LoopLog *m_loop_log_host = nullptr;  // pinned
+    std::vector<int> m_loop_kind;        // StepCapture kinds of the captured loops
+    // the fields (and the warm start's previous solution) and their copies,
+    // as device arrays of pointers for one copy launch each way
+    std::vector<double *> m_rollback;
+    double **m_state_dev = nullptr, **m_rollback_dev = nullptr;
+    int m_state_count = 0;
+    GraphStats m_graph_stats;
     void do_step_ns(double t);
     void do_step_nstempturb(double t);
     void do_step_nstemp(double t);

@@ -122,6 +122,9 @@ def parse():
                     help="members of the ensemble measurement (uneven plume boxes; 0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=50, help="timed steps of the ensemble measurement")
     ap.add_argument("--ensemble-threads", type=int, default=8, help="host threads issuing the members' steps")
+    ap.add_argument("--ensemble-graph", type=int, default=1,
+                    help="also time the members stepping through captured graphs with this margin of "
+                         "iterations per loop (`concurrent_graph`; a negative value: not timed)")
     ap.add_argument("--device", type=int, default=0)
     return ap.parse_args()
 
@@ -311,8 +314,12 @@ def ensemble_run(artss_amd, args):
     another, from the same state."""
     from artss_amd import scenarios
     xmls = scenarios.ensemble_members(args.ensemble_members, seed=2026)
-    out = {"members": args.ensemble_members, "threads": args.ensemble_threads, "steps": args.ensemble_steps}
-    for label, threads in (("concurrent", args.ensemble_threads), ("sequential", 1)):
+    out = {"members": args.ensemble_members, "threads": args.ensemble_threads, "steps": args.ensemble_steps,
+           "step_graph_margin": args.ensemble_graph}
+    runs = [("concurrent", args.ensemble_threads, None), ("sequential", 1, None)]
+    if args.ensemble_graph is not None:
+        runs.append(("concurrent_graph", args.ensemble_threads, args.ensemble_graph))
+    for label, threads, graph in runs:
         members = []
         for x in xmls:
             m = artss_amd.Solver(x, device=args.device)
@@ -324,6 +331,8 @@ def ensemble_run(artss_amd, args):
                 m.set_warm_start(3, depth=args.depth)
             else:
                 m.set_warm_start(1)
+            if graph is not None:
+                m.set_step_graph(True, graph)
             members.append(m)
         cells = sum(m.domain.info(0)["inner_cells"] for m in members)
         ens = artss_amd.Ensemble(members, threads=threads)
@@ -336,6 +345,12 @@ def ensemble_run(artss_amd, args):
         out["cells"] = cells
         out[label] = {"value": cells * args.ensemble_steps / secs, "unit": "cell-updates/s",
                       "ms_per_step": secs * 1e3 / args.ensemble_steps}
+        if graph is not None:
+            st = [m.step_graph_stats() for m in members]
+            out[label]["step_graph"] = {"replays": sum(x["replays"] for x in st),
+                                        "rollbacks": sum(x["rollbacks"] for x in st),
+                                        "captures": sum(x["captures"] for x in st),
+                                        "notes": sorted({x["note"] for x in st if x["note"]})}
         ens.close()
         for m in members:
             m.close()
@@ -345,6 +360,8 @@ def ensemble_run(artss_amd, args):
 
 def main():
     args = parse()
+    if args.ensemble_graph is not None and args.ensemble_graph < 0:
+        args.ensemble_graph = None
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path (the step would shard "
               "as z-slabs with halo exchanges; not implemented)", file=sys.stderr)

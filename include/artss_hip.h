@@ -243,6 +243,23 @@ int artss_solver_set_warm_start(artss_solver *s, int mode, artss_unet *net, int 
  * that norm is never larger than that of mode 1, alpha = (1, 0, ...)).  No
  * reference counterpart.  A restore or a load clears the kept increments. */
 int artss_solver_set_warm_start_depth(artss_solver *s, int mode, int depth);
+
+/* Step graph (no reference counterpart; for ensembles of small domains,
+ * where a step's launches and host round trips outweigh its kernels).
+ * After one eager step that records how many iterations each residual-target
+ * loop took, artss_solver_step captures the whole step in one HIP graph
+ * (budget: that count + `margin` iterations per loop, behind the loop's
+ * device test) and replays it, reading one outcome per loop.  The graph first
+ * copies the fields aside; a replay in which a loop would have needed more
+ * than its budget is rolled back and the step runs eagerly and is captured
+ * again.  The fields are the eager step's bit for bit.  Needs the
+ * residual-target GCR pressure solve, warm start 0 or 1 without recording,
+ * and colored Gauss-Seidel diffusion with an idempotent fill; otherwise the
+ * steps run eagerly and `note` (valid until the next call) says why. */
+int artss_solver_set_step_graph(artss_solver *s, int on, int margin);  /* margin < 0: budgets below the
+                                                                         recorded counts (rollback tests) */
+int artss_solver_step_graph_stats(artss_solver *s, long long *replays, long long *rollbacks, long long *captures,
+                                  int *active, const char **note);
 /* copies the last recorded sample to device buffers x [V][2] and t [V] (V
  * inner box cells, x fastest); *available = 0 when none is recorded yet */
 int artss_solver_warm_sample(artss_solver *s, float *x, float *t, int *available);

@@ -41,3 +41,69 @@ def test_ensemble_matches_members_run_alone(threads):
     sizes = [m.domain.info(0)["n"] for m in members]
     assert len(set(sizes)) > 1  # uneven members
     ens.close()
+
+
+def _steps(s, n0, n1):
+    for n in range(n0, n1 + 1):
+        s.step(n * 0.005)
+
+
+@pytest.mark.parametrize("margin", [2, -100], ids=["replayed", "rolled_back"])
+def test_step_graph_matches_eager_steps(margin):
+    """A member stepped through its captured graph ends every step with the
+    fields and counters of the eager step, whether the replays succeed
+    (margin 2) or every replay runs out of budget and is rolled back
+    (margin -100: no loop gets an iteration)."""
+    xml = scenarios.ensemble_members(1, seed=5, nx=(48, 48), ny=(32, 32), nz=(32, 32), steps=12)[0]
+    eager, graph = _member(xml), _member(xml)
+    graph.set_step_graph(True, margin)
+    for n in range(1, 13):
+        eager.step(n * 0.005)
+        graph.step(n * 0.005)
+        assert graph.counters() == eager.counters(), n
+        for f in FIELDS:
+            assert np.array_equal(graph.field(f), eager.field(f)), (n, f)
+    st = graph.step_graph_stats()
+    print("margin", margin, st)
+    assert st["note"] == ""
+    assert st["captures"] >= 1
+    if margin > 0:
+        assert st["replays"] >= 9 and st["rollbacks"] <= 1
+    else:
+        assert st["replays"] == 0 and st["rollbacks"] >= 9
+    eager.close()
+    graph.close()
+
+
+def test_step_graph_ensemble_matches_members_run_alone():
+    xmls = scenarios.ensemble_members(4, seed=13, nx=(32, 64), ny=(32, 48), nz=(32, 48), steps=6)
+    alone = []
+    for x in xmls:
+        s = _member(x)
+        _steps(s, 1, 6)
+        alone.append({f: s.field(f) for f in FIELDS})
+        s.close()
+    members = [_member(x) for x in xmls]
+    for m in members:
+        m.set_step_graph(True, 2)
+    ens = artss_amd.Ensemble(members, threads=4)
+    for n in range(1, 7):
+        ens.step(n)
+    for m, fields in zip(members, alone):
+        for f in FIELDS:
+            assert np.array_equal(m.field(f), fields[f]), f
+        assert m.step_graph_stats()["replays"] >= 3
+    ens.close()
+
+
+def test_step_graph_declines_what_it_cannot_capture():
+    """The projection start keeps slot indices that change every step, so its
+    step is not captured: the solver says so and steps eagerly."""
+    xml = scenarios.ensemble_members(1, seed=5, nx=(32, 32), ny=(32, 32), nz=(32, 32), steps=4)[0]
+    s = _member(xml)
+    s.set_warm_start(3)
+    s.set_step_graph(True, 2)
+    _steps(s, 1, 4)
+    st = s.step_graph_stats()
+    assert "not captured" in st["note"] and st["replays"] == 0
+    s.close()
