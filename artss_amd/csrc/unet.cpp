@@ -17,6 +17,8 @@ struct Spec {
 const Spec kLayers[] = {{2, 16, 0},  {16, 16, 0}, {16, 32, 1}, {32, 64, 2}, {64, 64, 3},
                         {64, 64, 2}, {64, 32, 1}, {32, 16, 0}, {16, 1, 0}};
 enum { L_IN, L_E0, L_E1, L_E2, L_E3, L_D2, L_D1, L_D0, L_OUT, NLAYERS };
+// the layers that read a nearest-upsampled input
+inline bool upsampling(size_t layer) { return layer == L_D2 || layer == L_D1 || layer == L_D0; }
 
 template <typename T>
 T *dalloc(size_t n, std::vector<void *> *owner = nullptr) {
@@ -57,6 +59,7 @@ UNet::UNet(hipStream_t stream, unsigned long long seed) : m_s(stream) {
             L.W_bf = dalloc<uint16_t>(nw);
             L.Wt_bf = dalloc<uint16_t>(nw);
         }
+        if (upsampling(m_layers.size())) L.Wup_bf = dalloc<uint16_t>(64 * static_cast<size_t>(s.ci) * s.co);
         // He initialisation; the output layer starts at zero, so an untrained
         // network predicts no correction
         const bool out_layer = m_layers.size() + 1 == sizeof(kLayers) / sizeof(kLayers[0]);
@@ -65,6 +68,7 @@ UNet::UNet(hipStream_t stream, unsigned long long seed) : m_s(stream) {
         for (auto &x : host) x = static_cast<float>(std_ * nd(rng));
         ARTSS_HIP(hipMemcpy(L.W, host.data(), sizeof(float) * nw, hipMemcpyHostToDevice));
         if (L.W_bf) nn::weights_bf16(m_s, static_cast<int>(nw), s.ci, s.co, L.W, L.W_bf, L.Wt_bf);
+        if (L.Wup_bf) nn::weights_up_bf16(m_s, s.ci, s.co, L.W, L.Wup_bf);
         m_layers.push_back(L);
     }
     m_loss = dalloc<float>(1);
@@ -79,7 +83,7 @@ UNet::~UNet() {
         for (void *p : {static_cast<void *>(L.W), static_cast<void *>(L.b), static_cast<void *>(L.gW),
                         static_cast<void *>(L.gb), static_cast<void *>(L.mW), static_cast<void *>(L.vW),
                         static_cast<void *>(L.mb), static_cast<void *>(L.vb), static_cast<void *>(L.W_bf),
-                        static_cast<void *>(L.Wt_bf)})
+                        static_cast<void *>(L.Wt_bf), static_cast<void *>(L.Wup_bf)})
             if (p) (void)hipFree(p);
     (void)hipFree(m_loss);
     (void)hipHostFree(m_loss_host);
@@ -94,6 +98,7 @@ void UNet::set_params(const float *host) {
         ARTSS_HIP(hipMemcpy(L.b, host + off, sizeof(float) * L.co, hipMemcpyHostToDevice));
         off += L.co;
         if (L.W_bf) nn::weights_bf16(m_s, static_cast<int>(nw), L.ci, L.co, L.W, L.W_bf, L.Wt_bf);
+        if (L.Wup_bf) nn::weights_up_bf16(m_s, L.ci, L.co, L.W, L.Wup_bf);
         ARTSS_HIP(hipMemset(L.mW, 0, sizeof(float) * nw));
         ARTSS_HIP(hipMemset(L.vW, 0, sizeof(float) * nw));
         ARTSS_HIP(hipMemset(L.mb, 0, sizeof(float) * L.co));
@@ -174,9 +179,9 @@ void UNet::run_forward(const float *x) {
     nn::conv_fwd(m_s, 32, 64, false, D2, p2, L[L_E2].Wt_bf, L[L_E2].b, e2, nullptr, nullptr, true);
     nn::down(m_s, 64, D3, e2, p3);
     nn::conv_fwd(m_s, 64, 64, false, D3, p3, L[L_E3].Wt_bf, L[L_E3].b, e3, nullptr, nullptr, true);
-    nn::conv_fwd(m_s, 64, 64, true, D2, e3, L[L_D2].Wt_bf, L[L_D2].b, d2, c2, e2, true);
-    nn::conv_fwd(m_s, 64, 32, true, D1, d2, L[L_D1].Wt_bf, L[L_D1].b, d1, c1, e1, true);
-    nn::conv_fwd(m_s, 32, 16, true, D0, d1, L[L_D0].Wt_bf, L[L_D0].b, d0, c0, e0, true);
+    nn::conv_fwd(m_s, 64, 64, true, D2, e3, L[L_D2].Wup_bf, L[L_D2].b, d2, c2, e2, true);
+    nn::conv_fwd(m_s, 64, 32, true, D1, d2, L[L_D1].Wup_bf, L[L_D1].b, d1, c1, e1, true);
+    nn::conv_fwd(m_s, 32, 16, true, D0, d1, L[L_D0].Wup_bf, L[L_D0].b, d0, c0, e0, true);
     nn::conv_out(m_s, D0, d0, L[L_OUT].W, L[L_OUT].b, y);
     m_x = x;
 }
@@ -268,6 +273,7 @@ void UNet::adam_step(float lr) {
     for (Layer &l : m_layers) {
         const int nw = 27 * l.ci * l.co;
         nn::adam(m_s, nw, l.ci, l.co, l.W, l.gW, l.mW, l.vW, lr, b1, b2, eps, c1, c2, l.W_bf, l.Wt_bf);
+        if (l.Wup_bf) nn::weights_up_bf16(m_s, l.ci, l.co, l.W, l.Wup_bf);
         nn::adam(m_s, l.co, 1, l.co, l.b, l.gb, l.mb, l.vb, lr, b1, b2, eps, c1, c2, nullptr, nullptr);
     }
 }

@@ -53,6 +53,7 @@ class UNet {
         float *W = nullptr, *b = nullptr, *gW = nullptr, *gb = nullptr;
         float *mW = nullptr, *vW = nullptr, *mb = nullptr, *vb = nullptr;
         uint16_t *W_bf = nullptr, *Wt_bf = nullptr;  // [27][ci][co], [27][co][ci]
+        uint16_t *Wup_bf = nullptr;  // layers on an upsampled input: nn::weights_up_bf16's form
     };
     void ensure(int X, int Y, int Z);
     void run_forward(const float *x);

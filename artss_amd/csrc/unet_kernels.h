@@ -16,8 +16,9 @@ struct Dims {
 };
 
 // 3x3x3 convolution with ci, co in {16,32,64} (MFMA); up: the input is at half
-// resolution and read nearest-upsampled; pre (may be null): pre-activation;
-// skip (may be null): added after the activation
+// resolution and read nearest-upsampled, and Wt holds weights_up_bf16's
+// coarse-form weights instead of [27][co][ci]; pre (may be null):
+// pre-activation; skip (may be null): added after the activation
 void conv_fwd(hipStream_t s, int ci, int co, bool up, Dims D, const uint16_t *in, const uint16_t *Wt,
               const float *bias, uint16_t *out, uint16_t *pre, const uint16_t *skip, bool relu);
 // d in (at the resolution of D) from d out
@@ -43,6 +44,9 @@ void mse(hipStream_t s, long long n, const float *y, const float *target, float 
 void adam(hipStream_t s, int n, int ci, int co, float *w, const float *g, float *m, float *v, float lr, float b1,
           float b2, float eps, float c1, float c2, uint16_t *w_bf, uint16_t *wt_bf);
 void weights_bf16(hipStream_t s, int n, int ci, int co, const float *w, uint16_t *w_bf, uint16_t *wt_bf);
+// the [8 parities][8 taps][co][ci] BF16 weights conv_fwd takes for up = true:
+// per output parity, the fine taps that read the same coarse voxel summed
+void weights_up_bf16(hipStream_t s, int ci, int co, const float *w, uint16_t *wup);
 
 }  // namespace nn
 }  // namespace artss

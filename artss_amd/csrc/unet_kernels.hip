@@ -4,11 +4,14 @@
 // resolution of their level, without ghost cells.  Activations and weights
 // are BF16 (raw bit patterns, round-to-nearest-even), gradients FP32, and
 // every product accumulates in FP32.  The 3x3x3 convolutions with 16 or more
-// channels on both sides are implicit GEMMs on the gfx950 matrix cores
-// (v_mfma_f32_16x16x16_bf16): a wavefront owns 16 consecutive voxels along
-// x and all output channels; per tap and 16-channel chunk it loads one
-// 16x16 BF16 operand of the input (8 bytes per lane, contiguous in the
-// channel) and one of the weights per output tile.  Operand layouts:
+// channels on both sides are implicit GEMMs on the gfx950 matrix cores.
+// The forward pass (k_conv_fwd, the one the warm start runs every step) uses
+// v_mfma_f32_16x16x32_bf16 with a register tile per wavefront (see there);
+// the gradient kernels use v_mfma_f32_16x16x16_bf16: a wavefront owns 16
+// consecutive voxels along x and all output channels; per tap and 16-channel
+// chunk it loads one 16x16 BF16 operand of the input (8 bytes per lane,
+// contiguous in the channel) and one of the weights per output tile.
+// Operand layouts of the 16x16x16 form:
 //   A (16x16, rows m, cols k): lane l holds A[l%16][4*(l/16) .. +3]
 //   B (16x16, rows k, cols n): lane l holds B[4*(l/16) .. +3][l%16]
 //   D (16x16 FP32):            lane l holds D[4*(l/16) .. +3][l%16]
@@ -25,6 +28,7 @@ namespace nn {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float(static_cast<unsigned>(b) << 16); }
@@ -49,86 +53,192 @@ __device__ inline void tap_offset(int t, int &dx, int &dy, int &dz) {
 // out[v][co] = act(b[co] + sum_t sum_ci in[v + off_t][ci] * W[t][ci][co]) (+ skip[v][co])
 // Wt is [27][COUT][CIN].  UP: the input lives at half resolution and is read
 // nearest-upsampled.  pre (optional): the pre-activation values.
-template <int CIN, int COUT, bool UP>
+//
+// UP is evaluated on the coarse input directly: along each axis a fine
+// voxel of parity p reads coarse offsets floor((p + d) / 2) for its taps
+// d = -1, 0, 1, which are p - 1 and p, and a tap outside the fine volume is
+// exactly a coarse offset outside the coarse one.  So the 27-tap
+// convolution of the upsampled input equals, per parity (px, py, pz), a
+// 2x2x2 convolution of the coarse input with the weights of the fine taps
+// that meet on each coarse tap summed (weights_up_bf16: Wt is then
+// [8 parities][8 taps][COUT][CIN]).  A wavefront's voxel tiles each hold 16
+// voxels of one x parity (every other voxel), so the weight operand is the
+// same for all columns of a tile and the tile reads 16 consecutive coarse
+// voxels; y and z parities are those of the wavefront's row and plane.
+//
+// An implicit GEMM D[co][v] += sum_k A[co][k] B[k][v] over k = (tap, ci) on
+// v_mfma_f32_16x16x32_bf16.  A wavefront owns one row of NV x 16 consecutive
+// voxels along x and all COUT / 16 output-channel tiles, so per K step of 32
+// it loads COUT / 16 weight operands and NV input operands (16 bytes per
+// lane each, contiguous in the channel) and issues (COUT / 16) x NV matrix
+// instructions on them.  A K step is 32 channels of one tap (CIN >= 32) or
+// 16 channels of two consecutive taps (CIN = 16, the 28th tap reads zero).
+// Operand layouts (both operands give lane l the same eight k, so their
+// order within the lane does not change the sum):
+//   A (16x32): lane l holds row l%16, k = 8*(l/16) .. +7
+//   B (32x16): lane l holds column l%16, k = 8*(l/16) .. +7
+//   D (16x16 FP32): lane l holds D[4*(l/16) .. +3][l%16]
+// The epilogue stores the four channels a lane holds as one 8-byte write.
+template <int CIN, int COUT, bool UP, int NV>
 __global__ void __launch_bounds__(256) k_conv_fwd(Dims D, const uint16_t *__restrict__ in,
                                                   const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
                                                   uint16_t *__restrict__ out, uint16_t *__restrict__ pre,
                                                   const uint16_t *__restrict__ skip, int relu) {
+    static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channels come in tiles of 16");
+    static_assert(!UP || (CIN >= 32 && NV % 2 == 0), "the upsampling form takes 32-channel K steps, tile pairs");
+    constexpr int MT = COUT / 16;
+    constexpr int KSTEPS = UP ? 8 * (CIN / 32) : (CIN >= 32 ? 27 * (CIN / 32) : 14);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x0 = (blockIdx.x * 4 + wave) * 16;
-    const int y = blockIdx.y, z = blockIdx.z;
-    if (x0 >= D.X) return;  // no workgroup barrier in this kernel
-    const int vx = x0 + (lane & 15), kq = lane >> 4;
-    f32x4 acc[COUT / 16];
+    const int y = static_cast<int>(blockIdx.y) * 4 + wave, z = blockIdx.z;
+    const int x0 = static_cast<int>(blockIdx.x) * (16 * NV);
+    if (y >= D.Y) return;  // no workgroup barrier in this kernel
+    const int col = lane & 15, kg = lane >> 4;
+    const Dims C{D.X >> 1, D.Y >> 1, D.Z >> 1};
+    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+    f32x4 acc[MT][NV];
 #pragma unroll
-    for (int ct = 0; ct < COUT / 16; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int t = 0; t < 27; t++) {
-        int dx, dy, dz;
-        tap_offset(t, dx, dy, dz);
-        const int xx = vx + dx, yy = y + dy, zz = z + dz;
-        const bool inside = vx < D.X && xx >= 0 && xx < D.X && yy >= 0 && yy < D.Y && zz >= 0 && zz < D.Z;
-        long long src = 0;
-        if (inside) {
-            if (UP) {
-                const Dims C{D.X >> 1, D.Y >> 1, D.Z >> 1};
-                src = vox(C, xx >> 1, yy >> 1, zz >> 1);
-            } else {
-                src = vox(D, xx, yy, zz);
+    for (int ct = 0; ct < MT; ct++)
+#pragma unroll
+        for (int j = 0; j < NV; j++) acc[ct][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (UP) {
+        const int py = y & 1, pz = z & 1;
+        for (int st = 0; st < KSTEPS; st++) {
+            const int t8 = st / (CIN / 32), c = (st % (CIN / 32)) * 32 + 8 * kg;
+            const int ox = t8 & 1, oy = (t8 >> 1) & 1, oz = t8 >> 2;
+            bf16x8 a[2][MT], b[NV];
+#pragma unroll
+            for (int px = 0; px < 2; px++)
+#pragma unroll
+                for (int ct = 0; ct < MT; ct++)
+                    a[px][ct] = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const uint4 *>(
+                                    Wt + ((static_cast<long long>(((pz * 2 + py) * 2 + px) * 8 + t8)) * COUT + ct * 16 +
+                                          col) * CIN + c));
+            const int cy = (y >> 1) + py - 1 + oy, cz = (z >> 1) + pz - 1 + oz;
+            const bool row_ok = cy >= 0 && cy < C.Y && cz >= 0 && cz < C.Z;
+#pragma unroll
+            for (int j = 0; j < NV; j++) {
+                const int cx = x0 / 2 + 16 * (j / 2) + col + (j & 1) - 1 + ox;
+                b[j] = zero8;
+                if (row_ok && cx >= 0 && cx < C.X)
+                    b[j] = __builtin_bit_cast(bf16x8,
+                                              *reinterpret_cast<const uint4 *>(in + vox(C, cx, cy, cz) * CIN + c));
             }
-        }
 #pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 16) {
-            bf16x4 b = bf16x4{0, 0, 0, 0};
-            if (inside) b = *reinterpret_cast<const bf16x4 *>(in + src * CIN + c0 + 4 * kq);
+            for (int ct = 0; ct < MT; ct++)
 #pragma unroll
-            for (int ct = 0; ct < COUT / 16; ct++) {
-                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(
-                    Wt + (static_cast<long long>(t) * COUT + ct * 16 + (lane & 15)) * CIN + c0 + 4 * kq);
-                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
-            }
+                for (int j = 0; j < NV; j++)
+                    acc[ct][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j & 1][ct], b[j], acc[ct][j], 0, 0, 0);
         }
     }
-    if (vx >= D.X) return;
-    const long long v = vox(D, vx, y, z);
-#pragma unroll
-    for (int ct = 0; ct < COUT / 16; ct++)
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int co = ct * 16 + 4 * kq + e;
-            const float c = acc[ct][e] + bias[co];
-            if (pre) pre[v * COUT + co] = f2bf(c);
-            float r = relu ? fmaxf(c, 0.f) : c;
-            if (skip) r += bf2f(skip[v * COUT + co]);
-            out[v * COUT + co] = f2bf(r);
+    for (int st = 0; st < (UP ? 0 : KSTEPS); st++) {
+        // tap and first channel of this lane's eight k values
+        int t, c;
+        if (CIN >= 32) {
+            t = st / (CIN / 32);
+            c = (st % (CIN / 32)) * 32 + 8 * kg;
+        } else {
+            t = 2 * st + (kg >> 1);
+            c = 8 * (kg & 1);
         }
+        const bool tap_ok = t < 27;
+        int dx, dy, dz;
+        tap_offset(tap_ok ? t : 0, dx, dy, dz);
+        bf16x8 a[MT], b[NV];
+#pragma unroll
+        for (int ct = 0; ct < MT; ct++)
+            a[ct] = tap_ok ? __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(
+                                                            Wt + (static_cast<long long>(t) * COUT + ct * 16 + col) *
+                                                                     CIN + c))
+                           : zero8;
+        const int yy = y + dy, zz = z + dz;
+        const bool row_ok = tap_ok && yy >= 0 && yy < D.Y && zz >= 0 && zz < D.Z;
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            const int xx = x0 + 16 * j + col + dx;
+            b[j] = zero8;
+            if (row_ok && xx >= 0 && xx < D.X)
+                b[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(in + vox(D, xx, yy, zz) * CIN + c));
+        }
+#pragma unroll
+        for (int ct = 0; ct < MT; ct++)
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                acc[ct][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ct], b[j], acc[ct][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        const int vx = UP ? x0 + 32 * (j / 2) + 2 * col + (j & 1) : x0 + 16 * j + col;
+        if (vx >= D.X) continue;
+        const long long v = vox(D, vx, y, z);
+#pragma unroll
+        for (int ct = 0; ct < MT; ct++) {
+            const int co = ct * 16 + 4 * kg;
+            float c4[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) c4[e] = acc[ct][j][e] + bias[co + e];
+            if (pre) {
+                uint2 pk;
+                pk.x = static_cast<unsigned>(f2bf(c4[0])) | (static_cast<unsigned>(f2bf(c4[1])) << 16);
+                pk.y = static_cast<unsigned>(f2bf(c4[2])) | (static_cast<unsigned>(f2bf(c4[3])) << 16);
+                *reinterpret_cast<uint2 *>(pre + v * COUT + co) = pk;
+            }
+            float r[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) r[e] = relu ? fmaxf(c4[e], 0.f) : c4[e];
+            if (skip) {
+                const uint2 sk = *reinterpret_cast<const uint2 *>(skip + v * COUT + co);
+                r[0] += bf2f(static_cast<uint16_t>(sk.x & 0xFFFFu));
+                r[1] += bf2f(static_cast<uint16_t>(sk.x >> 16));
+                r[2] += bf2f(static_cast<uint16_t>(sk.y & 0xFFFFu));
+                r[3] += bf2f(static_cast<uint16_t>(sk.y >> 16));
+            }
+            uint2 o;
+            o.x = static_cast<unsigned>(f2bf(r[0])) | (static_cast<unsigned>(f2bf(r[1])) << 16);
+            o.y = static_cast<unsigned>(f2bf(r[2])) | (static_cast<unsigned>(f2bf(r[3])) << 16);
+            *reinterpret_cast<uint2 *>(out + v * COUT + co) = o;
+        }
+    }
 }
 
 // First layer: x is FP32 with CI (2) channels, 16 BF16 outputs with ReLU.
+// One voxel per thread; the 27 taps are unrolled with zero operands outside
+// the volume, so their loads are in flight together, and the 16 outputs
+// leave as two 16-byte stores.
 template <int CI, int CO>
 __global__ void k_conv_in(Dims D, const float *__restrict__ x, const float *__restrict__ W,
                           const float *__restrict__ bias, uint16_t *__restrict__ out) {
+    static_assert(CI == 2 && CO == 16, "two input channels, sixteen outputs");
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
     if (i >= D.X) return;
-    float acc[CO];
+    float2 xv[27];
 #pragma unroll
-    for (int c = 0; c < CO; c++) acc[c] = bias[c];
     for (int t = 0; t < 27; t++) {
         int dx, dy, dz;
         tap_offset(t, dx, dy, dz);
         const int xx = i + dx, yy = y + dy, zz = z + dz;
-        if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
-        const long long s = vox(D, xx, yy, zz);
-#pragma unroll
-        for (int ci = 0; ci < CI; ci++) {
-            const float a = x[s * CI + ci];
-#pragma unroll
-            for (int c = 0; c < CO; c++) acc[c] += a * W[(t * CI + ci) * CO + c];
-        }
+        const bool inside = xx >= 0 && xx < D.X && yy >= 0 && yy < D.Y && zz >= 0 && zz < D.Z;
+        xv[t] = inside ? *reinterpret_cast<const float2 *>(x + vox(D, xx, yy, zz) * CI) : make_float2(0.f, 0.f);
     }
-    const long long v = vox(D, i, y, z);
+    float acc[CO];
 #pragma unroll
-    for (int c = 0; c < CO; c++) out[v * CO + c] = f2bf(fmaxf(acc[c], 0.f));
+    for (int c = 0; c < CO; c++) acc[c] = bias[c];
+#pragma unroll
+    for (int t = 0; t < 27; t++)
+#pragma unroll
+        for (int c = 0; c < CO; c++) {
+            acc[c] += xv[t].x * W[(t * CI + 0) * CO + c];
+            acc[c] += xv[t].y * W[(t * CI + 1) * CO + c];
+        }
+    unsigned pk[CO / 2];
+#pragma unroll
+    for (int c = 0; c < CO; c += 2)
+        pk[c / 2] = static_cast<unsigned>(f2bf(fmaxf(acc[c], 0.f))) |
+                    (static_cast<unsigned>(f2bf(fmaxf(acc[c + 1], 0.f))) << 16);
+    uint4 *o = reinterpret_cast<uint4 *>(out + vox(D, i, y, z) * CO);
+    o[0] = uint4{pk[0], pk[1], pk[2], pk[3]};
+    o[1] = uint4{pk[4], pk[5], pk[6], pk[7]};
 }
 
 // Last layer: CI BF16 channels -> one FP32 output, no activation.
@@ -453,6 +563,30 @@ __global__ void k_weights_bf16(int n, int ci, int co, const float *__restrict__ 
     }
 }
 
+// BF16 weights of the upsampling convolution's coarse form (k_conv_fwd):
+// wup[par][t8][o][i] = sum of W[t][i][o] over the fine taps t that meet on
+// coarse tap t8 for output parity par (bits x, y, z from the lowest), summed
+// in FP32 in tap order, rounded once
+__global__ void k_weights_up_bf16(int ci, int co, const float *__restrict__ w, uint16_t *__restrict__ wup) {
+    const int n = 64 * ci * co;
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += 256 * gridDim.x) {
+        const int i = q % ci, o = (q / ci) % co, t8 = (q / (ci * co)) % 8, par = q / (8 * ci * co);
+        const int p[3] = {par & 1, (par >> 1) & 1, par >> 2}, oo[3] = {t8 & 1, (t8 >> 1) & 1, t8 >> 2};
+        float acc = 0.f;
+        for (int t = 0; t < 27; t++) {
+            int d[3];
+            tap_offset(t, d[0], d[1], d[2]);
+            bool hit = true;
+            for (int a = 0; a < 3; a++) {
+                const int c = (p[a] + d[a] + 2) / 2 - 1;  // floor((p + d) / 2)
+                hit = hit && c == p[a] - 1 + oo[a];
+            }
+            if (hit) acc += w[(t * ci + i) * co + o];
+        }
+        wup[q] = f2bf(acc);
+    }
+}
+
 __global__ void k_fill_f(long long n, float *__restrict__ d, float v) {
     for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += 256LL * gridDim.x) d[q] = v;
 }
@@ -469,12 +603,21 @@ inline unsigned blocks_for(long long n) {
 
 void conv_fwd(hipStream_t s, int cin, int cout, bool up, Dims D, const uint16_t *in, const uint16_t *Wt,
               const float *bias, uint16_t *out, uint16_t *pre, const uint16_t *skip, bool relu) {
-    const dim3 g((D.X + 63) / 64, D.Y, D.Z);
-#define CASE(CI, CO)                                                                                            \
-    if (cin == CI && cout == CO) {                                                                             \
-        if (up) k_conv_fwd<CI, CO, true><<<g, 256, 0, s>>>(D, in, Wt, bias, out, pre, skip, relu ? 1 : 0);    \
-        else k_conv_fwd<CI, CO, false><<<g, 256, 0, s>>>(D, in, Wt, bias, out, pre, skip, relu ? 1 : 0);      \
-        return;                                                                                                \
+    // voxels per wavefront row: 16 x NV with NV = 8, 4, 2 for 16, 32, 64
+    // output channels (8 matrix tiles per K step)
+#define CASE(CI, CO)                                                                                             \
+    if (cin == CI && cout == CO) {                                                                              \
+        constexpr int NV = CO == 16 ? 8 : (CO == 32 ? 4 : 2);                                                    \
+        const dim3 g((D.X + 16 * NV - 1) / (16 * NV), (D.Y + 3) / 4, D.Z);                                     \
+        if (!up) {                                                                                              \
+            k_conv_fwd<CI, CO, false, NV><<<g, 256, 0, s>>>(D, in, Wt, bias, out, pre, skip, relu ? 1 : 0);     \
+            return;                                                                                             \
+        }                                                                                                       \
+        if constexpr (CI >= 32) {                                                                               \
+            k_conv_fwd<CI, CO, true, NV><<<g, 256, 0, s>>>(D, in, Wt, bias, out, pre, skip, relu ? 1 : 0);      \
+            return;                                                                                             \
+        }                                                                                                       \
+        throw std::invalid_argument("conv_fwd: an upsampled input needs at least 32 channels");               \
     }
     ARTSS_CONV_CASES(CASE)
 #undef CASE
@@ -572,6 +715,10 @@ void adam(hipStream_t s, int n, int ci, int co, float *w, const float *g, float 
           float b2, float eps, float c1, float c2, uint16_t *w_bf, uint16_t *wt_bf) {
     k_adam<<<blocks_for(n), 256, 0, s>>>(n, ci, co, w, g, m, v, lr, b1, b2, eps, c1, c2, w_bf, wt_bf);
 }
+void weights_up_bf16(hipStream_t s, int ci, int co, const float *w, uint16_t *wup) {
+    k_weights_up_bf16<<<blocks_for(64LL * ci * co), 256, 0, s>>>(ci, co, w, wup);
+}
+
 void weights_bf16(hipStream_t s, int n, int ci, int co, const float *w, uint16_t *w_bf, uint16_t *wt_bf) {
     k_weights_bf16<<<blocks_for(n), 256, 0, s>>>(n, ci, co, w, w_bf, wt_bf);
 }

@@ -73,6 +73,22 @@ def test_forward_matches_reference(setup):
     assert err < 3e-2
 
 
+def test_forward_matches_reference_wide(setup):
+    """288 x 32 x 16 voxels: a wavefront row of the forward convolution
+    covers 128, 64 and 32 voxels at 16, 32 and 64 output channels, so every
+    level runs several workgroups along x with a partial last tile"""
+    dom, net, p, _, _ = setup
+    x = np.random.default_rng(11).standard_normal((16, 32, 288, 2))
+    xb = artss_amd.Buffer(dom, x.astype(np.float32))
+    yb = artss_amd.Buffer(dom, np.zeros((16, 32, 288, 1), np.float32))
+    net.forward(xb, yb)
+    y = yb.download().astype(np.float64)
+    y_ref, _ = U.forward(p, x)
+    err = _rel(y, y_ref)
+    print("wide forward relative L2 error", err)
+    assert err < 3e-2
+
+
 def test_gradients_match_reference(setup):
     dom, net, p, x, target = setup
     xb = artss_amd.Buffer(dom, x.astype(np.float32))
