@@ -167,7 +167,7 @@ void UNet::ensure(int X, int Y, int Z) {
     m_Z = Z;
 }
 
-void UNet::run_forward(const float *x) {
+void UNet::run_forward(const float *x, bool keep_pre) {
     const nn::Dims D0{m_X, m_Y, m_Z}, D1{m_X / 2, m_Y / 2, m_Z / 2}, D2{m_X / 4, m_Y / 4, m_Z / 4},
         D3{m_X / 8, m_Y / 8, m_Z / 8};
     const Layer *L = m_layers.data();
@@ -179,23 +179,23 @@ void UNet::run_forward(const float *x) {
     nn::conv_fwd(m_s, 32, 64, false, D2, p2, L[L_E2].Wt_bf, L[L_E2].b, e2, nullptr, nullptr, true);
     nn::down(m_s, 64, D3, e2, p3);
     nn::conv_fwd(m_s, 64, 64, false, D3, p3, L[L_E3].Wt_bf, L[L_E3].b, e3, nullptr, nullptr, true);
-    nn::conv_fwd(m_s, 64, 64, true, D2, e3, L[L_D2].Wup_bf, L[L_D2].b, d2, c2, e2, true);
-    nn::conv_fwd(m_s, 64, 32, true, D1, d2, L[L_D1].Wup_bf, L[L_D1].b, d1, c1, e1, true);
-    nn::conv_fwd(m_s, 32, 16, true, D0, d1, L[L_D0].Wup_bf, L[L_D0].b, d0, c0, e0, true);
+    nn::conv_fwd(m_s, 64, 64, true, D2, e3, L[L_D2].Wup_bf, L[L_D2].b, d2, keep_pre ? c2 : nullptr, e2, true);
+    nn::conv_fwd(m_s, 64, 32, true, D1, d2, L[L_D1].Wup_bf, L[L_D1].b, d1, keep_pre ? c1 : nullptr, e1, true);
+    nn::conv_fwd(m_s, 32, 16, true, D0, d1, L[L_D0].Wup_bf, L[L_D0].b, d0, keep_pre ? c0 : nullptr, e0, true);
     nn::conv_out(m_s, D0, d0, L[L_OUT].W, L[L_OUT].b, y);
     m_x = x;
 }
 
 void UNet::forward(int X, int Y, int Z, const float *x, float *out) {
     ensure(X, Y, Z);
-    run_forward(x);
+    run_forward(x, false);
     ARTSS_HIP(hipMemcpyAsync(out, y, sizeof(float) * static_cast<size_t>(X) * Y * Z, hipMemcpyDeviceToDevice, m_s));
     ARTSS_HIP(hipStreamSynchronize(m_s));
 }
 
 float UNet::gradients(int X, int Y, int Z, const float *x, const float *target, float *grads) {
     ensure(X, Y, Z);
-    run_forward(x);
+    run_forward(x, true);
     const nn::Dims D0{m_X, m_Y, m_Z}, D1{m_X / 2, m_Y / 2, m_Z / 2}, D2{m_X / 4, m_Y / 4, m_Z / 4},
         D3{m_X / 8, m_Y / 8, m_Z / 8};
     const long long V0 = static_cast<long long>(m_X) * m_Y * m_Z, V1 = V0 / 8, V2 = V1 / 8, V3 = V2 / 8;

@@ -56,7 +56,9 @@ class UNet {
         uint16_t *Wup_bf = nullptr;  // layers on an upsampled input: nn::weights_up_bf16's form
     };
     void ensure(int X, int Y, int Z);
-    void run_forward(const float *x);
+    // keep_pre: store the decoder's pre-activations, which only the backward
+    // pass of gradients() reads
+    void run_forward(const float *x, bool keep_pre);
     void free_buffers();
     hipStream_t m_s;
     std::vector<Layer> m_layers;

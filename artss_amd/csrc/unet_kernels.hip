@@ -241,24 +241,57 @@ __global__ void k_conv_in(Dims D, const float *__restrict__ x, const float *__re
     o[1] = uint4{pk[4], pk[5], pk[6], pk[7]};
 }
 
-// Last layer: CI BF16 channels -> one FP32 output, no activation.
-template <int CI>
+// Last layer: CI BF16 channels -> one FP32 output, no activation.  A thread
+// owns NZ voxels of one (x, y) column and marches through the input planes
+// z0-1 .. z0+NZ: each tap value it loads (16 bytes per eight channels) is
+// added, with the weight of its z offset, to each of the up to three
+// outputs whose stencil holds that plane.  Every output still receives its
+// terms in tap order (z offset, then y, x, channel), skipping taps outside
+// the volume, as a per-voxel loop over the 27 taps adds them.
+template <int CI, int NZ>
 __global__ void k_conv_out(Dims D, const uint16_t *__restrict__ in, const float *__restrict__ W,
                            const float *__restrict__ bias, float *__restrict__ y) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int yy0 = blockIdx.y, z = blockIdx.z;
-    if (i >= D.X) return;
-    float acc = bias[0];
-    for (int t = 0; t < 27; t++) {
-        int dx, dy, dz;
-        tap_offset(t, dx, dy, dz);
-        const int xx = i + dx, yy = yy0 + dy, zz = z + dz;
-        if (xx < 0 || xx >= D.X || yy < 0 || yy >= D.Y || zz < 0 || zz >= D.Z) continue;
-        const long long s = vox(D, xx, yy, zz);
+    static_assert(CI % 8 == 0, "channels in groups of eight");
+    const int i = static_cast<int>(blockIdx.x) * 64 + static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.y) * 4 + static_cast<int>(threadIdx.y);
+    const int z0 = static_cast<int>(blockIdx.z) * NZ;
+    if (i >= D.X || j >= D.Y) return;
+    float acc[NZ];
 #pragma unroll
-        for (int ci = 0; ci < CI; ci++) acc += bf2f(in[s * CI + ci]) * W[t * CI + ci];
+    for (int q = 0; q < NZ; q++) acc[q] = bias[0];
+#pragma unroll
+    for (int r = 0; r < NZ + 2; r++) {
+        const int zp = z0 - 1 + r;  // input plane; feeds outputs q = r (dz -1), r - 1 (dz 0), r - 2 (dz +1)
+        if (zp < 0 || zp >= D.Z) continue;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+            const int yy = j + dy;
+            if (yy < 0 || yy >= D.Y) continue;
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int xx = i + dx;
+                if (xx < 0 || xx >= D.X) continue;
+                const uint4 *src = reinterpret_cast<const uint4 *>(in + vox(D, xx, yy, zp) * CI);
+                const int tb = (dy + 1) * 3 + (dx + 1);  // tap within its z plane
+#pragma unroll
+                for (int g = 0; g < CI / 8; g++) {
+                    const uint4 qv = src[g];
+                    const unsigned w4[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const float v = bf2f(static_cast<uint16_t>((w4[e >> 1] >> (16 * (e & 1))) & 0xFFFFu));
+                        const int ci = 8 * g + e;
+                        if (r < NZ) acc[r] += v * W[tb * CI + ci];
+                        if (r >= 1 && r - 1 < NZ) acc[r - 1] += v * W[(9 + tb) * CI + ci];
+                        if (r >= 2) acc[r - 2] += v * W[(18 + tb) * CI + ci];
+                    }
+                }
+            }
+        }
     }
-    y[vox(D, i, yy0, z)] = acc;
+#pragma unroll
+    for (int q = 0; q < NZ; q++)
+        if (z0 + q < D.Z) y[vox(D, i, j, z0 + q)] = acc[q];
 }
 
 // 2x2x2 mean, D: the output (coarse) dims
@@ -656,7 +689,7 @@ void conv_in(hipStream_t s, Dims D, const float *x, const float *W, const float 
     k_conv_in<2, 16><<<dim3((D.X + 63) / 64, D.Y, D.Z), 64, 0, s>>>(D, x, W, bias, out);
 }
 void conv_out(hipStream_t s, Dims D, const uint16_t *in, const float *W, const float *bias, float *y) {
-    k_conv_out<16><<<dim3((D.X + 63) / 64, D.Y, D.Z), 64, 0, s>>>(D, in, W, bias, y);
+    k_conv_out<16, 8><<<dim3((D.X + 63) / 64, (D.Y + 3) / 4, (D.Z + 7) / 8), dim3(64, 4, 1), 0, s>>>(D, in, W, bias, y);
 }
 void conv_in_wgrad(hipStream_t s, Dims D, const float *x, const float *g, float *gW, float *gb) {
     const long long n = static_cast<long long>(D.X) * D.Y * D.Z;
